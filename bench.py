@@ -1,0 +1,203 @@
+"""Bandwidth benchmark of the K17 time step on one MI355X.
+
+Workload: Taylor-Green vortex, D3Q27 cumulant K17, 256^3 fully periodic, FP32 shifted
+populations and FP32 arithmetic, plain forward step (collide + esoteric-twist stream in
+place), u0 = 0.05, Re = 1600. A step is one launch of the dense-box K17 kernel over all
+16,777,216 nodes; inputs are resident in HBM before timing starts.
+
+Prints ONE JSON line: MLUPS, the kernel's HBM roofline (algorithmic bytes 2*27*4 = 216 B
+per node update, divided by the event-timed average launch duration), the FP16/BF16
+storage variants measured the same way, a measured copy bandwidth, and the CPU oracle
+timed on the host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 20]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, AMD public spec
+METRIC = "MLUPS + %HBM roofline, D3Q27 K17 256^3; FP16-store and adjoint vs plain FP32"
+
+
+def tgv_fields(grid, u0):
+    import numpy as np
+
+    x, y, z = grid.coords()
+    k = 2 * math.pi / grid.nx
+    ux = u0 * np.sin(k * x) * np.cos(k * y) * np.cos(k * z)
+    uy = -u0 * np.cos(k * x) * np.sin(k * y) * np.cos(k * z)
+    uz = np.zeros_like(ux)
+    drho = 3.0 * (u0 ** 2 / 16.0) * (np.cos(2 * k * x) + np.cos(2 * k * y)) * (np.cos(2 * k * z) + 2.0)
+    return drho, ux, uy, uz
+
+
+def time_steps(L, steps, warmup):
+    """Event-timed K17 launches on torch's current stream (the stream the kernel runs on)."""
+    import torch
+
+    for _ in range(warmup):
+        L.collide()
+        L.swap()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        L.collide()
+        L.swap()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / 1000.0
+
+
+def copy_bandwidth():
+    """Device-to-device copy of 2 GiB: (read + write bytes) / time."""
+    import torch
+
+    n = 1 << 29
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    torch.cuda.synchronize()
+    t = s.elapsed_time(e) / 1000.0 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 4 * n / t / 1e9
+
+
+def cpu_baseline(target_s=12.0):
+    """The FP64 CPU oracle (OpenMP over z-slabs) on a 128^3 Taylor-Green sample."""
+    import numpy as np
+
+    import oracle
+    from virtualfluids_amd.grid import periodic_box
+
+    n = 128
+    grid = periodic_box(n, n, n)
+    nu = 0.05 * (n / (2 * math.pi)) / 1600.0
+    omega = 1.0 / (3 * nu + 0.5)
+    L = oracle.Lattice(grid.nbx, grid.nby, grid.nbz)
+    L.init_eq(*tgv_fields(grid, 0.05))
+    even = True
+    for _ in range(2):
+        oracle.collide_box(L.f, n, n, n, omega, even, limiters=(1e6, 1e6, 1e6))
+        even = not even
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < target_s and steps < 400:
+        oracle.collide_box(L.f, n, n, n, omega, even, limiters=(1e6, 1e6, 1e6))
+        even = not even
+        steps += 1
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = min(cores, os.cpu_count() or cores)
+    return {"value": round(grid.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"FP64 CPU oracle, Taylor-Green 128^3 periodic, {steps} K17 steps ({dt:.1f} s)"}
+
+
+def traffic_from_profile():
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "k17_box_fp32_256_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--variant-steps", type=int, default=500, help="timed steps of the FP16/BF16 variants")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
+                    help="time one storage type only (for profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on exactly one GPU (one GPU owns the domain); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from virtualfluids_amd.grid import periodic_box
+    from virtualfluids_amd.lattice import Lattice
+
+    n = args.n
+    grid = periodic_box(n, n, n)
+    N = grid.N
+    u0 = 0.05
+    nu = u0 * (n / (2 * math.pi)) / 1600.0
+    omega = 1.0 / (3 * nu + 0.5)
+    fields = tgv_fields(grid, u0)
+    lim = (1e6, 1e6, 1e6)  # the Taylor-Green app switches the limiter off
+
+    results = {}
+    for storage in (["fp32", "fp16", "bf16"] if args.only is None else [args.only]):
+        L = Lattice(grid, omega, storage=storage, limiters=lim)
+        L.init_equilibrium(*fields)
+        steps = args.steps if storage == "fp32" else args.variant_steps
+        t = time_steps(L, steps, args.warmup)
+        bytes_per_node = 2 * 27 * L.f.element_size()
+        results[storage] = {"t": t, "steps": steps, "mlups": N * steps / t / 1e6,
+                            "ms": 1e3 * t / steps, "gbps": bytes_per_node * N * steps / t / 1e9,
+                            "bytes_per_node": bytes_per_node}
+        s = L.moment_sums()
+        results[storage]["mass_drift"] = float(s[0])
+        del L
+        torch.cuda.empty_cache()
+
+    main_key = "fp32" if "fp32" in results else args.only
+    r = results[main_key]
+    copy_gbps = copy_bandwidth()
+    traffic = traffic_from_profile() if main_key == "fp32" else None
+    line = {
+        "metric": METRIC,
+        "value": round(r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": r["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": round(r["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"Taylor-Green vortex D3Q27 K17 {n}^3 periodic, {main_key} storage, "
+                               f"plain forward (dense-box kernel)", "nodes": N, "u0": u0, "Re": 1600,
+                   "omega": round(omega, 6), "storage": main_key},
+        "roofline": {"bound": "hbm", "achieved": round(r["gbps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(r["gbps"] / HBM_PEAK_GBPS, 4), "traffic": traffic},
+        "storage_variants": {k: {"MLUPS": round(v["mlups"], 1), "GBps": round(v["gbps"], 1),
+                                 "ms_per_step": round(v["ms"], 4), "steps": v["steps"],
+                                 "vs_fp32": (round(v["mlups"] / results["fp32"]["mlups"], 3)
+                                             if "fp32" in results else None)}
+                             for k, v in results.items()},
+        "hbm_copy_GBps": round(copy_gbps, 1),
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,201 @@
+/*
+ * virtualfluids_amd — C ABI of the MI355X (gfx950) LBM time step.
+ *
+ * One GPU owns the whole domain. Every entry point launches asynchronously on the
+ * given HIP stream (pass the hipStream_t as `stream`; NULL is the null stream),
+ * never allocates device memory per call, and returns VF_OK or an error code;
+ * vf_get_last_error() then describes the failure. All pointers inside the
+ * structs are device pointers owned by the caller, unless marked "host".
+ *
+ * Each declaration names the reference interface it replaces (paths are
+ * relative to the VirtualFluids source tree).
+ */
+#ifndef VIRTUALFLUIDS_AMD_H
+#define VIRTUALFLUIDS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VF_ABI_VERSION 1
+
+enum vf_status {
+    VF_OK = 0,
+    VF_ERR_INVALID_ARGUMENT = 1, /* null pointer, size out of range, unknown enum value */
+    VF_ERR_HIP = 2,              /* the HIP runtime reported an error on launch */
+    VF_ERR_UNSUPPORTED = 3       /* valid request this build does not implement */
+};
+
+/* Storage type of the population buffer. Arithmetic is always FP32. */
+enum vf_storage {
+    VF_STORAGE_FP32 = 0,
+    VF_STORAGE_FP16 = 1, /* IEEE binary16 shifted populations f_i - w_i */
+    VF_STORAGE_BF16 = 2  /* bfloat16 shifted populations f_i - w_i */
+};
+
+/* CollisionTemplate (src/gpu/core/Calculation/Calculation.h:56-69) */
+enum vf_collision_template {
+    VF_COLLISION_DEFAULT = 0,
+    VF_COLLISION_WRITE_MACRO_VARS = 1,
+    VF_COLLISION_APPLY_BODY_FORCE = 2,
+    VF_COLLISION_ALL_FEATURES = 3,
+    VF_COLLISION_SUBDOMAIN_BORDER = 4
+};
+
+/* Node types (src/gpu/core/Calculation/Calculation.h:36-45) */
+#define VF_GEO_PM_0 5u
+#define VF_GEO_PM_1 6u
+#define VF_GEO_PM_2 7u
+#define VF_GEO_SOLID 15u
+#define VF_GEO_VOID 16u
+#define VF_GEO_FLUID 19u
+
+/*
+ * One grid level: the fields of LBMSimulationParameter the time step reads
+ * (src/gpu/core/Parameter/Parameter.h:67-256).
+ *
+ * Populations: ONE in-place buffer of 27*numberOfNodes values, structure of arrays
+ * f[d*numberOfNodes + k], d in the D3Q27 order of src/lbm/constants/D3Q27.h:46-72
+ * (src/gpu/core/Cuda/CudaMemoryManager.cpp:169). Esoteric-twist parity
+ * isEvenTimestep selects the slot table (src/gpu/core/Utilities/KernelUtilities.h:66-125).
+ *
+ * Addressing: neighborX/Y/Z hold the +x/+y/+z neighbour of every node
+ * (Parameter.h:84-86). When nx*ny*nz == numberOfNodes the node index is
+ * k = x + nx*(y + ny*z) of a periodic box and vf_k17_run_box() recomputes the
+ * neighbours instead of loading them.
+ */
+typedef struct vf_lbm_parameter {
+    void* distributions;            /* 27*numberOfNodes values of type `storage` */
+    int32_t storage;                /* enum vf_storage */
+    int32_t isEvenTimestep;         /* Parameter.h:71 */
+    uint64_t numberOfNodes;         /* includes any dummy node 0 of a sparse grid */
+    const uint32_t* neighborX;      /* +x neighbour, numberOfNodes entries */
+    const uint32_t* neighborY;      /* +y neighbour */
+    const uint32_t* neighborZ;      /* +z neighbour */
+    const uint32_t* typeOfGridNode; /* Parameter.h:79, VF_GEO_* */
+    int32_t nx, ny, nz;             /* dense periodic box extents (0 when sparse) */
+    const uint8_t* fluidMask;       /* box path only: collide where != 0 (NULL: all nodes) */
+    float omega;                    /* 1/(3 nu + 1/2) on this level (Parameter.cpp:400-418) */
+    float quadricLimiters[3];       /* host values P, M, D (Parameter.cpp:69, default 0.01) */
+    float forces[3];                /* host values, global body force (Parameter::getForcesDev) */
+    float forceFactor;              /* 2^-level (src/gpu/core/Utilities/KernelUtilities.h:56-64) */
+    float* bodyForceX;              /* per-node force, ApplyBodyForce templates (reset to 0 after use) */
+    float* bodyForceY;
+    float* bodyForceZ;
+    float* rho;                     /* drho = rho - 1 output / init input */
+    float* velocityX;
+    float* velocityY;
+    float* velocityZ;
+    float* pressure;
+    const uint32_t* fluidNodeIndices; /* taggedFluidNodeIndices[Default] (Parameter.h:245-256) */
+    uint32_t numberOfFluidNodes;
+} vf_lbm_parameter;
+
+/*
+ * QforBoundaryConditions / QforDirectionalBoundaryCondition
+ * (src/gpu/core/Calculation/Calculation.h:105-127).
+ * q27 is SoA: q27[d*numberOfBCnodes + i]; a link is active when 0 <= q <= 1.
+ */
+typedef struct vf_q_boundary_conditions {
+    const int32_t* k;         /* BC node indices */
+    const int32_t* kN;        /* pressure BC: neighbour node whose state is extrapolated */
+    const float* q27;         /* 27*numberOfBCnodes sub-grid distances */
+    uint32_t numberOfBCnodes;
+    const float* Vx;          /* velocity BCs: wall velocity per BC node */
+    const float* Vy;
+    const float* Vz;
+    const float* RhoBC;       /* pressure BC: prescribed drho per BC node */
+    int32_t direction;        /* pressure BC: D3Q27 index of the outflow side (dP00, dM00, ...) */
+} vf_q_boundary_conditions;
+
+/* ICells / ICellNeigh (src/gpu/core/Calculation/Calculation.h:72-89) */
+typedef struct vf_interpolation_cells {
+    const uint32_t* fineCellIndices;
+    const uint32_t* coarseCellIndices;
+    uint32_t numberOfCells;
+} vf_interpolation_cells;
+
+typedef struct vf_interpolation_cell_neighbor {
+    const float* x; /* per-cell offset of the interpolation point (refinement towards walls) */
+    const float* y;
+    const float* z;
+} vf_interpolation_cell_neighbor;
+
+/* ---------------------------------------------------------------------------------
+ * Library
+ * --------------------------------------------------------------------------------- */
+int vf_abi_version(void);
+const char* vf_get_last_error(void);
+
+/* ---------------------------------------------------------------------------------
+ * K17 collide + eso-twist stream
+ * --------------------------------------------------------------------------------- */
+
+/* K17CompressibleNavierStokes<None>::run() (src/gpu/core/Kernel/Compressible/NavierStokes/K17/K17CompressibleNavierStokes.cu:83-101):
+ * every node of fluidNodeIndices, CollisionTemplate::Default. */
+int vf_k17_run(const vf_lbm_parameter* para, void* stream);
+
+/* K17CompressibleNavierStokes<None>::runOnIndices(indices, n, template, stream)
+ * (K17CompressibleNavierStokes.cu:103-145; kernel body src/gpu/core/Utilities/RunCollision.cuh:74-143). */
+int vf_k17_run_on_indices(const vf_lbm_parameter* para, const uint32_t* indices, uint32_t numberOfIndices,
+                          int32_t collisionTemplate, void* stream);
+
+/* The same update on a dense periodic box (nx, ny, nz set): neighbours recomputed from
+ * the node coordinates, no index traffic. Replaces run()/runOnIndices() for uniform
+ * boxes; optional fluidMask skips solid nodes (NoSlipDelayBounceBack). */
+int vf_k17_run_box(const vf_lbm_parameter* para, int32_t collisionTemplate, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Boundary conditions: the free launchers BoundaryConditionFactory hands out
+ * (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.cpp:49-217).
+ * Post-collision kernels read with the current parity and write the flipped one.
+ * --------------------------------------------------------------------------------- */
+
+/* NoSlipBounceBack (src/gpu/core/BoundaryConditions/NoSlip/NoSlip.cu:44-60) */
+int vf_NoSlipBounceBack(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, void* stream);
+/* NoSlipInterpolatedCompressible (NoSlip.cu:81-98) */
+int vf_NoSlipInterpolatedCompressible(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, void* stream);
+/* VelocityBounceBack (src/gpu/core/BoundaryConditions/Velocity/Velocity.cu:44-63) */
+int vf_VelocityBounceBack(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, void* stream);
+/* VelocityInterpolatedCompressible (Velocity.cu:87-107) */
+int vf_VelocityInterpolatedCompressible(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, void* stream);
+/* VelocityWithPressureInterpolatedCompressible (Velocity.cu:109-129) */
+int vf_VelocityWithPressureInterpolatedCompressible(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc,
+                                                    void* stream);
+/* PressureNonEquilibriumCompressible, pre-collision, after the parity swap
+ * (src/gpu/core/BoundaryConditions/Pressure/Pressure.cu:63-81) */
+int vf_PressureNonEquilibriumCompressible(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc,
+                                          void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Initialisation, output, conservation
+ * --------------------------------------------------------------------------------- */
+
+/* InitNavierStokesCompressible::init (src/gpu/core/PreProcessor/PreProcessorStrategy/InitNavierStokesCompressible/InitNavierStokesCompressible.cu:45-87):
+ * shifted equilibrium from para->rho (drho) and para->velocity*, every node that is not
+ * VF_GEO_SOLID/VF_GEO_VOID (all nodes when typeOfGridNode is NULL). */
+int vf_InitNavierStokesCompressible(const vf_lbm_parameter* para, void* stream);
+
+/* calculateMacroscopicQuantitiesCompressible (src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237, 953):
+ * fills para->rho, velocity*, pressure from the pre-collision state. */
+int vf_CalculateMacroscopicQuantitiesCompressible(const vf_lbm_parameter* para, void* stream);
+
+/* FP64 conservation sums over the pre-collision state of every node with
+ * typeOfGridNode == VF_GEO_FLUID (all nodes when NULL):
+ * out4[0] = sum drho, out4[1..3] = sum of momentum j. out4 is a device pointer to 4 doubles,
+ * overwritten. No counterpart in the reference (new: mass/momentum checks). */
+int vf_ConservationSums(const vf_lbm_parameter* para, double* out4, void* stream);
+
+/* One FP64 pass over the same nodes: out5[0..3] as vf_ConservationSums, out5[4] = sum of the
+ * kinetic energy 1/2 (1+drho)|u|^2 = |j|^2 / (2 (1+drho)) (out5 device pointer, overwritten).
+ * The energy term follows kineticEnergyKernel (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:99-153);
+ * divide by the fluid-node count for the reference's mean. */
+int vf_MomentSums(const vf_lbm_parameter* para, double* out5, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VIRTUALFLUIDS_AMD_H */

@@ -1,0 +1,446 @@
+// ORACLE — test infrastructure only. Not part of the product; only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+//
+// CPU restatement of VirtualFluids' single-GPU LBM time step: K17 collision with
+// esoteric-twist in-place streaming, the post-collision no-slip/velocity BCs, the
+// pre-collision pressure BC, equilibrium initialisation and macroscopic output.
+// Every function is templated on the arithmetic type; the C ABI below exports the
+// FP64 (and, where useful, FP32) instantiations. OpenMP parallelises over nodes.
+//
+// Reference semantics followed (file:line under the reference tree):
+//   eso-twist pointer table     src/gpu/core/Utilities/KernelUtilities.h:66-125
+//   neighbour chain (ListIndices) src/gpu/core/Utilities/KernelUtilities.h:221-245
+//   pre/post gathers, scatters  src/gpu/core/Utilities/KernelUtilities.h:252-379
+//   runCollision                src/gpu/core/Utilities/RunCollision.cuh:74-143
+//   BC closed forms             src/gpu/core/Utilities/KernelUtilities.h:171-214
+//   NoSlipBounceBack            src/gpu/core/BoundaryConditions/NoSlip/NoSlipBounceBack.cu:44-176
+//   NoSlipInterpolatedCompr.    src/gpu/core/BoundaryConditions/NoSlip/NoSlipInterpolatedCompressible.cu:44-378
+//   VelocityBounceBack          src/gpu/core/BoundaryConditions/Velocity/VelocityBounceBack.cu:44-186
+//   VelocityInterpolatedCompr.  src/gpu/core/BoundaryConditions/Velocity/VelocityInterpolatedCompressible.cu:44-418
+//   VelocityWithPressureInterp. src/gpu/core/BoundaryConditions/Velocity/VelocityWithPressureInterpolatedCompressible.cu
+//   PressureNonEquilibriumCompr src/gpu/core/BoundaryConditions/Pressure/PressureNonEquilibriumCompressible.cu:45-311
+//   InitNavierStokesCompressible src/gpu/core/PreProcessor/PreProcessorStrategy/InitNavierStokesCompressible/InitNavierStokesCompressible_Device.cu:44-204
+//   macroscopic output          src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237
+//   step ordering               src/gpu/core/Calculation/UpdateGrid27.cpp:54-96 (driven from Python)
+#include <cstdint>
+#include <cstring>
+
+#include "oracle_k17.h"
+
+namespace orc {
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+// One population buffer of 27*N values, SoA: buf[d*N + k] (src/gpu/core/Cuda/CudaMemoryManager.cpp:169).
+template <class T> struct Lattice {
+    T* buf;
+    u64 N;
+    const u32* nbx; // +x neighbour
+    const u32* nby; // +y neighbour
+    const u32* nbz; // +z neighbour
+};
+
+// Eso-twist slot base per direction for a parity: even -> d, odd -> inv(d), d000 fixed.
+template <class T> inline void slot_table(const Lattice<T>& L, bool even, T* P[27])
+{
+    for (int d = 0; d < 27; ++d) P[d] = L.buf + (u64)(even ? d : INV[d]) * L.N;
+}
+
+// The 8 nodes one update touches: bit0 = +x, bit1 = +y, bit2 = +z.
+template <class T> inline void cell_nodes(const Lattice<T>& L, u32 k, u32 n[8])
+{
+    n[0] = k;
+    n[1] = L.nbx[k];
+    n[2] = L.nby[k];
+    n[3] = L.nby[n[1]];
+    n[4] = L.nbz[k];
+    n[5] = L.nbz[n[1]];
+    n[6] = L.nbz[n[2]];
+    n[7] = L.nbz[n[3]];
+}
+
+// Node at which direction d of node k is stored: shifted along every axis where c_d < 0.
+inline int node_sel(int d) { return (CX[d] < 0 ? 1 : 0) | (CY[d] < 0 ? 2 : 0) | (CZ[d] < 0 ? 4 : 0); }
+
+template <class T> inline void read_pre(T* const P[27], const u32 n[8], T* f)
+{
+    for (int d = 0; d < 27; ++d) f[d] = P[d][n[node_sel(d)]];
+}
+template <class T> inline void write_post(T* const P[27], const u32 n[8], const T* f)
+{
+    for (int d = 0; d < 27; ++d) P[d][n[node_sel(d)]] = f[INV[d]];
+}
+template <class T> inline void read_post(T* const P[27], const u32 n[8], T* f)
+{
+    for (int d = 0; d < 27; ++d) f[INV[d]] = P[d][n[node_sel(d)]];
+}
+template <class T> inline void write_pre(T* const P[27], const u32 n[8], int d, T v)
+{
+    P[d][n[node_sel(d)]] = v;
+}
+
+// ---- collision over an index list (runCollision) -------------------------------------------
+template <class T>
+void collide_indexed(const Lattice<T>& L, const u32* fluid, u64 nfluid, const K17Param<T>& prm, bool even, T* rho,
+                     T* vx, T* vy, T* vz)
+{
+    T* P[27];
+    slot_table(L, even, P);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nfluid; ++i) {
+        const u32 k = fluid[i];
+        u32 n[8];
+        cell_nodes(L, k, n);
+        T f[27];
+        read_pre(P, n, f);
+        Macro<T> m;
+        k17_collide(f, prm, &m);
+        if (rho) {
+            rho[k] = m.drho;
+            vx[k] = m.vx;
+            vy[k] = m.vy;
+            vz[k] = m.vz;
+        }
+        write_post(P, n, f);
+    }
+}
+
+// ---- collision on a dense periodic box with neighbours computed on the fly -----------------
+// Node index k = x + nx*(y + ny*z). mask (optional, 1 byte per node): collide only where != 0.
+template <class T>
+void collide_box(T* buf, int nx, int ny, int nz, const uint8_t* mask, const K17Param<T>& prm, bool even)
+{
+    const u64 N = (u64)nx * ny * nz;
+    T* P[27];
+    for (int d = 0; d < 27; ++d) P[d] = buf + (u64)(even ? d : INV[d]) * N;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+            const int yp = (y + 1 == ny) ? 0 : y + 1;
+            const int zp = (z + 1 == nz) ? 0 : z + 1;
+            for (int x = 0; x < nx; ++x) {
+                const int xp = (x + 1 == nx) ? 0 : x + 1;
+                const u64 k = (u64)x + (u64)nx * ((u64)y + (u64)ny * z);
+                if (mask && !mask[k]) continue;
+                u32 n[8];
+                const u64 r00 = (u64)nx * ((u64)y + (u64)ny * z), r10 = (u64)nx * ((u64)yp + (u64)ny * z);
+                const u64 r01 = (u64)nx * ((u64)y + (u64)ny * zp), r11 = (u64)nx * ((u64)yp + (u64)ny * zp);
+                n[0] = (u32)(r00 + x); n[1] = (u32)(r00 + xp);
+                n[2] = (u32)(r10 + x); n[3] = (u32)(r10 + xp);
+                n[4] = (u32)(r01 + x); n[5] = (u32)(r01 + xp);
+                n[6] = (u32)(r11 + x); n[7] = (u32)(r11 + xp);
+                T f[27];
+                read_pre(P, n, f);
+                k17_collide(f, prm, (Macro<T>*)nullptr);
+                write_post(P, n, f);
+            }
+        }
+}
+
+// ---- boundary conditions -------------------------------------------------------------------
+// Sub-grid distances are SoA: q[d*nbc + i] (src/gpu/core/Utilities/KernelUtilities.h:140-169);
+// a link is active when 0 <= q <= 1.
+template <class T> inline bool active(T q) { return q >= T(0) && q <= T(1); }
+
+template <class T> inline T c_dot(int d, T ux, T uy, T uz)
+{
+    T s = T(0);
+    if (CX[d]) s = s + T(CX[d]) * ux;
+    if (CY[d]) s = s + T(CY[d]) * uy;
+    if (CZ[d]) s = s + T(CZ[d]) * uz;
+    return s;
+}
+
+// Macroscopic values of the post-collision state as the interpolated BC kernels form them
+// (plain sums, src/gpu/core/BoundaryConditions/Velocity/VelocityInterpolatedCompressible.cu:170-186).
+template <class T> inline void post_macro(const T* f, T& drho, T& ux, T& uy, T& uz)
+{
+    drho = T(0);
+    T jx = T(0), jy = T(0), jz = T(0);
+    for (int d = 0; d < 27; ++d) {
+        drho += f[d];
+        jx += T(CX[d]) * f[d];
+        jy += T(CY[d]) * f[d];
+        jz += T(CZ[d]) * f[d];
+    }
+    ux = jx / (T(1) + drho);
+    uy = jy / (T(1) + drho);
+    uz = jz / (T(1) + drho);
+}
+
+// getEquilibriumForBC (KernelUtilities.h:171-174): the even part of the shifted equilibrium.
+template <class T> inline T feq_even(T drho, T cu, T cu_sq, T w)
+{
+    return w * (drho + T(4.5) * cu * cu * (T(1) + drho) - cu_sq);
+}
+
+// getInterpolatedDistributionFor{NoSlip,Velo,VeloWithPressure}BC (KernelUtilities.h:176-214)
+template <class T> inline T interp_link(T q, T f, T finv, T feq, T omega, T wall_term, T pressure_term)
+{
+    return (T(1) - q) / (T(1) + q) * (f - finv + (f + finv - T(2) * feq * omega) / (T(1) - omega)) * T(0.5) +
+           (q * (f + finv) - wall_term) / (T(1) + q) - pressure_term;
+}
+
+enum BcKind : int {
+    BC_NOSLIP_BOUNCEBACK = 0,
+    BC_VELOCITY_BOUNCEBACK = 1,
+    BC_NOSLIP_INTERPOLATED = 2,
+    BC_VELOCITY_INTERPOLATED = 3,
+    BC_VELOCITY_PRESSURE_INTERPOLATED = 4,
+};
+
+// Post-collision BC: read post-collision values with the current parity, write the
+// reflected pre-collision values of the next step with the flipped parity.
+template <class T>
+void bc_post(int kind, const Lattice<T>& L, const int* k, const T* q, u64 nbc, const T* vxbc, const T* vybc,
+             const T* vzbc, T omega, bool even)
+{
+    T* P[27];
+    T* Pn[27];
+    slot_table(L, even, P);
+    slot_table(L, !even, Pn);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nbc; ++i) {
+        u32 n[8];
+        cell_nodes(L, (u32)k[i], n);
+        T f[27];
+        read_post(P, n, f);
+        T drho = 0, ux = 0, uy = 0, uz = 0, cu_sq = 0;
+        if (kind >= BC_NOSLIP_INTERPOLATED) {
+            post_macro(f, drho, ux, uy, uz);
+            cu_sq = T(1.5) * (ux * ux + uy * uy + uz * uz) * (T(1) + drho);
+        }
+        const T Ux = vxbc ? vxbc[i] : T(0), Uy = vybc ? vybc[i] : T(0), Uz = vzbc ? vzbc[i] : T(0);
+        for (int d = 1; d < 27; ++d) {
+            const T qd = q[(u64)d * nbc + i];
+            if (!active(qd)) continue;
+            const int r = INV[d];
+            const T w = weight<T>(d);
+            T v;
+            switch (kind) {
+            case BC_NOSLIP_BOUNCEBACK:
+                v = f[d];
+                break;
+            case BC_VELOCITY_BOUNCEBACK:
+                v = f[d] - T(6) * w * c_dot(d, Ux, Uy, Uz);
+                break;
+            default: {
+                const T feq = feq_even(drho, c_dot(d, ux, uy, uz), cu_sq, w);
+                const T wall = (kind == BC_NOSLIP_INTERPOLATED) ? T(0) : T(6) * w * c_dot(d, Ux, Uy, Uz);
+                const T pres = (kind == BC_VELOCITY_PRESSURE_INTERPOLATED) ? w * drho : T(0);
+                v = interp_link(qd, f[d], f[r], feq, omega, wall, pres);
+            }
+            }
+            write_pre(Pn, n, r, v);
+        }
+    }
+}
+
+// Full shifted equilibrium used by init and by the pressure BC (InitNavierStokesCompressible_Device.cu:173-201).
+template <class T> inline T feq_full(int d, T drho, T ux, T uy, T uz, T cu_sq)
+{
+    if (d == d000) return T(8) / T(27) * (drho - cu_sq * (T(1) + drho));
+    const T cu = c_dot(d, ux, uy, uz);
+    return weight<T>(d) * (drho + (T(1) + drho) * (T(3) * cu + T(4.5) * cu * cu - cu_sq));
+}
+
+// Pre-collision pressure BC (runs after the parity swap). direction selects the face:
+// dM00 writes the slots with c_x >= 0, dP00 those with c_x <= 0, and so on in y and z.
+template <class T>
+void bc_pressure_neq(const Lattice<T>& L, const int* k, const int* kN, const T* rhoBC, u64 nbc, bool even, int direction)
+{
+    T* P[27];
+    slot_table(L, even, P);
+    int dsel = -1;
+    for (int d = 1; d <= 6; ++d)
+        if (d == direction) dsel = d;
+    if (dsel < 0) return;
+    // the neighbour gathers happen before any BC-node write, as in one launch where every
+    // thread reads first (the written BC-node slots and the neighbour slots are disjoint).
+    T* fn = new T[27 * (nbc ? nbc : 1)];
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nbc; ++i) {
+        u32 n[8];
+        cell_nodes(L, (u32)kN[i], n);
+        T* f = fn + 27 * i;
+        read_pre(P, n, f);
+        T drho, inv_rho, ux, uy, uz;
+        compressible_macro(f, drho, inv_rho, ux, uy, uz);
+        const T cu_sq = T(1.5) * (ux * ux + uy * uy + uz * uz);
+        for (int d = 0; d < 27; ++d) f[d] -= feq_full(d, drho, ux, uy, uz, cu_sq);
+        for (int d = 0; d < 27; ++d) f[d] += feq_full(d, rhoBC[i], ux, uy, uz, cu_sq);
+    }
+    const int ax = (dsel - 1) / 2;          // 0:x 1:y 2:z
+    const int sgn = (dsel % 2 == 1) ? 1 : -1; // dP00 -> +1, dM00 -> -1
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nbc; ++i) {
+        u32 n[8];
+        cell_nodes(L, (u32)k[i], n);
+        const T* f = fn + 27 * i;
+        for (int d = 0; d < 27; ++d) {
+            const int c = ax == 0 ? CX[d] : (ax == 1 ? CY[d] : CZ[d]);
+            if (c * sgn <= 0) write_pre(P, n, d, f[d]);
+        }
+    }
+    delete[] fn;
+}
+
+// ---- initialisation and output -------------------------------------------------------------
+template <class T>
+void init_eq(const Lattice<T>& L, const u32* geo, const T* rho, const T* ux, const T* uy, const T* uz, bool even)
+{
+    T* P[27];
+    slot_table(L, even, P);
+#pragma omp parallel for schedule(static)
+    for (long long kk = 0; kk < (long long)L.N; ++kk) {
+        const u32 k = (u32)kk;
+        if (geo && (geo[k] == 15u || geo[k] == 16u)) continue; // GEO_SOLID, GEO_VOID
+        u32 n[8];
+        cell_nodes(L, k, n);
+        const T cu_sq = T(1.5) * (ux[k] * ux[k] + uy[k] * uy[k] + uz[k] * uz[k]);
+        for (int d = 0; d < 27; ++d) write_pre(P, n, d, feq_full(d, rho[k], ux[k], uy[k], uz[k], cu_sq));
+    }
+}
+
+template <class T>
+void macroscopic(const Lattice<T>& L, const u32* geo, bool even, T* rho, T* vx, T* vy, T* vz, T* press)
+{
+    T* P[27];
+    slot_table(L, even, P);
+#pragma omp parallel for schedule(static)
+    for (long long kk = 0; kk < (long long)L.N; ++kk) {
+        const u32 k = (u32)kk;
+        rho[k] = vx[k] = vy[k] = vz[k] = T(0);
+        if (press) press[k] = T(0);
+        // isValidFluidNode: GEO_FLUID or porous-media types (KernelUtilities.h:216-219)
+        if (geo && !(geo[k] == 19u || geo[k] == 5u || geo[k] == 6u || geo[k] == 7u)) continue;
+        u32 n[8];
+        cell_nodes(L, k, n);
+        T f[27];
+        read_pre(P, n, f);
+        const T r = density(f);
+        rho[k] = r;
+        vx[k] = momentum_x(f) / (r + T(1));
+        vy[k] = momentum_y(f) / (r + T(1));
+        vz[k] = momentum_z(f) / (r + T(1));
+        if (press) press[k] = pressure(f, r, vx[k], vy[k], vz[k]);
+    }
+}
+
+} // namespace orc
+
+// =============================================================================================
+// C ABI (ctypes). All buffers are caller-owned host arrays.
+// =============================================================================================
+using namespace orc;
+
+namespace {
+template <class T> K17Param<T> make_param(T omega, const T* lim3, const T* force3)
+{
+    K17Param<T> p;
+    p.omega = omega;
+    p.limP = lim3[0];
+    p.limM = lim3[1];
+    p.limD = lim3[2];
+    p.fx = force3 ? force3[0] : T(0);
+    p.fy = force3 ? force3[1] : T(0);
+    p.fz = force3 ? force3[2] : T(0);
+    return p;
+}
+} // namespace
+
+#define ORC_INSTANTIATE(T, SFX)                                                                                  \
+    extern "C" void orc_k17_nodes_##SFX(const T* fin, T* fout, long long n, T omega, const T* lim3,            \
+                                        const T* force3, T* macro4)                                             \
+    {                                                                                                          \
+        const K17Param<T> p = make_param<T>(omega, lim3, force3);                                              \
+        _Pragma("omp parallel for schedule(static)") for (long long i = 0; i < n; ++i)                         \
+        {                                                                                                      \
+            T f[27];                                                                                           \
+            std::memcpy(f, fin + 27 * i, sizeof(f));                                                           \
+            Macro<T> m;                                                                                        \
+            k17_collide(f, p, &m);                                                                             \
+            std::memcpy(fout + 27 * i, f, sizeof(f));                                                          \
+            if (macro4) {                                                                                      \
+                macro4[4 * i + 0] = m.drho;                                                                    \
+                macro4[4 * i + 1] = m.vx;                                                                      \
+                macro4[4 * i + 2] = m.vy;                                                                      \
+                macro4[4 * i + 3] = m.vz;                                                                      \
+            }                                                                                                  \
+        }                                                                                                      \
+    }                                                                                                          \
+    extern "C" void orc_collide_indexed_##SFX(T* buf, unsigned long long N, const uint32_t* nbx,              \
+                                              const uint32_t* nby, const uint32_t* nbz, const uint32_t* fluid, \
+                                              unsigned long long nfluid, T omega, const T* lim3,             \
+                                              const T* force3, int even, T* rho, T* vx, T* vy, T* vz)         \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        collide_indexed(L, fluid, nfluid, make_param<T>(omega, lim3, force3), even != 0, rho, vx, vy, vz);    \
+    }                                                                                                          \
+    extern "C" void orc_collide_box_##SFX(T* buf, int nx, int ny, int nz, const uint8_t* mask, T omega,        \
+                                          const T* lim3, const T* force3, int even)                            \
+    {                                                                                                          \
+        collide_box(buf, nx, ny, nz, mask, make_param<T>(omega, lim3, force3), even != 0);                     \
+    }                                                                                                          \
+    extern "C" void orc_bc_post_##SFX(int kind, T* buf, unsigned long long N, const uint32_t* nbx,           \
+                                      const uint32_t* nby, const uint32_t* nbz, const int* k, const T* q,     \
+                                      unsigned long long nbc, const T* vx, const T* vy, const T* vz, T omega, \
+                                      int even)                                                                \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        bc_post(kind, L, k, q, nbc, vx, vy, vz, omega, even != 0);                                             \
+    }                                                                                                          \
+    extern "C" void orc_bc_pressure_neq_##SFX(T* buf, unsigned long long N, const uint32_t* nbx,             \
+                                              const uint32_t* nby, const uint32_t* nbz, const int* k,         \
+                                              const int* kN, const T* rhoBC, unsigned long long nbc, int even, \
+                                              int direction)                                                   \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        bc_pressure_neq(L, k, kN, rhoBC, nbc, even != 0, direction);                                           \
+    }                                                                                                          \
+    extern "C" void orc_init_eq_##SFX(T* buf, unsigned long long N, const uint32_t* nbx, const uint32_t* nby, \
+                                      const uint32_t* nbz, const uint32_t* geo, const T* rho, const T* ux,    \
+                                      const T* uy, const T* uz, int even)                                     \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        init_eq(L, geo, rho, ux, uy, uz, even != 0);                                                           \
+    }                                                                                                          \
+    extern "C" void orc_macroscopic_##SFX(T* buf, unsigned long long N, const uint32_t* nbx,                  \
+                                          const uint32_t* nby, const uint32_t* nbz, const uint32_t* geo,      \
+                                          int even, T* rho, T* vx, T* vy, T* vz, T* press)                     \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        macroscopic(L, geo, even != 0, rho, vx, vy, vz, press);                                                \
+    }
+
+ORC_INSTANTIATE(double, f64)
+ORC_INSTANTIATE(float, f32)
+
+// Single-line chimera transforms and moments, for the reference's known-answer tests.
+// kind: 0 forward-with-K, 1 backward-with-K, 2 forward, 3 backward.
+extern "C" void orc_chimera_f64(int kind, double* abc, double u, double u2, double Kinv, double K)
+{
+    if (kind == 0) chimera_fwd(abc[0], abc[1], abc[2], u, u2, Kinv, K);
+    if (kind == 1) chimera_bwd(abc[0], abc[1], abc[2], u, u2, Kinv, K);
+    if (kind == 2) chimera_fwd(abc[0], abc[1], abc[2], u, u2, 0.0, 0.0);
+    if (kind == 3) chimera_bwd(abc[0], abc[1], abc[2], u, u2, 0.0, 0.0);
+}
+
+// out: drho, jx, jy, jz (incompressible sums), 1/rho, ux, uy, uz (compressible)
+extern "C" void orc_moments_f64(const double* f, double* out8)
+{
+    out8[0] = density(f);
+    out8[1] = momentum_x(f);
+    out8[2] = momentum_y(f);
+    out8[3] = momentum_z(f);
+    double d, ir, ux, uy, uz;
+    compressible_macro(f, d, ir, ux, uy, uz);
+    out8[4] = ir;
+    out8[5] = ux;
+    out8[6] = uy;
+    out8[7] = uz;
+}
+
+// Eso-twist slot of direction d for a parity (KernelUtilities.h:66-125): returns the array index.
+extern "C" int orc_eso_slot(int d, int even) { return even ? d : INV[d]; }

@@ -1,0 +1,202 @@
+"""ORACLE — test infrastructure only (ctypes front end of liboracle.so).
+
+The CPU restatement of the K17 path lives in oracle.cpp / oracle_k17.h; this
+module loads it and, where the reference tree was present at build time, the
+reference's own header-only arithmetic compiled into oracle/_ref/.
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg use it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_lib = None
+
+# BC kinds of orc_bc_post (oracle.cpp)
+BC_NOSLIP_BOUNCEBACK = 0
+BC_VELOCITY_BOUNCEBACK = 1
+BC_NOSLIP_INTERPOLATED = 2
+BC_VELOCITY_INTERPOLATED = 3
+BC_VELOCITY_PRESSURE_INTERPOLATED = 4
+
+
+def build(with_ref: bool | None = None) -> None:
+    """Compile liboracle.so (and oracle/_ref/ when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if with_ref is None:
+        with_ref = os.path.isdir("/root/reference/src/lbm")
+    if with_ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build(with_ref=False)
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _ctype(dtype):
+    return ctypes.c_double if np.dtype(dtype) == np.float64 else ctypes.c_float
+
+
+def _sfx(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+def k17_nodes(fin, omega, limiters=(0.01, 0.01, 0.01), force=(0.0, 0.0, 0.0), dtype=np.float64):
+    """K17 collision of n independent nodes, fin shape (n, 27). Returns (fout, macro[n,4])."""
+    fin = np.ascontiguousarray(fin, dtype=dtype)
+    n = fin.shape[0]
+    fout = np.empty_like(fin)
+    macro = np.empty((n, 4), dtype=dtype)
+    lim = np.asarray(limiters, dtype=dtype)
+    frc = np.asarray(force, dtype=dtype)
+    getattr(lib(), "orc_k17_nodes_" + _sfx(dtype))(
+        _p(fin), _p(fout), ctypes.c_longlong(n), _ctype(dtype)(omega), _p(lim), _p(frc), _p(macro))
+    return fout, macro
+
+
+def chimera(kind, abc, u, u2, kinv=0.0, k=0.0):
+    a = np.ascontiguousarray(abc, dtype=np.float64).copy()
+    lib().orc_chimera_f64(ctypes.c_int(kind), _p(a), ctypes.c_double(u), ctypes.c_double(u2),
+                          ctypes.c_double(kinv), ctypes.c_double(k))
+    return a
+
+
+def moments(f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty(8)
+    lib().orc_moments_f64(_p(f), _p(out))
+    return out
+
+
+def eso_slot(d, even):
+    return lib().orc_eso_slot(ctypes.c_int(d), ctypes.c_int(1 if even else 0))
+
+
+class Lattice:
+    """One population buffer of 27*N values (SoA) plus +x/+y/+z neighbour arrays."""
+
+    def __init__(self, nbx, nby, nbz, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.nbx = np.ascontiguousarray(nbx, dtype=np.uint32)
+        self.nby = np.ascontiguousarray(nby, dtype=np.uint32)
+        self.nbz = np.ascontiguousarray(nbz, dtype=np.uint32)
+        self.N = int(self.nbx.shape[0])
+        self.f = np.zeros(27 * self.N, dtype=self.dtype)
+        self.even = True
+        self._fn = lambda name: getattr(lib(), name + "_" + _sfx(self.dtype))
+        self._T = _ctype(self.dtype)
+
+    def _nb(self):
+        return (_p(self.f), ctypes.c_ulonglong(self.N), _p(self.nbx), _p(self.nby), _p(self.nbz))
+
+    def collide(self, fluid, omega, limiters=(0.01, 0.01, 0.01), force=(0.0, 0.0, 0.0), macro=None):
+        fluid = np.ascontiguousarray(fluid, dtype=np.uint32)
+        lim = np.asarray(limiters, dtype=self.dtype)
+        frc = np.asarray(force, dtype=self.dtype)
+        r = vx = vy = vz = None
+        if macro is not None:
+            r, vx, vy, vz = macro
+        self._fn("orc_collide_indexed")(*self._nb(), _p(fluid), ctypes.c_ulonglong(fluid.size), self._T(omega),
+                                        _p(lim), _p(frc), ctypes.c_int(int(self.even)), _p(r), _p(vx), _p(vy),
+                                        _p(vz))
+
+    def bc_post(self, kind, k, q, omega=0.0, velocity=None):
+        k = np.ascontiguousarray(k, dtype=np.int32)
+        q = np.ascontiguousarray(q, dtype=self.dtype)
+        nbc = k.size
+        assert q.size == 27 * nbc
+        vx = vy = vz = None
+        if velocity is not None:
+            vx, vy, vz = (np.ascontiguousarray(v, dtype=self.dtype) for v in velocity)
+        self._fn("orc_bc_post")(ctypes.c_int(kind), *self._nb(), _p(k), _p(q), ctypes.c_ulonglong(nbc), _p(vx),
+                                _p(vy), _p(vz), self._T(omega), ctypes.c_int(int(self.even)))
+
+    def bc_pressure(self, k, kN, rho_bc, direction):
+        k = np.ascontiguousarray(k, dtype=np.int32)
+        kN = np.ascontiguousarray(kN, dtype=np.int32)
+        rho_bc = np.ascontiguousarray(rho_bc, dtype=self.dtype)
+        self._fn("orc_bc_pressure_neq")(*self._nb(), _p(k), _p(kN), _p(rho_bc), ctypes.c_ulonglong(k.size),
+                                        ctypes.c_int(int(self.even)), ctypes.c_int(direction))
+
+    def init_eq(self, rho, ux, uy, uz, geo=None):
+        arrs = [np.ascontiguousarray(a, dtype=self.dtype) for a in (rho, ux, uy, uz)]
+        g = None if geo is None else np.ascontiguousarray(geo, dtype=np.uint32)
+        self._fn("orc_init_eq")(*self._nb(), _p(g), *(_p(a) for a in arrs), ctypes.c_int(int(self.even)))
+
+    def macroscopic(self, geo=None):
+        g = None if geo is None else np.ascontiguousarray(geo, dtype=np.uint32)
+        out = [np.empty(self.N, dtype=self.dtype) for _ in range(5)]
+        self._fn("orc_macroscopic")(*self._nb(), _p(g), ctypes.c_int(int(self.even)), *(_p(a) for a in out))
+        return out  # drho, vx, vy, vz, p
+
+    def swap(self):
+        self.even = not self.even
+
+
+def collide_box(f, nx, ny, nz, omega, even, limiters=(0.01, 0.01, 0.01), mask=None):
+    """Dense periodic box collision (node k = x + nx*(y + ny*z)), in place on f (27*N)."""
+    lim = np.asarray(limiters, dtype=f.dtype)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    getattr(lib(), "orc_collide_box_" + _sfx(f.dtype))(
+        _p(f), ctypes.c_int(nx), ctypes.c_int(ny), ctypes.c_int(nz), _p(m), _ctype(f.dtype)(omega), _p(lim),
+        None, ctypes.c_int(1 if even else 0))
+
+
+# ---- the reference's own arithmetic, compiled from its headers (present only where built) ----
+_refs = {}
+
+
+def ref(precision="f64"):
+    """ctypes handle of oracle/_ref/libref_<precision>.so, or None where it was not built."""
+    if precision not in _refs:
+        path = os.path.join(REF_DIR, f"libref_{precision}.so")
+        _refs[precision] = ctypes.CDLL(path) if os.path.exists(path) else None
+    return _refs[precision]
+
+
+def ref_k17_nodes(fin, omega, limiters=(0.01, 0.01, 0.01), force=(0.0, 0.0, 0.0), precision="f64"):
+    r = ref(precision)
+    dtype = np.float64 if precision == "f64" else np.float32
+    T = _ctype(dtype)
+    fin = np.ascontiguousarray(fin, dtype=dtype)
+    n = fin.shape[0]
+    fout = np.empty_like(fin)
+    macro = np.empty((n, 4), dtype=dtype)
+    lim = np.asarray(limiters, dtype=dtype)
+    frc = np.asarray(force, dtype=dtype)
+    r.ref_k17_nodes(_p(fin), _p(fout), ctypes.c_longlong(n), T(omega), _p(lim), _p(frc), _p(macro))
+    return fout, macro
+
+
+def ref_chimera(kind, abc, u, u2, kinv=0.0, k=0.0, precision="f64"):
+    r = ref(precision)
+    dtype = np.float64 if precision == "f64" else np.float32
+    T = _ctype(dtype)
+    a = np.ascontiguousarray(abc, dtype=dtype).copy()
+    r.ref_chimera(ctypes.c_int(kind), _p(a), T(u), T(u2), T(kinv), T(k))
+    return a
+
+
+def ref_moments(f, precision="f64"):
+    r = ref(precision)
+    dtype = np.float64 if precision == "f64" else np.float32
+    f = np.ascontiguousarray(f, dtype=dtype)
+    out = np.empty(9, dtype=dtype)
+    r.ref_moments(_p(f), _p(out))
+    return out

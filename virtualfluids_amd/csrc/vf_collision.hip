@@ -1,0 +1,328 @@
+// K17 collide + esoteric-twist stream kernels for gfx950 and their C entry points.
+//
+// One thread updates one node in place: it gathers the 27 pre-collision populations
+// from the 8 nodes {k, +x, +y, +z, ...} of its cell, collides, and writes the
+// post-collision populations back to exactly the addresses it read, in the opposite
+// slots (src/gpu/core/Utilities/KernelUtilities.h:252-281, 350-379). Because every
+// address is read and written by one thread only, the update needs no second buffer
+// and no synchronisation; the parity flag swaps the slot table between steps.
+//
+// Two addressings:
+//  * box:     dense periodic nx*ny*nz block, neighbours recomputed from (x,y,z);
+//             a workgroup covers whole x-rows, so every cache line it touches in a
+//             direction array belongs to rows it owns (no line is shared between
+//             workgroups, hence none between the XCDs' L2 caches).
+//  * indexed: the reference's sparse layout (fluid-node list + neighbour arrays,
+//             src/gpu/core/Utilities/RunCollision.cuh:74-143).
+#include <cmath>
+#include <cstdio>
+
+#include "vf_k17.h"
+
+namespace vfa {
+
+// ---- host: omega-derived constants ------------------------------------------------------
+// Relaxation rates of src/lbm/collision/K17CompressibleNavierStokes.h:251-273 evaluated
+// once per launch in double precision.
+K17Rates make_k17_rates(const vf_lbm_parameter& p)
+{
+    const double w = p.omega;
+    const double OxyyPxzz = 8.0 * (-2.0 + w) * (1.0 + 2.0 * w) / (-8.0 - 14.0 * w + 7.0 * w * w);
+    const double OxyyMxzz = 8.0 * (-2.0 + w) * (-7.0 + 4.0 * w) / (56.0 - 50.0 * w + 9.0 * w * w);
+    const double Oxyz = 24.0 * (-2.0 + w) * (-2.0 - 7.0 * w + 3.0 * w * w) /
+                        (48.0 + 152.0 * w - 130.0 * w * w + 29.0 * w * w * w);
+    const double factorA = (4.0 + 2.0 * w - 3.0 * w * w) / (2.0 - 7.0 * w + 5.0 * w * w);
+    const double factorB = (4.0 + 28.0 * w - 14.0 * w * w) / (6.0 - 21.0 * w + 15.0 * w * w);
+    const double nu = 1.0 / w - 0.5;
+    K17Rates r;
+    r.omega = (float)w;
+    r.oneMinusOmega = (float)(1.0 - w);
+    r.mHalfOmega = (float)(-0.5 * w);
+    r.threeHalfOmega = (float)(1.5 * w);
+    r.mThreeOmega = (float)(-3.0 * w);
+    r.galileo = (float)(3.0 * (1.0 - 0.5 * w));
+    r.limP = p.quadricLimiters[0];
+    r.limM = p.quadricLimiters[1];
+    r.limD = p.quadricLimiters[2];
+    r.limNumP = (float)((1.0 - OxyyPxzz) * (double)p.quadricLimiters[0]);
+    r.limNumM = (float)((1.0 - OxyyMxzz) * (double)p.quadricLimiters[1]);
+    r.limNumD = (float)((1.0 - Oxyz) * (double)p.quadricLimiters[2]);
+    r.A4 = (float)(-nu * (2.0 / 3.0) * factorA);
+    r.B4 = (float)(-nu * (1.0 / 3.0) * factorB * (-3.0 * w));
+    r.fx = p.forces[0] * 0.5f * p.forceFactor;
+    r.fy = p.forces[1] * 0.5f * p.forceFactor;
+    r.fz = p.forces[2] * 0.5f * p.forceFactor;
+    return r;
+}
+
+// ---- device: gather / scatter over the 8 cell nodes -------------------------------------
+template <class S, bool EVEN>
+__device__ __forceinline__ void gather(const S* __restrict__ f, uint64_t N, const uint32_t (&n)[8], float (&v)[27])
+{
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const char* base = reinterpret_cast<const char*>(f) + (uint64_t)slot<EVEN>(d) * N * sizeof(S);
+        v[d] = Codec<S>::load(reinterpret_cast<const S*>(base + n[node_sel(d)] * (uint32_t)sizeof(S)));
+    }
+}
+
+template <class S, bool EVEN>
+__device__ __forceinline__ void scatter(S* __restrict__ f, uint64_t N, const uint32_t (&n)[8], const float (&v)[27])
+{
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        char* base = reinterpret_cast<char*>(f) + (uint64_t)slot<EVEN>(d) * N * sizeof(S);
+        Codec<S>::store(reinterpret_cast<S*>(base + n[node_sel(d)] * (uint32_t)sizeof(S)), v[inv(d)]);
+    }
+}
+
+struct BoxArgs {
+    uint64_t N;
+    int nx, ny, nz;
+    const uint8_t* mask;
+    float* rho;
+    float* vx;
+    float* vy;
+    float* vz;
+};
+
+constexpr int kBoxBlock = 256;
+
+template <class S, bool EVEN, bool MASK, bool MACRO>
+__global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, BoxArgs a, K17Rates r)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int z = blockIdx.z;
+    if (x >= a.nx) return;
+    const int xp = (x + 1 == a.nx) ? 0 : x + 1;
+    const int yp = (y + 1 == a.ny) ? 0 : y + 1;
+    const int zp = (z + 1 == a.nz) ? 0 : z + 1;
+    const uint32_t row00 = (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)z);
+    const uint32_t row10 = (uint32_t)a.nx * ((uint32_t)yp + (uint32_t)a.ny * (uint32_t)z);
+    const uint32_t row01 = (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)zp);
+    const uint32_t row11 = (uint32_t)a.nx * ((uint32_t)yp + (uint32_t)a.ny * (uint32_t)zp);
+    const uint32_t n[8] = {row00 + x, row00 + xp, row10 + x, row10 + xp, row01 + x, row01 + xp, row11 + x, row11 + xp};
+    if (MASK && a.mask[n[0]] == 0) return;
+
+    float v[27];
+    gather<S, EVEN>(f, a.N, n, v);
+    MacroOut mo;
+    k17_collide(v, r, mo);
+    if (MACRO) {
+        a.rho[n[0]] = mo.drho;
+        a.vx[n[0]] = mo.vx;
+        a.vy[n[0]] = mo.vy;
+        a.vz[n[0]] = mo.vz;
+    }
+    scatter<S, EVEN>(f, a.N, n, v);
+}
+
+struct IndexedArgs {
+    uint64_t N;
+    const uint32_t* nbx;
+    const uint32_t* nby;
+    const uint32_t* nbz;
+    const uint32_t* indices;
+    uint32_t count;
+    float* rho;
+    float* vx;
+    float* vy;
+    float* vz;
+    float* bfx;
+    float* bfy;
+    float* bfz;
+    float forceScale; // 1/2 * forceFactor, applied to the per-node body force
+};
+
+constexpr int kIndexedBlock = 256;
+
+template <class S, bool EVEN, bool MACRO, bool BODYFORCE>
+__global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restrict__ f, IndexedArgs a, K17Rates r)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.count) return;
+    const uint32_t k = a.indices[i];
+    uint32_t n[8];
+    n[0] = k;
+    n[1] = a.nbx[k];
+    n[2] = a.nby[k];
+    n[4] = a.nbz[k];
+    n[3] = a.nby[n[1]];
+    n[5] = a.nbz[n[1]];
+    n[6] = a.nbz[n[2]];
+    n[7] = a.nbz[n[3]];
+    K17Rates rr = r;
+    if (BODYFORCE) {
+        // RunCollision.cuh:104-110: (forces + bodyForce[k]) * 1/2 * forceFactor, then reset
+        rr.fx += a.bfx[k] * a.forceScale;
+        rr.fy += a.bfy[k] * a.forceScale;
+        rr.fz += a.bfz[k] * a.forceScale;
+        a.bfx[k] = 0.0f;
+        a.bfy[k] = 0.0f;
+        a.bfz[k] = 0.0f;
+    }
+    float v[27];
+    gather<S, EVEN>(f, a.N, n, v);
+    MacroOut mo;
+    k17_collide(v, rr, mo);
+    if (MACRO) {
+        a.rho[k] = mo.drho;
+        a.vx[k] = mo.vx;
+        a.vy[k] = mo.vy;
+        a.vz[k] = mo.vz;
+    }
+    scatter<S, EVEN>(f, a.N, n, v);
+}
+
+// ---- host launchers ----------------------------------------------------------------------
+template <class S, bool EVEN>
+static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
+{
+    BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, p.rho, p.velocityX, p.velocityY, p.velocityZ};
+    const K17Rates r = make_k17_rates(p);
+    const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
+    dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
+    S* f = static_cast<S*>(p.distributions);
+    const bool mask = p.fluidMask != nullptr;
+    if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r);
+    else if (macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, true>), grid, dim3(bx), 0, s, f, a, r);
+    else hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, false>), grid, dim3(bx), 0, s, f, a, r);
+}
+
+template <class S, bool EVEN>
+static void launch_indexed_t(const vf_lbm_parameter& p, const uint32_t* idx, uint32_t n, bool macro, bool force,
+                             hipStream_t s)
+{
+    IndexedArgs a{p.numberOfNodes, p.neighborX, p.neighborY, p.neighborZ, idx, n,
+                  p.rho, p.velocityX, p.velocityY, p.velocityZ, p.bodyForceX, p.bodyForceY, p.bodyForceZ,
+                  0.5f * p.forceFactor};
+    const K17Rates r = make_k17_rates(p);
+    dim3 grid((n + kIndexedBlock - 1) / kIndexedBlock);
+    S* f = static_cast<S*>(p.distributions);
+    if (macro && force) hipLaunchKernelGGL((k17_indexed_kernel<S, EVEN, true, true>), grid, dim3(kIndexedBlock), 0, s, f, a, r);
+    else if (macro) hipLaunchKernelGGL((k17_indexed_kernel<S, EVEN, true, false>), grid, dim3(kIndexedBlock), 0, s, f, a, r);
+    else if (force) hipLaunchKernelGGL((k17_indexed_kernel<S, EVEN, false, true>), grid, dim3(kIndexedBlock), 0, s, f, a, r);
+    else hipLaunchKernelGGL((k17_indexed_kernel<S, EVEN, false, false>), grid, dim3(kIndexedBlock), 0, s, f, a, r);
+}
+
+template <template <class, bool> class L, class... A>
+static int dispatch_storage(const vf_lbm_parameter& p, A&&... args)
+{
+    const bool even = p.isEvenTimestep != 0;
+    switch (p.storage) {
+    case VF_STORAGE_FP32:
+        even ? L<float, true>::run(p, args...) : L<float, false>::run(p, args...);
+        return VF_OK;
+    case VF_STORAGE_FP16:
+        even ? L<__half, true>::run(p, args...) : L<__half, false>::run(p, args...);
+        return VF_OK;
+    case VF_STORAGE_BF16:
+        even ? L<__hip_bfloat16, true>::run(p, args...) : L<__hip_bfloat16, false>::run(p, args...);
+        return VF_OK;
+    default:
+        set_last_error("unknown storage type");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+}
+
+template <class S, bool EVEN> struct BoxL {
+    static void run(const vf_lbm_parameter& p, bool macro, hipStream_t s) { launch_box_t<S, EVEN>(p, macro, s); }
+};
+template <class S, bool EVEN> struct IndexedL {
+    static void run(const vf_lbm_parameter& p, const uint32_t* idx, uint32_t n, bool macro, bool force, hipStream_t s)
+    {
+        launch_indexed_t<S, EVEN>(p, idx, n, macro, force, s);
+    }
+};
+
+static int check_common(const vf_lbm_parameter* p)
+{
+    if (!p || !p->distributions) {
+        set_last_error("null parameter or distribution buffer");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (p->numberOfNodes == 0 || p->numberOfNodes >= (1ull << 30)) {
+        set_last_error("numberOfNodes must be in [1, 2^30)");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (!(p->omega > 0.0f && p->omega < 2.0f)) {
+        set_last_error("omega must lie in (0, 2)");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    return VF_OK;
+}
+
+} // namespace vfa
+
+using namespace vfa;
+
+extern "C" int vf_k17_run_on_indices(const vf_lbm_parameter* p, const uint32_t* indices, uint32_t n,
+                                     int32_t collisionTemplate, void* stream)
+{
+    int rc = check_common(p);
+    if (rc) return rc;
+    if (n == 0) return VF_OK;
+    if (!indices || !p->neighborX || !p->neighborY || !p->neighborZ) {
+        set_last_error("runOnIndices needs indices and neighborX/Y/Z");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    bool macro = false, force = false;
+    switch (collisionTemplate) {
+    case VF_COLLISION_DEFAULT: break;
+    case VF_COLLISION_WRITE_MACRO_VARS: macro = true; break;
+    case VF_COLLISION_APPLY_BODY_FORCE: force = true; break;
+    case VF_COLLISION_ALL_FEATURES:
+    case VF_COLLISION_SUBDOMAIN_BORDER: macro = force = true; break;
+    default:
+        // K17CompressibleNavierStokes.cu:139-141 throws on an unknown template
+        set_last_error("Invalid CollisionTemplate in CumulantK17::runOnIndices()");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (macro && (!p->rho || !p->velocityX || !p->velocityY || !p->velocityZ)) {
+        set_last_error("WriteMacroVars needs rho and velocityX/Y/Z");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (force && (!p->bodyForceX || !p->bodyForceY || !p->bodyForceZ)) {
+        set_last_error("ApplyBodyForce needs bodyForceX/Y/Z");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    rc = dispatch_storage<IndexedL>(*p, indices, n, macro, force, (hipStream_t)stream);
+    if (rc) return rc;
+    return check_launch("K17CompressibleNavierStokes (indexed)");
+}
+
+extern "C" int vf_k17_run(const vf_lbm_parameter* p, void* stream)
+{
+    if (!p) {
+        set_last_error("null parameter");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    return vf_k17_run_on_indices(p, p->fluidNodeIndices, p->numberOfFluidNodes, VF_COLLISION_DEFAULT, stream);
+}
+
+extern "C" int vf_k17_run_box(const vf_lbm_parameter* p, int32_t collisionTemplate, void* stream)
+{
+    int rc = check_common(p);
+    if (rc) return rc;
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 ||
+        (uint64_t)p->nx * (uint64_t)p->ny * (uint64_t)p->nz != p->numberOfNodes || p->ny > 65535 || p->nz > 65535) {
+        set_last_error("box path needs nx*ny*nz == numberOfNodes (ny, nz <= 65535)");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    bool macro = false;
+    if (collisionTemplate == VF_COLLISION_WRITE_MACRO_VARS) {
+        macro = true;
+        if (!p->rho || !p->velocityX || !p->velocityY || !p->velocityZ) {
+            set_last_error("WriteMacroVars needs rho and velocityX/Y/Z");
+            return VF_ERR_INVALID_ARGUMENT;
+        }
+    } else if (collisionTemplate != VF_COLLISION_DEFAULT) {
+        set_last_error("box path supports the Default and WriteMacroVars templates");
+        return VF_ERR_UNSUPPORTED;
+    }
+    rc = dispatch_storage<BoxL>(*p, macro, (hipStream_t)stream);
+    if (rc) return rc;
+    return check_launch("K17CompressibleNavierStokes (box)");
+}

@@ -1,0 +1,239 @@
+// Initialisation, macroscopic output and FP64 conservation reductions.
+//
+//   init         InitNavierStokesCompressible_Device
+//                (src/gpu/core/PreProcessor/PreProcessorStrategy/InitNavierStokesCompressible/InitNavierStokesCompressible_Device.cu:44-204)
+//   macroscopic  calculateMacroscopicQuantitiesCompressible_device
+//                (src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237)
+//   reductions   sums of drho, momentum and kinetic energy accumulated in FP64: per-wave
+//                shuffles, one LDS pass per workgroup, one FP64 atomic per workgroup.
+#include "vf_lattice.h"
+
+namespace vfa {
+
+template <class S>
+__global__ __launch_bounds__(256) void init_kernel(S* __restrict__ f, Addr a, const uint32_t* geo, const float* rho,
+                                                   const float* ux, const float* uy, const float* uz, int even_flag)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.N) return;
+    if (geo && (geo[k] == VF_GEO_SOLID || geo[k] == VF_GEO_VOID)) return;
+    uint32_t n[8];
+    cell_nodes(a, (uint32_t)k, n);
+    const float r = rho[k], vx = ux[k], vy = uy[k], vz = uz[k];
+    const float cu_sq = 1.5f * (vx * vx + vy * vy + vz * vz);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) write_pre(f, a, n, d, even_flag != 0, feq_shifted(d, r, vx, vy, vz, cu_sq));
+}
+
+__device__ __forceinline__ bool is_valid_fluid(uint32_t t)
+{
+    return t == VF_GEO_FLUID || t == VF_GEO_PM_0 || t == VF_GEO_PM_1 || t == VF_GEO_PM_2;
+}
+
+template <class S> __device__ __forceinline__ void node_moments(const S* f, const Addr& a, uint32_t k, bool even,
+                                                               float& drho, float& jx, float& jy, float& jz)
+{
+    uint32_t n[8];
+    cell_nodes(a, k, n);
+    drho = jx = jy = jz = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const float v = read_pre(f, a, n, d, even);
+        drho += v;
+        if (cx(d)) jx += float(cx(d)) * v;
+        if (cy(d)) jy += float(cy(d)) * v;
+        if (cz(d)) jz += float(cz(d)) * v;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void macroscopic_kernel(const S* __restrict__ f, Addr a, const uint32_t* geo, float* rho,
+                                                          float* vx, float* vy, float* vz, float* press, int even_flag)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.N) return;
+    rho[k] = vx[k] = vy[k] = vz[k] = 0.0f;
+    if (press) press[k] = 0.0f;
+    if (geo && !is_valid_fluid(geo[k])) return;
+    uint32_t n[8];
+    cell_nodes(a, (uint32_t)k, n);
+    float fv[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) fv[d] = read_pre(f, a, n, d, even_flag != 0);
+    float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f, e1 = 0.0f, e2 = 0.0f, e3 = 0.0f;
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        drho += fv[d];
+        if (cx(d)) jx += float(cx(d)) * fv[d];
+        if (cy(d)) jy += float(cy(d)) * fv[d];
+        if (cz(d)) jz += float(cz(d)) * fv[d];
+        const int s = cx(d) * cx(d) + cy(d) * cy(d) + cz(d) * cz(d);
+        if (s == 1) e1 += fv[d];
+        if (s == 2) e2 += fv[d];
+        if (s == 3) e3 += fv[d];
+    }
+    const float ir = 1.0f / (drho + 1.0f);
+    const float ux = jx * ir, uy = jy * ir, uz = jz * ir;
+    rho[k] = drho;
+    vx[k] = ux;
+    vy[k] = uy;
+    vz[k] = uz;
+    // getPressure (src/lbm/MacroscopicQuantities.h:139-150)
+    if (press)
+        press[k] = (e1 + 2.0f * e2 + 3.0f * e3 - drho - (ux * ux + uy * uy + uz * uz) * (1.0f + drho)) * 0.5f + drho;
+}
+
+__global__ void zero_doubles_kernel(double* out, int n)
+{
+    if ((int)threadIdx.x < n) out[threadIdx.x] = 0.0;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// out[0..3] += sum drho, jx, jy, jz; out[4] += sum 1/2 (1+drho) |u|^2 (when KE)
+template <class S, bool KE>
+__global__ __launch_bounds__(256) void moments_reduce_kernel(const S* __restrict__ f, Addr a, const uint32_t* geo,
+                                                             int even_flag, double* out)
+{
+    constexpr int NV = KE ? 5 : 4;
+    __shared__ double part[4][NV];
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.N; k += (uint64_t)gridDim.x * blockDim.x) {
+        if (geo && geo[k] != VF_GEO_FLUID) continue;
+        float drho, jx, jy, jz;
+        node_moments(f, a, (uint32_t)k, even_flag != 0, drho, jx, jy, jz);
+        acc[0] += drho;
+        acc[1] += jx;
+        acc[2] += jy;
+        acc[3] += jz;
+        if (KE) {
+            const double ir = 1.0 / (1.0 + (double)drho);
+            acc[NV - 1] += 0.5 * ((double)jx * jx + (double)jy * jy + (double)jz * jz) * ir;
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const double s = wave_sum(acc[j]);
+        if (lane == 0) part[wave][j] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const double s = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        atomicAdd(out + threadIdx.x, s);
+    }
+}
+
+template <template <class> class K, class... A> static int by_storage(int storage, A&&... args)
+{
+    switch (storage) {
+    case VF_STORAGE_FP32: K<float>::run(args...); return VF_OK;
+    case VF_STORAGE_FP16: K<__half>::run(args...); return VF_OK;
+    case VF_STORAGE_BF16: K<__hip_bfloat16>::run(args...); return VF_OK;
+    default: set_last_error("unknown storage type"); return VF_ERR_INVALID_ARGUMENT;
+    }
+}
+
+template <class S> struct InitL {
+    static void run(const vf_lbm_parameter& p, hipStream_t s)
+    {
+        const dim3 grid((unsigned)((p.numberOfNodes + 255) / 256));
+        hipLaunchKernelGGL(init_kernel<S>, grid, dim3(256), 0, s, (S*)p.distributions, make_addr(p), p.typeOfGridNode,
+                           p.rho, p.velocityX, p.velocityY, p.velocityZ, p.isEvenTimestep);
+    }
+};
+template <class S> struct MacroL {
+    static void run(const vf_lbm_parameter& p, hipStream_t s)
+    {
+        const dim3 grid((unsigned)((p.numberOfNodes + 255) / 256));
+        hipLaunchKernelGGL(macroscopic_kernel<S>, grid, dim3(256), 0, s, (const S*)p.distributions, make_addr(p),
+                           p.typeOfGridNode, p.rho, p.velocityX, p.velocityY, p.velocityZ, p.pressure, p.isEvenTimestep);
+    }
+};
+template <class S> struct ReduceL {
+    static void run(const vf_lbm_parameter& p, double* out, bool ke, hipStream_t s)
+    {
+        const uint64_t blocks = (p.numberOfNodes + 255) / 256;
+        const dim3 grid((unsigned)(blocks < 2048 ? blocks : 2048));
+        hipLaunchKernelGGL(zero_doubles_kernel, dim3(1), dim3(64), 0, s, out, ke ? 5 : 4);
+        if (ke)
+            hipLaunchKernelGGL((moments_reduce_kernel<S, true>), grid, dim3(256), 0, s, (const S*)p.distributions,
+                               make_addr(p), p.typeOfGridNode, p.isEvenTimestep, out);
+        else
+            hipLaunchKernelGGL((moments_reduce_kernel<S, false>), grid, dim3(256), 0, s, (const S*)p.distributions,
+                               make_addr(p), p.typeOfGridNode, p.isEvenTimestep, out);
+    }
+};
+
+static int check_lattice(const vf_lbm_parameter* p)
+{
+    if (!p || !p->distributions || p->numberOfNodes == 0 || p->numberOfNodes >= (1ull << 30)) {
+        set_last_error("null parameter/distributions or numberOfNodes out of [1, 2^30)");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (!p->neighborX && (uint64_t)p->nx * p->ny * p->nz != p->numberOfNodes) {
+        set_last_error("lattice needs neighbour arrays or a dense box");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    return VF_OK;
+}
+
+} // namespace vfa
+
+using namespace vfa;
+
+extern "C" int vf_InitNavierStokesCompressible(const vf_lbm_parameter* p, void* stream)
+{
+    int rc = check_lattice(p);
+    if (rc) return rc;
+    if (!p->rho || !p->velocityX || !p->velocityY || !p->velocityZ) {
+        set_last_error("init needs rho and velocityX/Y/Z");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    rc = by_storage<InitL>(p->storage, *p, (hipStream_t)stream);
+    return rc ? rc : check_launch("InitNavierStokesCompressible_Device execution failed");
+}
+
+extern "C" int vf_CalculateMacroscopicQuantitiesCompressible(const vf_lbm_parameter* p, void* stream)
+{
+    int rc = check_lattice(p);
+    if (rc) return rc;
+    if (!p->rho || !p->velocityX || !p->velocityY || !p->velocityZ) {
+        set_last_error("macroscopic output needs rho and velocityX/Y/Z");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    rc = by_storage<MacroL>(p->storage, *p, (hipStream_t)stream);
+    return rc ? rc : check_launch("calculateMacroscopicQuantitiesCompressible execution failed");
+}
+
+extern "C" int vf_ConservationSums(const vf_lbm_parameter* p, double* out4, void* stream)
+{
+    int rc = check_lattice(p);
+    if (rc) return rc;
+    if (!out4) {
+        set_last_error("null output");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    rc = by_storage<ReduceL>(p->storage, *p, out4, false, (hipStream_t)stream);
+    return rc ? rc : check_launch("conservation sums");
+}
+
+/* Sums drho, jx, jy, jz and kinetic energy in one pass (out5 device pointer, overwritten). */
+extern "C" int vf_MomentSums(const vf_lbm_parameter* p, double* out5, void* stream)
+{
+    int rc = check_lattice(p);
+    if (rc) return rc;
+    if (!out5) {
+        set_last_error("null output");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    rc = by_storage<ReduceL>(p->storage, *p, out5, true, (hipStream_t)stream);
+    return rc ? rc : check_launch("moment sums");
+}

@@ -1,0 +1,163 @@
+"""Structured grids in the reference's data layout.
+
+Builds, on the host, the arrays the GridGenerator/GridProvider pair hands to the
+time step (src/gpu/core/DataStructureInitializer/GridProvider.h:61-85):
++x/+y/+z neighbour arrays (src/gpu/GridGenerator/grid/GridImp.cpp:1012-1078),
+node types (src/gpu/core/Calculation/Calculation.h:36-45), the fluid-node list, and
+boundary-condition node lists with SoA sub-grid distances q (q = 0.5 on box sides,
+src/gpu/GridGenerator/grid/BoundaryConditions/Side.cpp:149-167).
+
+Node index of a box: k = x + nx*(y + ny*z). Boxes carry no dummy node 0; the
+neighbour arrays wrap periodically, so a solid layer closes a walled box.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+GEO_SOLID = 15
+GEO_FLUID = 19
+
+# D3Q27 lattice velocities in the order of src/lbm/constants/D3Q27.h:46-72
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1])
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, -1])
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, 1, -1, -1, -1, -1])
+INV = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17, 26, 25, 24, 23, 22, 21, 20, 19])
+W = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
+
+# the six sides by their D3Q27 direction (Side::getD3Q27Direction)
+dP00, dM00, d0P0, d0M0, d00P, d00M = 1, 2, 3, 4, 5, 6
+
+
+@dataclass
+class BoundaryNodes:
+    """A QforBoundaryConditions list: BC node indices and SoA q27[d*n + i] (-1 = no link)."""
+
+    k: np.ndarray
+    q27: np.ndarray
+    velocity: tuple | None = None      # (Vx, Vy, Vz) per BC node
+    kN: np.ndarray | None = None       # pressure BC neighbour nodes
+    rho_bc: np.ndarray | None = None   # pressure BC drho per node
+    direction: int = 0                 # pressure BC side
+
+    @property
+    def n(self) -> int:
+        return int(self.k.size)
+
+
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    nz: int
+    nbx: np.ndarray
+    nby: np.ndarray
+    nbz: np.ndarray
+    geo: np.ndarray
+    fluid: np.ndarray
+    bcs: dict = field(default_factory=dict)
+
+    @property
+    def N(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def coords(self):
+        k = np.arange(self.N)
+        return k % self.nx, (k // self.nx) % self.ny, k // (self.nx * self.ny)
+
+    def index(self, x, y, z):
+        return np.asarray(x) + self.nx * (np.asarray(y) + self.ny * np.asarray(z))
+
+    @property
+    def fluid_mask(self) -> np.ndarray:
+        return (self.geo == GEO_FLUID).astype(np.uint8)
+
+
+def _neighbours(nx, ny, nz):
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    x, y, z = x.transpose(2, 1, 0).ravel(), y.transpose(2, 1, 0).ravel(), z.transpose(2, 1, 0).ravel()
+    idx = lambda a, b, c: (a % nx) + nx * ((b % ny) + ny * (c % nz))
+    return (idx(x + 1, y, z).astype(np.uint32), idx(x, y + 1, z).astype(np.uint32),
+            idx(x, y, z + 1).astype(np.uint32), x, y, z)
+
+
+def periodic_box(nx: int, ny: int, nz: int) -> Grid:
+    nbx, nby, nbz, *_ = _neighbours(nx, ny, nz)
+    N = nx * ny * nz
+    geo = np.full(N, GEO_FLUID, dtype=np.uint32)
+    return Grid(nx, ny, nz, nbx, nby, nbz, geo, np.arange(N, dtype=np.uint32))
+
+
+def _wall_links(grid: Grid, x, y, z, solid_of):
+    """For fluid nodes (x,y,z): per direction, whether the link ends in a solid node."""
+    out = np.zeros((27, x.size), dtype=bool)
+    for d in range(1, 27):
+        out[d] = solid_of(x + CX[d], y + CY[d], z + CZ[d])
+    return out
+
+
+def walled_box(n: tuple, walls: dict, q: float = 0.5, periodic: tuple = (False, False, False)) -> Grid:
+    """Fluid block of n = (fx, fy, fz) nodes with one solid layer on every non-periodic side.
+
+    walls maps a side (dP00, dM00, d0P0, d0M0, d00P, d00M) to ("noslip",) or
+    ("velocity", (Ux, Uy, Uz)). A link belongs to the side whose plane it crosses
+    (z sides first, then y, then x); every wall link gets sub-grid distance q.
+    Returns the grid with bcs["noslip"] and bcs["velocity"] lists.
+    """
+    fx, fy, fz = n
+    pad = [0 if p else 1 for p in periodic]
+    nx, ny, nz = fx + 2 * pad[0], fy + 2 * pad[1], fz + 2 * pad[2]
+    nbx, nby, nbz, x, y, z = _neighbours(nx, ny, nz)
+    lo = np.array(pad)
+    hi = np.array([nx - 1 - pad[0], ny - 1 - pad[1], nz - 1 - pad[2]])
+    fluid_sel = (x >= lo[0]) & (x <= hi[0]) & (y >= lo[1]) & (y <= hi[1]) & (z >= lo[2]) & (z <= hi[2])
+    geo = np.where(fluid_sel, GEO_FLUID, GEO_SOLID).astype(np.uint32)
+    grid = Grid(nx, ny, nz, nbx, nby, nbz, geo, np.nonzero(fluid_sel)[0].astype(np.uint32))
+
+    kf = np.nonzero(fluid_sel)[0]
+    xf, yf, zf = x[kf], y[kf], z[kf]
+
+    def side_of(d):
+        tx, ty, tz = xf + CX[d], yf + CY[d], zf + CZ[d]
+        s = np.zeros(kf.size, dtype=np.int64)
+        if not periodic[0]:
+            s = np.where(tx > hi[0], dP00, np.where(tx < lo[0], dM00, s))
+        if not periodic[1]:
+            s = np.where(ty > hi[1], d0P0, np.where(ty < lo[1], d0M0, s))
+        if not periodic[2]:
+            s = np.where(tz > hi[2], d00P, np.where(tz < lo[2], d00M, s))
+        return s
+
+    sides = np.stack([side_of(d) for d in range(27)])  # 27 x nfluid, 0 = no wall
+    for kind in ("noslip", "velocity"):
+        active = np.zeros_like(sides, dtype=bool)
+        ux = np.zeros(kf.size)
+        uy = np.zeros(kf.size)
+        uz = np.zeros(kf.size)
+        for side, spec in walls.items():
+            if spec[0] != kind:
+                continue
+            hit = sides == side
+            active |= hit
+            if kind == "velocity":
+                any_hit = hit.any(axis=0)
+                ux = np.where(any_hit, spec[1][0], ux)
+                uy = np.where(any_hit, spec[1][1], uy)
+                uz = np.where(any_hit, spec[1][2], uz)
+        nodes = np.nonzero(active.any(axis=0))[0]
+        if nodes.size == 0:
+            continue
+        qarr = np.where(active[:, nodes], q, -1.0).astype(np.float64)
+        bc = BoundaryNodes(k=kf[nodes].astype(np.int32), q27=qarr.ravel())
+        if kind == "velocity":
+            bc.velocity = (ux[nodes], uy[nodes], uz[nodes])
+        grid.bcs[kind] = bc
+    return grid
+
+
+def cavity(n: int, lid_velocity: float = 0.05) -> Grid:
+    """Lid-driven cavity: n^3 fluid nodes, lid moving in +x on the z-max side, no-slip elsewhere."""
+    walls = {s: ("noslip",) for s in (dP00, dM00, d0P0, d0M0, d00M)}
+    walls[d00P] = ("velocity", (lid_velocity, 0.0, 0.0))
+    return walled_box((n, n, n), walls)

@@ -1,0 +1,178 @@
+"""Device state of one grid level and the per-kernel calls of the time step.
+
+A ``Lattice`` owns, in HBM (torch tensors used as plain device memory):
+the single in-place population buffer (27*N, FP32/FP16/BF16), neighbour arrays,
+node types, the fluid-node list and the macroscopic output arrays. It fills the
+``vf_lbm_parameter`` struct of the C ABI and calls the HIP kernels through it.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import capi
+from .grid import Grid
+
+STORAGE = {
+    "fp32": (capi.VF_STORAGE_FP32, torch.float32),
+    "fp16": (capi.VF_STORAGE_FP16, torch.float16),
+    "bf16": (capi.VF_STORAGE_BF16, torch.bfloat16),
+}
+
+
+def omega_from_viscosity(nu: float) -> float:
+    """omega = 1 / (3 nu + 1/2) (src/gpu/core/Parameter/Parameter.cpp:400-418)."""
+    return 1.0 / (3.0 * nu + 0.5)
+
+
+def _u32(a: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to(device)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class DeviceBC:
+    """A boundary-condition node list resident on the device (QforBoundaryConditions)."""
+
+    def __init__(self, nodes, device):
+        self.n = nodes.n
+        self.k = torch.from_numpy(nodes.k.astype(np.int32)).to(device)
+        self.q27 = torch.from_numpy(nodes.q27.astype(np.float32)).to(device)
+        self.vel = None
+        if nodes.velocity is not None:
+            self.vel = [torch.from_numpy(np.asarray(v, dtype=np.float32)).to(device) for v in nodes.velocity]
+        self.kN = None if nodes.kN is None else torch.from_numpy(nodes.kN.astype(np.int32)).to(device)
+        self.rho_bc = None if nodes.rho_bc is None else torch.from_numpy(nodes.rho_bc.astype(np.float32)).to(device)
+        self.direction = nodes.direction
+        self.struct = capi.QBoundaryConditions()
+        s = self.struct
+        s.k = _ptr(self.k)
+        s.kN = _ptr(self.kN)
+        s.q27 = _ptr(self.q27)
+        s.numberOfBCnodes = self.n
+        if self.vel is not None:
+            s.Vx, s.Vy, s.Vz = (_ptr(v) for v in self.vel)
+        s.RhoBC = _ptr(self.rho_bc)
+        s.direction = self.direction
+
+
+class Lattice:
+    """One grid level on the GPU."""
+
+    def __init__(self, grid: Grid, omega: float, storage: str = "fp32", limiters=(0.01, 0.01, 0.01),
+                 forces=(0.0, 0.0, 0.0), level: int = 0, box: bool | None = None, device="cuda"):
+        if storage not in STORAGE:
+            raise ValueError(f"storage must be one of {sorted(STORAGE)}")
+        self.grid = grid
+        self.device = torch.device(device)
+        self.storage = storage
+        code, dtype = STORAGE[storage]
+        N = grid.N
+        self.N = N
+        self.f = torch.zeros(27 * N, dtype=dtype, device=self.device)
+        self.nbx = _u32(grid.nbx, self.device)
+        self.nby = _u32(grid.nby, self.device)
+        self.nbz = _u32(grid.nbz, self.device)
+        self.geo = _u32(grid.geo, self.device)
+        self.fluid = _u32(grid.fluid, self.device)
+        all_fluid = bool(np.all(grid.geo == 19))
+        self.mask = None if all_fluid else torch.from_numpy(grid.fluid_mask).to(self.device)
+        self.rho = torch.zeros(N, dtype=torch.float32, device=self.device)
+        self.vx = torch.zeros_like(self.rho)
+        self.vy = torch.zeros_like(self.rho)
+        self.vz = torch.zeros_like(self.rho)
+        self.press = torch.zeros_like(self.rho)
+        self.sums = torch.zeros(5, dtype=torch.float64, device=self.device)
+        # the dense-box path needs no neighbour loads; default to it for boxes
+        self.box = True if box is None else box
+        self.bcs = {name: DeviceBC(nodes, self.device) for name, nodes in grid.bcs.items()}
+
+        p = capi.LBMParameter()
+        p.distributions = _ptr(self.f)
+        p.storage = code
+        p.isEvenTimestep = 1
+        p.numberOfNodes = N
+        p.neighborX, p.neighborY, p.neighborZ = _ptr(self.nbx), _ptr(self.nby), _ptr(self.nbz)
+        p.typeOfGridNode = _ptr(self.geo)
+        p.nx, p.ny, p.nz = grid.nx, grid.ny, grid.nz
+        p.fluidMask = _ptr(self.mask)
+        p.omega = omega
+        p.quadricLimiters[:] = list(limiters)
+        p.forces[:] = list(forces)
+        p.forceFactor = 1.0 / (2 ** level)
+        p.rho, p.velocityX, p.velocityY, p.velocityZ = (_ptr(t) for t in (self.rho, self.vx, self.vy, self.vz))
+        p.pressure = _ptr(self.press)
+        p.fluidNodeIndices = _ptr(self.fluid)
+        p.numberOfFluidNodes = int(grid.fluid.size)
+        self.param = p
+
+    # ---- parity -------------------------------------------------------------------------
+    @property
+    def even(self) -> bool:
+        return bool(self.param.isEvenTimestep)
+
+    @even.setter
+    def even(self, v: bool):
+        self.param.isEvenTimestep = 1 if v else 0
+
+    def swap(self):
+        """swapBetweenEvenAndOddTimestep (src/gpu/core/Calculation/UpdateGrid27.cpp:265-269)."""
+        self.even = not self.even
+
+    @property
+    def omega(self) -> float:
+        return float(self.param.omega)
+
+    # ---- kernels -------------------------------------------------------------------------
+    def pref(self):
+        return ctypes.byref(self.param)
+
+    def collide(self, write_macro: bool = False, stream=None):
+        tmpl = capi.COLLISION_WRITE_MACRO_VARS if write_macro else capi.COLLISION_DEFAULT
+        if self.box:
+            capi.call("vf_k17_run_box", self.pref(), tmpl, stream_handle(stream))
+        else:
+            capi.call("vf_k17_run_on_indices", self.pref(), _ptr(self.fluid), int(self.grid.fluid.size), tmpl,
+                      stream_handle(stream))
+
+    def apply_bc(self, fn_name: str, name: str, stream=None):
+        bc = self.bcs.get(name)
+        if bc is None or bc.n == 0:
+            return
+        capi.call(fn_name, self.pref(), ctypes.byref(bc.struct), stream_handle(stream))
+
+    def init_equilibrium(self, rho, ux, uy, uz, stream=None):
+        """Shifted equilibrium from drho and velocity (numpy or torch, N values each)."""
+        for dst, src in ((self.rho, rho), (self.vx, ux), (self.vy, uy), (self.vz, uz)):
+            dst.copy_(torch.as_tensor(np.asarray(src, dtype=np.float32) if not torch.is_tensor(src) else src,
+                                      dtype=torch.float32))
+        capi.call("vf_InitNavierStokesCompressible", self.pref(), stream_handle(stream))
+
+    def macroscopic(self, stream=None):
+        """drho, vx, vy, vz, p from the pre-collision state (host numpy arrays)."""
+        capi.call("vf_CalculateMacroscopicQuantitiesCompressible", self.pref(), stream_handle(stream))
+        torch.cuda.synchronize(self.device)
+        return tuple(t.cpu().numpy() for t in (self.rho, self.vx, self.vy, self.vz, self.press))
+
+    def moment_sums(self, stream=None) -> np.ndarray:
+        """FP64 sums over fluid nodes: drho, jx, jy, jz, kinetic energy."""
+        capi.call("vf_MomentSums", self.pref(), _ptr(self.sums), stream_handle(stream))
+        torch.cuda.synchronize(self.device)
+        return self.sums.cpu().numpy().copy()
+
+    def populations(self) -> np.ndarray:
+        """The raw 27*N buffer as FP32 numpy (slot layout of the current parity)."""
+        torch.cuda.synchronize(self.device)
+        return self.f.float().cpu().numpy()
+
+    def set_populations(self, host: np.ndarray):
+        self.f.copy_(torch.from_numpy(np.ascontiguousarray(host, dtype=np.float32)).to(self.f.dtype))
