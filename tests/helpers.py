@@ -59,3 +59,35 @@ def stream_periodic(post, nx, ny, nz):
     for d in range(27):
         out[..., d] = np.roll(f[..., d], shift=(CZ[d], CY[d], CX[d]), axis=(0, 1, 2))
     return out.reshape(-1, 27)
+
+
+# ---- the oracle driven with UpdateGrid27's order (src/gpu/core/Calculation/UpdateGrid27.cpp:54-96) ----
+ORACLE_POST_BC = {
+    "VelocityBounceBack": 1, "VelocityInterpolatedCompressible": 3,
+    "VelocityWithPressureInterpolatedCompressible": 4,
+    "NoSlipBounceBack": 0, "NoSlipInterpolatedCompressible": 2,
+}
+
+
+def oracle_run(orc, grid, omega, steps, velocity_bc=None, noslip_bc=None, pressure=False, init=None,
+               limiters=(0.01, 0.01, 0.01), dtype=np.float64):
+    """collision -> velocity BC -> no-slip BC -> swap -> pressure BC, `steps` times."""
+    L = orc.Lattice(grid.nbx, grid.nby, grid.nbz, dtype=dtype)
+    N = grid.N
+    if init is None:
+        L.init_eq(np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N), geo=grid.geo)
+    else:
+        L.init_eq(*init, geo=grid.geo)
+    vel = grid.bcs.get("velocity")
+    ns = grid.bcs.get("noslip")
+    pr = grid.bcs.get("pressure")
+    for _ in range(steps):
+        L.collide(grid.fluid, omega, limiters)
+        if velocity_bc and vel is not None:
+            L.bc_post(ORACLE_POST_BC[velocity_bc], vel.k, vel.q27, omega, vel.velocity)
+        if noslip_bc and ns is not None:
+            L.bc_post(ORACLE_POST_BC[noslip_bc], ns.k, ns.q27, omega)
+        L.swap()
+        if pressure and pr is not None:
+            L.bc_pressure(pr.k, pr.kN, pr.rho_bc, pr.direction)
+    return L

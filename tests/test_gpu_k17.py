@@ -131,9 +131,9 @@ def test_mass_and_momentum_sums_fp64(gpu, orc):
     s = L.moment_sums()
     drho = f32.sum(1)
     jx, jy, jz = f32 @ CX, f32 @ CY, f32 @ CZ
-    np.testing.assert_allclose(s[:4], [drho.sum(), jx.sum(), jy.sum(), jz.sum()], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(s[:4], [drho.sum(), jx.sum(), jy.sum(), jz.sum()], rtol=1e-12, atol=1e-12)
     ke = 0.5 * (jx ** 2 + jy ** 2 + jz ** 2) / (1 + drho)
-    assert s[4] == pytest.approx(ke.sum(), rel=1e-5)
+    assert s[4] == pytest.approx(ke.sum(), rel=1e-12)
     # conservation over steps: FP32 arithmetic keeps the total mass to round-off
     m0 = s[0]
     for _ in range(20):

@@ -30,19 +30,20 @@ __device__ __forceinline__ bool is_valid_fluid(uint32_t t)
     return t == VF_GEO_FLUID || t == VF_GEO_PM_0 || t == VF_GEO_PM_1 || t == VF_GEO_PM_2;
 }
 
+// Moments of one node accumulated in FP64 (every population converted before the add).
 template <class S> __device__ __forceinline__ void node_moments(const S* f, const Addr& a, uint32_t k, bool even,
-                                                               float& drho, float& jx, float& jy, float& jz)
+                                                               double& drho, double& jx, double& jy, double& jz)
 {
     uint32_t n[8];
     cell_nodes(a, k, n);
-    drho = jx = jy = jz = 0.0f;
+    drho = jx = jy = jz = 0.0;
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
-        const float v = read_pre(f, a, n, d, even);
+        const double v = (double)read_pre(f, a, n, d, even);
         drho += v;
-        if (cx(d)) jx += float(cx(d)) * v;
-        if (cy(d)) jy += float(cy(d)) * v;
-        if (cz(d)) jz += float(cz(d)) * v;
+        if (cx(d)) jx += double(cx(d)) * v;
+        if (cy(d)) jy += double(cy(d)) * v;
+        if (cz(d)) jz += double(cz(d)) * v;
     }
 }
 
@@ -107,15 +108,14 @@ __global__ __launch_bounds__(256) void moments_reduce_kernel(const S* __restrict
     for (int j = 0; j < NV; ++j) acc[j] = 0.0;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < a.N; k += (uint64_t)gridDim.x * blockDim.x) {
         if (geo && geo[k] != VF_GEO_FLUID) continue;
-        float drho, jx, jy, jz;
+        double drho, jx, jy, jz;
         node_moments(f, a, (uint32_t)k, even_flag != 0, drho, jx, jy, jz);
         acc[0] += drho;
         acc[1] += jx;
         acc[2] += jy;
         acc[3] += jz;
         if (KE) {
-            const double ir = 1.0 / (1.0 + (double)drho);
-            acc[NV - 1] += 0.5 * ((double)jx * jx + (double)jy * jy + (double)jz * jz) * ir;
+            acc[NV - 1] += 0.5 * (jx * jx + jy * jy + jz * jz) / (1.0 + drho);
         }
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;

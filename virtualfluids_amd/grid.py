@@ -89,21 +89,14 @@ def periodic_box(nx: int, ny: int, nz: int) -> Grid:
     return Grid(nx, ny, nz, nbx, nby, nbz, geo, np.arange(N, dtype=np.uint32))
 
 
-def _wall_links(grid: Grid, x, y, z, solid_of):
-    """For fluid nodes (x,y,z): per direction, whether the link ends in a solid node."""
-    out = np.zeros((27, x.size), dtype=bool)
-    for d in range(1, 27):
-        out[d] = solid_of(x + CX[d], y + CY[d], z + CZ[d])
-    return out
-
-
 def walled_box(n: tuple, walls: dict, q: float = 0.5, periodic: tuple = (False, False, False)) -> Grid:
     """Fluid block of n = (fx, fy, fz) nodes with one solid layer on every non-periodic side.
 
-    walls maps a side (dP00, dM00, d0P0, d0M0, d00P, d00M) to ("noslip",) or
-    ("velocity", (Ux, Uy, Uz)). A link belongs to the side whose plane it crosses
-    (z sides first, then y, then x); every wall link gets sub-grid distance q.
-    Returns the grid with bcs["noslip"] and bcs["velocity"] lists.
+    walls maps a side (dP00, dM00, d0P0, d0M0, d00P, d00M) to ("noslip",),
+    ("velocity", (Ux, Uy, Uz)) or ("pressure", drho). A link belongs to the side whose
+    plane it crosses (z sides first, then y, then x); every wall link gets sub-grid
+    distance q. Returns the grid with bcs["noslip"], bcs["velocity"] and bcs["pressure"]
+    lists (the pressure list holds the last fluid layer and its inward neighbours kN).
     """
     fx, fy, fz = n
     pad = [0 if p else 1 for p in periodic]
@@ -130,6 +123,20 @@ def walled_box(n: tuple, walls: dict, q: float = 0.5, periodic: tuple = (False, 
         return s
 
     sides = np.stack([side_of(d) for d in range(27)])  # 27 x nfluid, 0 = no wall
+    for side, spec in walls.items():
+        if spec[0] != "pressure":
+            continue
+        # pressure side: the last fluid layer, extrapolated from the next layer inwards
+        ax = (side - 1) // 2
+        plus = side % 2 == 1
+        coord = (xf, yf, zf)[ax]
+        on = coord == (hi[ax] if plus else lo[ax])
+        kb = kf[on]
+        step = (1 if ax == 0 else (nx if ax == 1 else nx * ny))
+        kn = kb - step if plus else kb + step
+        rho = np.full(kb.size, spec[1] if len(spec) > 1 else 0.0)
+        grid.bcs["pressure"] = BoundaryNodes(k=kb.astype(np.int32), q27=np.full(27 * kb.size, -1.0),
+                                             kN=kn.astype(np.int32), rho_bc=rho, direction=side)
     for kind in ("noslip", "velocity"):
         active = np.zeros_like(sides, dtype=bool)
         ux = np.zeros(kf.size)
@@ -154,6 +161,14 @@ def walled_box(n: tuple, walls: dict, q: float = 0.5, periodic: tuple = (False, 
             bc.velocity = (ux[nodes], uy[nodes], uz[nodes])
         grid.bcs[kind] = bc
     return grid
+
+
+def channel(n: tuple, inflow: float, q: float = 0.5) -> Grid:
+    """Channel along x: velocity inflow at x-min, pressure (drho = 0) at x-max, no-slip y/z walls."""
+    walls = {s: ("noslip",) for s in (d0P0, d0M0, d00P, d00M)}
+    walls[dM00] = ("velocity", (inflow, 0.0, 0.0))
+    walls[dP00] = ("pressure", 0.0)
+    return walled_box(n, walls, q=q)
 
 
 def cavity(n: int, lid_velocity: float = 0.05) -> Grid:
