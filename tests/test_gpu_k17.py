@@ -23,7 +23,7 @@ def _oracle_lattice(orc, grid):
     return orc.Lattice(grid.nbx, grid.nby, grid.nbz)
 
 
-@pytest.mark.parametrize("dims", [(24, 20, 16), (80, 6, 5), (64, 8, 4)])
+@pytest.mark.parametrize("dims", [(24, 20, 16), (80, 6, 5), (64, 8, 4), (256, 3, 2)])
 @pytest.mark.parametrize("box", [True, False])
 @pytest.mark.parametrize("even", [True, False])
 def test_one_step_matches_oracle(gpu, orc, dims, box, even):
@@ -47,6 +47,35 @@ def test_one_step_matches_oracle(gpu, orc, dims, box, even):
     err = np.abs(got - ref.f).max()
     # FP32 arithmetic on O(0.05) shifted populations: a few ulp of the inputs
     assert err < 2e-7, err
+
+
+# row kernel widths: 0 = scalar kernel; 1/2/4 nodes per lane (rows of 1-4 waves)
+@pytest.mark.parametrize("width", [0, 1, 2, 4])
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("dims", [(64, 5, 3), (128, 4, 3), (256, 3, 2), (512, 2, 2)])
+def test_row_kernel_widths_match_oracle(gpu, orc, monkeypatch, width, storage, dims):
+    """Every row-kernel width, storage type and row length (1 to 4 waves per row, with the
+    periodic wrap) against the FP64 oracle applied to the same stored (rounded) input."""
+    monkeypatch.setenv("VFAMD_ROW_WIDTH", str(width))
+    grid = periodic_box(*dims)
+    rng = np.random.default_rng(200 + width + sum(dims))
+    f0 = near_equilibrium(rng, grid.N, u_amp=0.04, noise=5e-3)
+    for even in (True, False):
+        L = _lattice(grid, 1.9, storage=storage)
+        L.even = even
+        L.set_populations(pack_pre(f0, grid, even))
+        stored = L.populations().astype(np.float64)  # input after storage rounding
+        ref = _oracle_lattice(orc, grid)
+        ref.even = even
+        ref.f[:] = stored
+        ref.collide(grid.fluid, 1.9)
+        L.collide()
+        got = L.populations()
+        # FP32: a few ulp; FP16/BF16: one storage rounding of the output (|f| < 0.06)
+        tol = {"fp32": 2e-7, "fp16": 0.06 * 2.0 ** -11 + 2e-7, "bf16": 0.06 * 2.0 ** -8 + 2e-7}[storage]
+        err = np.abs(got - ref.f).max()
+        print(f"width={width} {storage} dims={dims} even={even} err={err:.3e}")
+        assert err < tol, err
 
 
 def test_box_and_indexed_kernels_identical(gpu):

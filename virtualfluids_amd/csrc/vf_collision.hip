@@ -16,8 +16,10 @@
 //             src/gpu/core/Utilities/RunCollision.cuh:74-143).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 
 #include "vf_k17.h"
+#include "vf_rowio.h"
 
 namespace vfa {
 
@@ -118,6 +120,131 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, B
     scatter<S, EVEN>(f, a.N, n, v);
 }
 
+// ---- row kernel: one workgroup per x-row, V consecutive x-nodes per lane ----------------
+// Every global access of a wave is an aligned, contiguous, non-temporal vector access.
+// The nine directions with c_x = -1 live one node further in +x (node_sel bit 0); a lane
+// loads its own aligned elements and takes the element after its last node from lane+1
+// (__shfl), and stores the element before its first node from lane-1. Across the waves of
+// the row, and around the periodic wrap, the boundary lanes trade those values through LDS.
+__device__ constexpr int kShiftedDirs[9] = {dM00, dMM0, dMP0, dM0M, dM0P, dMPP, dMMP, dMPM, dMMM};
+__host__ __device__ constexpr int shifted_index(int d)
+{
+    return d == dM00 ? 0 : d == dMM0 ? 1 : d == dMP0 ? 2 : d == dM0M ? 3 : d == dM0P ? 4 : d == dMPP ? 5
+         : d == dMMP ? 6 : d == dMPM ? 7 : d == dMMM ? 8 : -1;
+}
+
+constexpr int kRowMaxThreads = 256;
+
+template <class S, int V, bool EVEN, bool MASK, bool MACRO>
+__global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__ f, BoxArgs a, K17Rates r)
+{
+    __shared__ float xin[kRowMaxThreads / 64][9];
+    __shared__ float xout[kRowMaxThreads / 64][9];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+    const uint32_t y = blockIdx.x, z = blockIdx.y;
+    const uint32_t yp = (y + 1 == (uint32_t)a.ny) ? 0 : y + 1;
+    const uint32_t zp = (z + 1 == (uint32_t)a.nz) ? 0 : z + 1;
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
+    const uint32_t x0 = threadIdx.x * V;
+    // node index of this lane's first node in the 4 rows its cells touch, indexed by the
+    // y/z bits of node_sel; each direction array has its own (uniform) 64-bit base and the
+    // per-lane byte offsets stay 32-bit (N * sizeof(S) < 4 GiB)
+    const uint32_t rows[4] = {nx * (y + ny * z) + x0, nx * (yp + ny * z) + x0, nx * (y + ny * zp) + x0,
+                              nx * (yp + ny * zp) + x0};
+    const uint32_t boff[4] = {rows[0] * (uint32_t)sizeof(S), rows[1] * (uint32_t)sizeof(S),
+                              rows[2] * (uint32_t)sizeof(S), rows[3] * (uint32_t)sizeof(S)};
+    const uint64_t slotBytes = a.N * sizeof(S);
+
+    float v[27][V];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const char* base = reinterpret_cast<const char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
+        RowIO<S, V>::load(reinterpret_cast<const S*>(base + boff[node_sel(d) >> 1]), v[d]);
+    }
+
+    // element after this lane's last node, for the c_x = -1 directions
+    float nxt[9];
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) xin[wave][j] = v[kShiftedDirs[j]][0];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 9; ++j) nxt[j] = __shfl(v[kShiftedDirs[j]][0], (lane + 1) & 63, 64);
+    if (lane == 63) {
+        const int src = (wave + 1 == W) ? 0 : wave + 1;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) nxt[j] = xin[src][j];
+    }
+
+    float carry[9];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        float fl[27];
+#pragma unroll
+        for (int d = 0; d < 27; ++d) {
+            if (node_sel(d) & 1) fl[d] = (i + 1 < V) ? v[d][(i + 1) % V] : nxt[shifted_index(d)];
+            else fl[d] = v[d][i];
+        }
+        bool fluid = true;
+        if (MASK) fluid = a.mask[rows[0] + i] != 0;
+        if (fluid) {
+            MacroOut mo;
+            k17_collide(fl, r, mo);
+            if (MACRO) {
+                a.rho[rows[0] + i] = mo.drho;
+                a.vx[rows[0] + i] = mo.vx;
+                a.vy[rows[0] + i] = mo.vy;
+                a.vz[rows[0] + i] = mo.vz;
+            }
+        } else {
+            // solid node: write back exactly what was read (post[inv d] := pre[d])
+#pragma unroll
+            for (int d = 1; d < 27; ++d) {
+                if (d < inv(d)) {
+                    const float t = fl[d];
+                    fl[d] = fl[inv(d)];
+                    fl[inv(d)] = t;
+                }
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 27; ++d) {
+            const float val = fl[inv(d)];
+            if (node_sel(d) & 1) {
+                if (i + 1 < V) v[d][(i + 1) % V] = val;
+                else carry[shifted_index(d)] = val;
+            } else {
+                v[d][i] = val;
+            }
+        }
+    }
+
+    // directions without an x-shift are complete: store them before the exchange
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        if (node_sel(d) & 1) continue;
+        char* base = reinterpret_cast<char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
+        RowIO<S, V>::store(reinterpret_cast<S*>(base + boff[node_sel(d) >> 1]), v[d]);
+    }
+
+    // element before this lane's first node, for the c_x = -1 directions
+    if (lane == 63) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) xout[wave][j] = carry[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        float s0 = __shfl(carry[j], (lane + 63) & 63, 64);
+        if (lane == 0) s0 = xout[(wave == 0) ? W - 1 : wave - 1][j];
+        const int d = kShiftedDirs[j];
+        v[d][0] = s0;
+        char* base = reinterpret_cast<char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
+        RowIO<S, V>::store(reinterpret_cast<S*>(base + boff[node_sel(d) >> 1]), v[d]);
+    }
+}
+
 struct IndexedArgs {
     uint64_t N;
     const uint32_t* nbx;
@@ -176,15 +303,46 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
 }
 
 // ---- host launchers ----------------------------------------------------------------------
+// Width (nodes per lane) of the row kernel per storage type; 0 selects the scalar kernel.
+// Measured on MI355X at 256^3: FP32 1 node/lane (5.8 TB/s vs 4.9 scalar); FP16 is bound by
+// VALU issue, where the scalar kernel is marginally ahead; BF16 2 nodes/lane.
+// VFAMD_ROW_WIDTH overrides the choice (0, 1, 2 or 4) for benchmarking.
+template <class S> static int row_width()
+{
+    const char* e = std::getenv("VFAMD_ROW_WIDTH");
+    if (e && *e) return std::atoi(e);
+    if (Codec<S>::storage == VF_STORAGE_FP32) return 1;
+    if (Codec<S>::storage == VF_STORAGE_BF16) return 2;
+    return 0;
+}
+
+template <class S, int V, bool EVEN>
+static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, hipStream_t s)
+{
+    dim3 grid(a.ny, a.nz);
+    dim3 block(a.nx / V);
+    if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true>), grid, block, 0, s, f, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false>), grid, block, 0, s, f, a, r);
+    else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true>), grid, block, 0, s, f, a, r);
+    else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false>), grid, block, 0, s, f, a, r);
+}
+
 template <class S, bool EVEN>
 static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
 {
     BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, p.rho, p.velocityX, p.velocityY, p.velocityZ};
     const K17Rates r = make_k17_rates(p);
-    const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
-    dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
+    // row kernel: a row of nx nodes is nx/V lanes, whole waves, at most one workgroup
+    const int V = row_width<S>();
+    if (V > 0 && p.nx % (64 * V) == 0 && p.nx / V <= kRowMaxThreads) {
+        if (V == 1 && sizeof(S) == 4) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, s); return; }
+        if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, s); return; }
+        if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, s); return; }
+    }
+    const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
+    dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
     if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r);
     else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r);
     else if (macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, true>), grid, dim3(bx), 0, s, f, a, r);
