@@ -76,14 +76,37 @@ def test_channel_pressure_outlet_matches_oracle(gpu, orc):
 
 def test_lid_driven_cavity_64_config(gpu, orc):
     """The 64^3 lid-driven cavity: lid 0.05 in x on z-max, bounce-back no-slip on the other five
-    faces, Re = 100 (nu = 0.05*64/100 = 0.032, omega = 1/(3 nu + 1/2)), limiter 0.01, rest
-    initial state, 1000 steps; GPU FP32 against the FP64 oracle."""
+    faces, Re = 100 (nu = 0.05*64/100 = 0.032, omega = 1/(3 nu + 1/2) = 1.678), limiter 0.01,
+    rest initial state, 1000 steps; GPU FP32 against the FP64 oracle.
+
+    At this omega the reference's third-order rate OxyyMxzz is negative (pole at 14/9, zero at
+    7/4): the limiter holds the growing modes, and the run amplifies round-off. The bar is
+    therefore the reference arithmetic's own FP32 deviation: the oracle's FP32 build on the
+    same input, against the same FP64 run."""
     n, steps = 64, 1000
     grid = cavity(n, lid_velocity=0.05)
     omega = 1.0 / (3 * 0.05 * n / 100.0 + 0.5)
-    ref = oracle_run(orc, grid, omega, steps, velocity_bc="VelocityBounceBack", noslip_bc="NoSlipBounceBack")
+    kw = dict(velocity_bc="VelocityBounceBack", noslip_bc="NoSlipBounceBack")
+    ref64 = oracle_run(orc, grid, omega, steps, **kw).macroscopic(grid.geo)
+    ref32 = oracle_run(orc, grid, omega, steps, dtype=np.float32, **kw).macroscopic(grid.geo)
     L = _gpu_run(grid, omega, steps, "VelocityBounceBack", "NoSlipBounceBack")
-    e = _errors(grid, L.macroscopic(), ref.macroscopic(grid.geo), 0.05)
-    print("cavity64", e)
-    # FP32 vs FP64 over 1000 steps (the limiter is active at this omega; see the K17 rate note)
-    assert e["u_L2"] < 1e-3 and e["u_max"] < 1e-2 and e["rho_max"] < 1e-4
+    e_gpu = _errors(grid, L.macroscopic(), ref64, 0.05)
+    e_cpu32 = _errors(grid, [a.astype(np.float64) for a in ref32], ref64, 0.05)
+    print("cavity64 gpu-fp32 vs fp64", e_gpu)
+    print("cavity64 cpu-fp32 vs fp64", e_cpu32)
+    for k in ("u_L2", "u_max", "rho_L2", "rho_max"):
+        assert e_gpu[k] < 3.0 * e_cpu32[k] + 1e-6, (k, e_gpu[k], e_cpu32[k])
+    assert e_gpu["u_L2"] < 1e-2
+
+
+def test_lid_driven_cavity_64_well_posed_rates(gpu, orc):
+    """The same cavity at omega = 1.8, where every K17 rate lies in (0, 2): GPU FP32 tracks
+    the FP64 oracle to FP32 round-off over 1000 steps."""
+    n, steps = 64, 1000
+    grid = cavity(n, lid_velocity=0.05)
+    kw = dict(velocity_bc="VelocityBounceBack", noslip_bc="NoSlipBounceBack")
+    ref64 = oracle_run(orc, grid, 1.8, steps, **kw).macroscopic(grid.geo)
+    L = _gpu_run(grid, 1.8, steps, "VelocityBounceBack", "NoSlipBounceBack")
+    e = _errors(grid, L.macroscopic(), ref64, 0.05)
+    print("cavity64 omega=1.8", e)
+    assert e["u_L2"] < 2e-5 and e["u_max"] < 2e-4 and e["rho_max"] < 2e-6

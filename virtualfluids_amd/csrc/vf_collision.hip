@@ -304,16 +304,15 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
 
 // ---- host launchers ----------------------------------------------------------------------
 // Width (nodes per lane) of the row kernel per storage type; 0 selects the scalar kernel.
-// Measured on MI355X at 256^3: FP32 1 node/lane (5.8 TB/s vs 4.9 scalar); FP16 is bound by
-// VALU issue, where the scalar kernel is marginally ahead; BF16 2 nodes/lane.
+// Measured on MI355X at 256^3: FP32 1 node/lane (5.8 TB/s vs 4.9 scalar); FP16 and BF16
+// 2 nodes/lane (4-byte accesses; both close to the VALU issue limit).
 // VFAMD_ROW_WIDTH overrides the choice (0, 1, 2 or 4) for benchmarking.
 template <class S> static int row_width()
 {
     const char* e = std::getenv("VFAMD_ROW_WIDTH");
     if (e && *e) return std::atoi(e);
     if (Codec<S>::storage == VF_STORAGE_FP32) return 1;
-    if (Codec<S>::storage == VF_STORAGE_BF16) return 2;
-    return 0;
+    return 2;
 }
 
 template <class S, int V, bool EVEN>

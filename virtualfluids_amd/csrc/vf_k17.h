@@ -58,21 +58,33 @@ __device__ __forceinline__ void fwd01(float& a, float& b, float c, float u)
     a = s;
     b = __builtin_fmaf(-u, sK, d);
 }
-// backward chimera line: zeroth, first, second central moment -> a,b,c
-// hu = u/2, hu2 = u^2/2; K enters as  (a + K)
+// Per-pass coefficients of the backward chimera line (u = velocity component):
+//   lo  = a0 (a+K) + b0 b + c/2,   hi = a2 (a+K) + b2 b + c/2,   mid = a - lo - hi
+// with a0 = (u^2-u)/2, b0 = u - 1/2, a2 = (u^2+u)/2, b2 = u + 1/2. The three populations
+// of a line sum to its zeroth moment a, which fixes mid exactly.
+struct BwdPass {
+    float a0, b0, a2, b2;
+};
+__device__ __forceinline__ BwdPass bwd_pass(float u, float u2)
+{
+    BwdPass p;
+    p.a0 = 0.5f * (u2 - u);
+    p.b0 = u - 0.5f;
+    p.a2 = 0.5f * (u2 + u);
+    p.b2 = u + 0.5f;
+    return p;
+}
 template <int KNUM, int KDEN>
-__device__ __forceinline__ void bwd(float& a, float& b, float& c, float u, float hu, float hu2)
+__device__ __forceinline__ void bwd(float& a, float& b, float& c, const BwdPass& p)
 {
     constexpr float K = float(KNUM) / float(KDEN);
-    const float th = KNUM ? __builtin_fmaf(a, hu2, K * hu2) : a * hu2; // (a+K) u^2 / 2
-    const float sh = KNUM ? __builtin_fmaf(a, hu, K * hu) : a * hu;    // (a+K) u / 2
-    const float bu = b * u;
-    const float P = __builtin_fmaf(0.5f, c, th) + bu;
-    const float Q = __builtin_fmaf(0.5f, b, sh);
-    const float mid = __builtin_fmaf(-2.0f, bu + th, a - c);
-    a = P - Q;
+    const float hc = 0.5f * c;
+    const float lo = __builtin_fmaf(p.a0, a, __builtin_fmaf(p.b0, b, KNUM ? hc + K * p.a0 : hc));
+    const float hi = __builtin_fmaf(p.a2, a, __builtin_fmaf(p.b2, b, KNUM ? hc + K * p.a2 : hc));
+    const float mid = (a - lo) - hi;
+    a = lo;
     b = mid;
-    c = P + Q;
+    c = hi;
 }
 
 struct MacroOut {
@@ -92,27 +104,31 @@ __device__ __forceinline__ void k17_collide(float (&f)[27], const K17Rates& r, M
     float& fM0P = f[dM0P]; float& f00P = f[d00P]; float& fP0P = f[dP0P];
     float& fMPP = f[dMPP]; float& f0PP = f[d0PP]; float& fPPP = f[dPPP];
 
-    // ---- density and velocity: pairwise sums of opposite directions --------------------
-    const float sPPP = fPPP + fMMM, dPPP_ = fPPP - fMMM;
-    const float sPMP = fPMP + fMPM, dPMP_ = fPMP - fMPM;
-    const float sMPP = fMPP + fPMM, dMPP_ = fMPP - fPMM;
-    const float sPPM = fPPM + fMMP, dPPM_ = fPPM - fMMP;
-    const float sP0P = fP0P + fM0M, dP0P_ = fP0P - fM0M;
-    const float sP0M = fP0M + fM0P, dP0M_ = fP0M - fM0P;
-    const float sPP0 = fPP0 + fMM0, dPP0_ = fPP0 - fMM0;
-    const float sPM0 = fPM0 + fMP0, dPM0_ = fPM0 - fMP0;
-    const float s0PP = f0PP + f0MM, d0PP_ = f0PP - f0MM;
-    const float s0PM = f0PM + f0MP, d0PM_ = f0PM - f0MP;
-    const float sP00 = fP00 + fM00, dP00_ = fP00 - fM00;
-    const float s0P0 = f0P0 + f0M0, d0P0_ = f0P0 - f0M0;
-    const float s00P = f00P + f00M, d00P_ = f00P - f00M;
-
-    const float drho = ((((sPPP + sPMP) + (sMPP + sPPM)) +
-                         (((s0PP + s0PM) + (sP0P + sP0M)) + (sPP0 + sPM0))) + f000) +
-                       ((sP00 + s0P0) + s00P);
-    const float jx = (((dPPP_ + dPMP_) - (dMPP_ - dPPM_)) + ((dP0M_ + dP0P_) + (dPM0_ + dPP0_))) + dP00_;
-    const float jy = (((dPPP_ - dPMP_) + (dMPP_ + dPPM_)) + ((d0PM_ + d0PP_) + (dPP0_ - dPM0_))) + d0P0_;
-    const float jz = (((dPPP_ + dPMP_) + (dMPP_ - dPPM_)) + ((d0PP_ - d0PM_) + (dP0P_ - dP0M_))) + d00P_;
+    // ---- z pass, velocity-free half: per (x,y) line e = a+c, d = c-a, s = e+b ---------------
+    // The nine line sums s carry the density and the x/y momenta, the nine differences d the
+    // z momentum, so the macroscopic values come out of the first chimera pass.
+    float eMM, dMM, sMM, eM0, dM0, sM0, eMP, dMP, sMP;
+    float e0M, d0M, s0M, e00, d00, s00, e0P, d0P, s0P;
+    float ePM, dPM, sPM, eP0, dP0, sP0, ePP, dPP, sPP;
+    auto zsum = [](float a, float b, float c, float& e, float& d, float& s) {
+        e = a + c;
+        d = c - a;
+        s = e + b;
+    };
+    zsum(fMMM, fMM0, fMMP, eMM, dMM, sMM);
+    zsum(fM0M, fM00, fM0P, eM0, dM0, sM0);
+    zsum(fMPM, fMP0, fMPP, eMP, dMP, sMP);
+    zsum(f0MM, f0M0, f0MP, e0M, d0M, s0M);
+    zsum(f00M, f000, f00P, e00, d00, s00);
+    zsum(f0PM, f0P0, f0PP, e0P, d0P, s0P);
+    zsum(fPMM, fPM0, fPMP, ePM, dPM, sPM);
+    zsum(fP0M, fP00, fP0P, eP0, dP0, sP0);
+    zsum(fPPM, fPP0, fPPP, ePP, dPP, sPP);
+    const float colM = (sMM + sMP) + sM0, colP = (sPM + sPP) + sP0, col0 = (s0M + s0P) + s00;
+    const float drho = (colM + colP) + col0;
+    const float jx = colP - colM;
+    const float jy = ((sMP + sPP) + s0P) - ((sMM + sPM) + s0M);
+    const float jz = (((dMM + dMP) + (dPM + dPP)) + ((dM0 + dP0) + (d0M + d0P))) + d00;
 
     const float oneOverRho = fast_rcp(drho + 1.0f);
     const float vx = __builtin_fmaf(jx, oneOverRho, r.fx);
@@ -120,18 +136,24 @@ __device__ __forceinline__ void k17_collide(float (&f)[27], const K17Rates& r, M
     const float vz = __builtin_fmaf(jz, oneOverRho, r.fz);
     const float vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
 
-    // ---- forward chimera: z, y, x ---------------------------------------------------------
+    // ---- z pass, velocity half: b = d - (s+K) u, c = e - 2 u d + (s+K) u^2 ------------------
     {
         const float m2u = -2.0f * vz;
-        fwd<1, 36>(fMMM, fMM0, fMMP, vz, vz2, m2u);
-        fwd<1, 9>(fM0M, fM00, fM0P, vz, vz2, m2u);
-        fwd<1, 36>(fMPM, fMP0, fMPP, vz, vz2, m2u);
-        fwd<1, 9>(f0MM, f0M0, f0MP, vz, vz2, m2u);
-        fwd<4, 9>(f00M, f000, f00P, vz, vz2, m2u);
-        fwd<1, 9>(f0PM, f0P0, f0PP, vz, vz2, m2u);
-        fwd<1, 36>(fPMM, fPM0, fPMP, vz, vz2, m2u);
-        fwd<1, 9>(fP0M, fP00, fP0P, vz, vz2, m2u);
-        fwd<1, 36>(fPPM, fPP0, fPPP, vz, vz2, m2u);
+        auto zfin = [&](float& a, float& b, float& c, float e, float d, float sv, float K) {
+            const float sK = sv + K;
+            a = sv;
+            b = __builtin_fmaf(-vz, sK, d);
+            c = __builtin_fmaf(m2u, d, __builtin_fmaf(sK, vz2, e));
+        };
+        zfin(fMMM, fMM0, fMMP, eMM, dMM, sMM, 1.0f / 36.0f);
+        zfin(fM0M, fM00, fM0P, eM0, dM0, sM0, 1.0f / 9.0f);
+        zfin(fMPM, fMP0, fMPP, eMP, dMP, sMP, 1.0f / 36.0f);
+        zfin(f0MM, f0M0, f0MP, e0M, d0M, s0M, 1.0f / 9.0f);
+        zfin(f00M, f000, f00P, e00, d00, s00, 4.0f / 9.0f);
+        zfin(f0PM, f0P0, f0PP, e0P, d0P, s0P, 1.0f / 9.0f);
+        zfin(fPMM, fPM0, fPMP, ePM, dPM, sPM, 1.0f / 36.0f);
+        zfin(fP0M, fP00, fP0P, eP0, dP0, sP0, 1.0f / 9.0f);
+        zfin(fPPM, fPP0, fPPP, ePP, dPP, sPP, 1.0f / 36.0f);
     }
     {
         const float m2u = -2.0f * vy;
@@ -247,40 +269,40 @@ __device__ __forceinline__ void k17_collide(float (&f)[27], const K17Rates& r, M
     float M012 = n012, M112 = n112, M212 = n212;
     float M022 = n022, M122 = n122, M222 = n222;
     {
-        const float hu = 0.5f * vx, hu2 = 0.5f * vx2;
-        bwd<1, 1>(M000, M100, M200, vx, hu, hu2);
-        bwd<0, 1>(M010, M110, M210, vx, hu, hu2);
-        bwd<1, 3>(M020, M120, M220, vx, hu, hu2);
-        bwd<0, 1>(M001, M101, M201, vx, hu, hu2);
-        bwd<0, 1>(M011, M111, M211, vx, hu, hu2);
-        bwd<0, 1>(M021, M121, M221, vx, hu, hu2);
-        bwd<1, 3>(M002, M102, M202, vx, hu, hu2);
-        bwd<0, 1>(M012, M112, M212, vx, hu, hu2);
-        bwd<1, 9>(M022, M122, M222, vx, hu, hu2);
+        const BwdPass p = bwd_pass(vx, vx2);
+        bwd<1, 1>(M000, M100, M200, p);
+        bwd<0, 1>(M010, M110, M210, p);
+        bwd<1, 3>(M020, M120, M220, p);
+        bwd<0, 1>(M001, M101, M201, p);
+        bwd<0, 1>(M011, M111, M211, p);
+        bwd<0, 1>(M021, M121, M221, p);
+        bwd<1, 3>(M002, M102, M202, p);
+        bwd<0, 1>(M012, M112, M212, p);
+        bwd<1, 9>(M022, M122, M222, p);
     }
     {
-        const float hu = 0.5f * vy, hu2 = 0.5f * vy2;
-        bwd<1, 6>(M000, M010, M020, vy, hu, hu2);
-        bwd<0, 1>(M001, M011, M021, vy, hu, hu2);
-        bwd<1, 18>(M002, M012, M022, vy, hu, hu2);
-        bwd<2, 3>(M100, M110, M120, vy, hu, hu2);
-        bwd<0, 1>(M101, M111, M121, vy, hu, hu2);
-        bwd<2, 9>(M102, M112, M122, vy, hu, hu2);
-        bwd<1, 6>(M200, M210, M220, vy, hu, hu2);
-        bwd<0, 1>(M201, M211, M221, vy, hu, hu2);
-        bwd<1, 18>(M202, M212, M222, vy, hu, hu2);
+        const BwdPass p = bwd_pass(vy, vy2);
+        bwd<1, 6>(M000, M010, M020, p);
+        bwd<0, 1>(M001, M011, M021, p);
+        bwd<1, 18>(M002, M012, M022, p);
+        bwd<2, 3>(M100, M110, M120, p);
+        bwd<0, 1>(M101, M111, M121, p);
+        bwd<2, 9>(M102, M112, M122, p);
+        bwd<1, 6>(M200, M210, M220, p);
+        bwd<0, 1>(M201, M211, M221, p);
+        bwd<1, 18>(M202, M212, M222, p);
     }
     {
-        const float hu = 0.5f * vz, hu2 = 0.5f * vz2;
-        bwd<1, 36>(M000, M001, M002, vz, hu, hu2);
-        bwd<1, 9>(M010, M011, M012, vz, hu, hu2);
-        bwd<1, 36>(M020, M021, M022, vz, hu, hu2);
-        bwd<1, 9>(M100, M101, M102, vz, hu, hu2);
-        bwd<4, 9>(M110, M111, M112, vz, hu, hu2);
-        bwd<1, 9>(M120, M121, M122, vz, hu, hu2);
-        bwd<1, 36>(M200, M201, M202, vz, hu, hu2);
-        bwd<1, 9>(M210, M211, M212, vz, hu, hu2);
-        bwd<1, 36>(M220, M221, M222, vz, hu, hu2);
+        const BwdPass p = bwd_pass(vz, vz2);
+        bwd<1, 36>(M000, M001, M002, p);
+        bwd<1, 9>(M010, M011, M012, p);
+        bwd<1, 36>(M020, M021, M022, p);
+        bwd<1, 9>(M100, M101, M102, p);
+        bwd<4, 9>(M110, M111, M112, p);
+        bwd<1, 9>(M120, M121, M122, p);
+        bwd<1, 36>(M200, M201, M202, p);
+        bwd<1, 9>(M210, M211, M212, p);
+        bwd<1, 36>(M220, M221, M222, p);
     }
     // moment m_abc is the population of direction (a-1, b-1, c-1)
     fMMM = M000; f0MM = M100; fPMM = M200;
