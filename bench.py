@@ -114,7 +114,7 @@ def cpu_baseline(target_s=12.0):
 
 def traffic_from_profile():
     """HBM bytes per launch from the committed rocprofv3 PMC summary, if present."""
-    path = os.path.join(ROOT, "profiles", "k17_box_fp32_256_pmc.md")
+    path = os.path.join(ROOT, "profiles", "k17_row_fp32_256_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
