@@ -22,10 +22,14 @@
 //   InitNavierStokesCompressible src/gpu/core/PreProcessor/PreProcessorStrategy/InitNavierStokesCompressible/InitNavierStokesCompressible_Device.cu:44-204
 //   macroscopic output          src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237
 //   step ordering               src/gpu/core/Calculation/UpdateGrid27.cpp:54-96 (driven from Python)
+//   refinement cell gather      src/gpu/core/Utilities/ScalingUtilities.h:53-241
+//   coarse -> fine kernel       src/gpu/core/GridScaling/scaleCoarseToFineCompressible.cu:46-528
+//   fine -> coarse kernel       src/gpu/core/GridScaling/scaleFineToCoarseCompressible.cu:43-124
 #include <cstdint>
 #include <cstring>
 
 #include "oracle_k17.h"
+#include "oracle_refine.h"
 
 namespace orc {
 
@@ -328,6 +332,78 @@ void macroscopic(const Lattice<T>& L, const u32* geo, bool even, T* rho, T* vx, 
     }
 }
 
+// ---- grid refinement ------------------------------------------------------------------------
+// The 8 nodes of the cell whose lowest corner is `base`: n = ix + 2 iy + 4 iz.
+template <class T> inline void cell_corner_nodes(const Lattice<T>& L, u32 base, u32 out[8])
+{
+    for (int n = 0; n < 8; ++n) {
+        u32 k = base;
+        if (n & 1) k = L.nbx[k];
+        if (n & 2) k = L.nby[k];
+        if (n & 4) k = L.nbz[k];
+        out[n] = k;
+    }
+}
+
+template <class T>
+Coefficients<T> cell_coefficients(const Lattice<T>& L, bool even, u32 base, T omega, T xo, T yo, T zo)
+{
+    T* P[27];
+    slot_table(L, even, P);
+    u32 corners[8];
+    cell_corner_nodes(L, base, corners);
+    SourceMoments<T> m[8];
+    for (int n = 0; n < 8; ++n) {
+        u32 nn[8];
+        cell_nodes(L, corners[n], nn);
+        T f[27];
+        read_pre(P, nn, f);
+        m[n] = source_moments(f, omega);
+    }
+    return coefficients(m, xo, yo, zo);
+}
+
+// coarse -> fine: coarse read with its parity, the 8 fine nodes written with parity even
+template <class T>
+void scale_cf(const Lattice<T>& C, bool evenC, const Lattice<T>& F, const u32* cellsC, const u32* cellsF, u64 n,
+              const T* ox, const T* oy, const T* oz, T omegaC, T omegaF)
+{
+    T* PF[27];
+    slot_table(F, true, PF);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        const Coefficients<T> c = cell_coefficients(C, evenC, cellsC[i], omegaC, ox[i], oy[i], oz[i]);
+        u32 corners[8];
+        cell_corner_nodes(F, cellsF[i], corners);
+        for (int q = 0; q < 8; ++q) {
+            const T x = (q & 1) ? T(0.25) : -T(0.25), y = (q & 2) ? T(0.25) : -T(0.25), z = (q & 4) ? T(0.25) : -T(0.25);
+            T f[27];
+            interpolate_cf(f, omegaF, T(0.5), c, x, y, z);
+            u32 nn[8];
+            cell_nodes(F, corners[q], nn);
+            for (int d = 0; d < 27; ++d) write_pre(PF, nn, d, f[d]);
+        }
+    }
+}
+
+// fine -> coarse: fine read with parity even, the coarse node written with the coarse parity
+template <class T>
+void scale_fc(const Lattice<T>& C, bool evenC, const Lattice<T>& F, const u32* cellsC, const u32* cellsF, u64 n,
+              const T* ox, const T* oy, const T* oz, T omegaC, T omegaF)
+{
+    T* PC[27];
+    slot_table(C, evenC, PC);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        const Coefficients<T> c = cell_coefficients(F, true, cellsF[i], omegaF, ox[i], oy[i], oz[i]);
+        T f[27];
+        interpolate_fc(f, T(2), omegaC, c);
+        u32 nn[8];
+        cell_nodes(C, cellsC[i], nn);
+        for (int d = 0; d < 27; ++d) write_pre(PC, nn, d, f[d]);
+    }
+}
+
 } // namespace orc
 
 // =============================================================================================
@@ -416,6 +492,37 @@ template <class T> K17Param<T> make_param(T omega, const T* lim3, const T* force
 
 ORC_INSTANTIATE(double, f64)
 ORC_INSTANTIATE(float, f32)
+
+#define ORC_SCALE(T, SFX)                                                                                         \
+    extern "C" void orc_scale_##SFX(int fine_to_coarse, T* fc, unsigned long long Nc, const uint32_t* nbxc,        \
+                                    const uint32_t* nbyc, const uint32_t* nbzc, T* ff, unsigned long long Nf,    \
+                                    const uint32_t* nbxf, const uint32_t* nbyf, const uint32_t* nbzf, int evenC, \
+                                    const uint32_t* cellsC, const uint32_t* cellsF, unsigned long long n,         \
+                                    const T* ox, const T* oy, const T* oz, T omegaC, T omegaF)                    \
+    {                                                                                                             \
+        Lattice<T> C{fc, Nc, nbxc, nbyc, nbzc};                                                                   \
+        Lattice<T> F{ff, Nf, nbxf, nbyf, nbzf};                                                                   \
+        if (fine_to_coarse) scale_fc(C, evenC != 0, F, cellsC, cellsF, n, ox, oy, oz, omegaC, omegaF);          \
+        else scale_cf(C, evenC != 0, F, cellsC, cellsF, n, ox, oy, oz, omegaC, omegaF);                          \
+    }
+ORC_SCALE(double, f64)
+ORC_SCALE(float, f32)
+
+// One interpolation cell from 8 given source distributions (node-major [8][27], n = ix + 2 iy + 4 iz):
+// coefficients (42 values, order of Coefficients<T>) and the rebuilt node (mode 0: coarse -> fine at
+// (x, y, z) with eps and omegaDst = omega_fine; mode 1: fine -> coarse with eps and omegaDst = omega_coarse).
+extern "C" void orc_interp_cell_f64(const double* f8, double omegaSrc, double xo, double yo, double zo, int mode,
+                                    double omegaDst, double eps, double x, double y, double z, double* coeff42,
+                                    double* fout27)
+{
+    SourceMoments<double> m[8];
+    for (int n = 0; n < 8; ++n) m[n] = source_moments(f8 + 27 * n, omegaSrc);
+    const Coefficients<double> c = coefficients(m, xo, yo, zo);
+    static_assert(sizeof(Coefficients<double>) == 42 * sizeof(double), "coefficient layout");
+    std::memcpy(coeff42, &c, sizeof(c));
+    if (mode == 0) interpolate_cf(fout27, omegaDst, eps, c, x, y, z);
+    else interpolate_fc(fout27, eps, omegaDst, c);
+}
 
 // Single-line chimera transforms and moments, for the reference's known-answer tests.
 // kind: 0 forward-with-K, 1 backward-with-K, 2 forward, 3 backward.

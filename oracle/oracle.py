@@ -200,3 +200,39 @@ def ref_moments(f, precision="f64"):
     out = np.empty(9, dtype=dtype)
     r.ref_moments(_p(f), _p(out))
     return out
+
+
+# ---- grid refinement --------------------------------------------------------------------------
+COEFF_NAMES = ("a000 a100 a010 a001 a200 a020 a002 a110 a101 a011 "
+               "b000 b100 b010 b001 b200 b020 b002 b110 b101 b011 "
+               "c000 c100 c010 c001 c200 c020 c002 c110 c101 c011 "
+               "d000 d100 d010 d001 d110 d101 d011 a111 b111 c111 d111 LaplaceRho").split()
+
+
+def interp_cell(f8, omega_src, offset=(0.0, 0.0, 0.0), mode=0, omega_dst=1.0, eps=0.5, pos=(0.0, 0.0, 0.0),
+                lib_handle=None, fn="orc_interp_cell_f64"):
+    """One refinement cell: f8 node-major (8, 27), n = ix + 2 iy + 4 iz. Returns (coeffs[42], f[27])."""
+    h = lib_handle or lib()
+    f8 = np.ascontiguousarray(f8, dtype=np.float64)
+    c = np.empty(42)
+    out = np.empty(27)
+    D = ctypes.c_double
+    getattr(h, fn)(_p(f8), D(omega_src), D(offset[0]), D(offset[1]), D(offset[2]), ctypes.c_int(mode),
+                   D(omega_dst), D(eps), D(pos[0]), D(pos[1]), D(pos[2]), _p(c), _p(out))
+    return c, out
+
+
+def ref_interp_cell(f8, omega_src, offset=(0.0, 0.0, 0.0), mode=0, omega_dst=1.0, eps=0.5, pos=(0.0, 0.0, 0.0)):
+    return interp_cell(f8, omega_src, offset, mode, omega_dst, eps, pos, lib_handle=ref("f64"), fn="ref_interp_cell")
+
+
+def scale(fine_to_coarse, coarse, fine, even_coarse, cells_coarse, cells_fine, offsets, omega_c, omega_f):
+    """orc_scale on two oracle Lattices (same dtype); offsets = (ox, oy, oz) arrays per cell."""
+    assert coarse.dtype == fine.dtype
+    T = coarse._T
+    cc = np.ascontiguousarray(cells_coarse, dtype=np.uint32)
+    cf = np.ascontiguousarray(cells_fine, dtype=np.uint32)
+    ox, oy, oz = (np.ascontiguousarray(o, dtype=coarse.dtype) for o in offsets)
+    getattr(lib(), "orc_scale_" + _sfx(coarse.dtype))(
+        ctypes.c_int(1 if fine_to_coarse else 0), *coarse._nb(), *fine._nb(), ctypes.c_int(1 if even_coarse else 0),
+        _p(cc), _p(cf), ctypes.c_ulonglong(cc.size), _p(ox), _p(oy), _p(oz), T(omega_c), T(omega_f))

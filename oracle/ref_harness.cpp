@@ -65,3 +65,28 @@ extern "C" void ref_moments(const real* f, real* out9)
     out9[7] = uz;
     out9[8] = vf::lbm::getPressure(f, d, ux, uy, uz);
 }
+
+// One interpolation cell through the reference's MomentsOnSourceNodeSet, calculateCoefficients,
+// interpolateCF / interpolateFC. f8: node-major [8][27] with n = ix + 2 iy + 4 iz (M = 0, P = 1).
+// coeff42 follows the member order of InterpolationCoefficients.
+extern "C" void ref_interp_cell(const real* f8, real omegaSrc, real xo, real yo, real zo, int mode, real omegaDst,
+                                real eps, real x, real y, real z, real* coeff42, real* fout27)
+{
+    vf::lbm::MomentsOnSourceNodeSet set;
+    set.calculateMMM(f8 + 27 * 0, omegaSrc);
+    set.calculatePMM(f8 + 27 * 1, omegaSrc);
+    set.calculateMPM(f8 + 27 * 2, omegaSrc);
+    set.calculatePPM(f8 + 27 * 3, omegaSrc);
+    set.calculateMMP(f8 + 27 * 4, omegaSrc);
+    set.calculatePMP(f8 + 27 * 5, omegaSrc);
+    set.calculateMPP(f8 + 27 * 6, omegaSrc);
+    set.calculatePPP(f8 + 27 * 7, omegaSrc);
+    vf::lbm::InterpolationCoefficients c;
+    set.calculateCoefficients(c, xo, yo, zo);
+    static_assert(sizeof(c) == 42 * sizeof(real), "coefficient layout");
+    std::memcpy(coeff42, &c, sizeof(c));
+    if (mode == 0)
+        vf::lbm::interpolateCF(fout27, omegaDst, eps, c, x, y, z);
+    else
+        vf::lbm::interpolateFC(fout27, eps, omegaDst, c);
+}

@@ -52,6 +52,24 @@ def main():
         f32, m32 = oracle.ref_k17_nodes(fin.astype(np.float32), om, lims, precision="f32")
         out[f"fout64_{i}"], out[f"macro64_{i}"] = f64, m64
         out[f"fout32_{i}"], out[f"macro32_{i}"] = f32, m32
+    # refinement cells: 8 source nodes each, with and without wall offsets, CF at all 8 fine
+    # positions and FC at the centre
+    cells = near_equilibrium(rng, 8 * 24, 0.04, 3e-3).reshape(24, 8, 27)
+    offs = np.array([(0.0, 0.0, 0.0)] * 12 + [tuple(rng.choice([-0.5, 0.0, 0.5], 3)) for _ in range(12)])
+    cf_pos = np.array([((q & 1) - 0.5) / 2 for q in range(8)]), np.array([((q >> 1 & 1) - 0.5) / 2 for q in range(8)]), \
+        np.array([((q >> 2 & 1) - 0.5) / 2 for q in range(8)])
+    coef, fcf, ffc = [], [], []
+    for i in range(24):
+        row = []
+        for q in range(8):
+            c, f = oracle.ref_interp_cell(cells[i], 1.7, offs[i], 0, 1.85, 0.5, (cf_pos[0][q], cf_pos[1][q], cf_pos[2][q]))
+            row.append(f)
+        coef.append(c)
+        fcf.append(row)
+        c2, f2 = oracle.ref_interp_cell(cells[i], 1.85, offs[i], 1, 1.7, 2.0)
+        ffc.append(f2)
+    out["refine_cells"], out["refine_offsets"] = cells, offs
+    out["refine_coeffs"], out["refine_cf"], out["refine_fc"] = np.array(coef), np.array(fcf), np.array(ffc)
     mom = np.stack([oracle.ref_moments(f) for f in fin[:32]])
     out["moments_in"] = fin[:32]
     out["moments64"] = mom

@@ -13,6 +13,17 @@ def equilibrium(drho, u):
     return W * (drho[:, None] + (1 + drho[:, None]) * (3 * cu + 4.5 * cu ** 2 - 1.5 * usq))
 
 
+def product_equilibrium(drho, u):
+    """Shifted equilibrium of the cumulant collision: a product of 1D factors per axis, whose
+    central moments are the ones K17 relaxes to (its collision leaves it unchanged)."""
+    def phi(c, v):
+        return np.where(c == 0, 2.0 / 3.0 - v * v, (1.0 / 3.0 + v * v + c * v) / 2.0)
+
+    u = np.atleast_2d(u)
+    rho = 1.0 + np.atleast_1d(drho)[:, None]
+    return rho * phi(CX, u[:, :1]) * phi(CY, u[:, 1:2]) * phi(CZ, u[:, 2:3]) - W
+
+
 def near_equilibrium(rng, n, u_amp=0.03, noise=1e-3, rho_amp=0.01):
     drho = rng.normal(0.0, rho_amp, n)
     u = rng.normal(0.0, u_amp, (n, 3))
@@ -91,3 +102,38 @@ def oracle_run(orc, grid, omega, steps, velocity_bc=None, noslip_bc=None, pressu
         if pressure and pr is not None:
             L.bc_pressure(pr.k, pr.kN, pr.rho_bc, pr.direction)
     return L
+
+
+def oracle_multilevel(orc, nb, omega_c, omega_f, steps, init_c, init_f, limiters=(0.01, 0.01, 0.01),
+                      dtype=np.float64):
+    """Two levels of a NestedBox on the oracle, one coarse step = two fine steps, the coarse
+    step, fine->coarse, then coarse->fine (UpdateGrid27::updateGrid)."""
+    C = orc.Lattice(nb.coarse.nbx, nb.coarse.nby, nb.coarse.nbz, dtype=dtype)
+    F = orc.Lattice(nb.fine.nbx, nb.fine.nby, nb.fine.nbz, dtype=dtype)
+    for L, g, init in ((C, nb.coarse, init_c), (F, nb.fine, init_f)):
+        if isinstance(init, np.ndarray):  # pre-collision populations (N, 27)
+            L.f[:] = pack_pre(init, g, True)
+        else:
+            L.init_eq(*init, geo=g.geo)
+    zc = [np.zeros(nb.cf.n)] * 3
+    zf = [np.zeros(nb.fc.n)] * 3
+    for _ in range(steps):
+        for _ in range(2):
+            F.collide(nb.fine.fluid, omega_f, limiters)
+            F.swap()
+        C.collide(nb.coarse.fluid, omega_c, limiters)
+        C.swap()
+        orc.scale(True, C, F, C.even, nb.fc.coarse, nb.fc.fine, zf, omega_c, omega_f)
+        orc.scale(False, C, F, C.even, nb.cf.coarse, nb.cf.fine, zc, omega_c, omega_f)
+    return C, F
+
+
+def shear_wave_init(nb, amp):
+    """u_x = amp sin(2 pi z / Lz) on both levels of a NestedBox (coarse lattice units)."""
+    k = 2 * np.pi / nb.coarse.nz
+    _, _, zc = nb.coarse.coords()
+    _, _, zf = nb.fine_coords()
+    Nc, Nf = nb.coarse.N, nb.fine.N
+    init_c = (np.zeros(Nc), amp * np.sin(k * zc), np.zeros(Nc), np.zeros(Nc))
+    init_f = (np.zeros(Nf), amp * np.sin(k * zf), np.zeros(Nf), np.zeros(Nf))
+    return k, init_c, init_f
