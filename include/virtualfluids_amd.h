@@ -170,6 +170,32 @@ int vf_PressureNonEquilibriumCompressible(const vf_lbm_parameter* para, const vf
                                           void* stream);
 
 /* ---------------------------------------------------------------------------------
+ * Grid refinement: the launchers GridScalingFactory hands out for
+ * GridScaling::ScaleCompressible (src/gpu/core/GridScaling/GridScalingFactory.cpp:43-69),
+ * whose signature is gridScaling(LBMSimulationParameter* coarse, LBMSimulationParameter* fine,
+ * ICells*, ICellNeigh&, stream) (src/gpu/core/GridScaling/GridScalingFactory.h:46). Coarse and fine levels must share the
+ * storage type; the coarse level's isEvenTimestep selects the coarse parity, the fine level is
+ * read/written with the even parity (it has just completed two steps).
+ * offsets may be NULL (all cells centred) or hold per-cell x/y/z offsets.
+ * --------------------------------------------------------------------------------- */
+
+/* scaleCoarseToFineCompressible<false> (src/gpu/core/GridScaling/Scaling.cu:105-138,
+ * kernel src/gpu/core/GridScaling/scaleCoarseToFineCompressible.cu:305-528): for every cell
+ * i, the 8 coarse nodes of the cell at coarseCellIndices[i] give the interpolation
+ * polynomials; the 8 fine nodes of the cell at fineCellIndices[i] are overwritten. */
+int vf_ScaleCoarseToFineCompressible(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                     const vf_interpolation_cells* coarseToFine,
+                                     const vf_interpolation_cell_neighbor* offsets, void* stream);
+
+/* scaleFineToCoarseCompressible<false> (Scaling.cu:155-188, kernel
+ * src/gpu/core/GridScaling/scaleFineToCoarseCompressible.cu:76-129): the 8 fine nodes of the
+ * cell at fineCellIndices[i] give the polynomials; coarse node coarseCellIndices[i] is
+ * overwritten. */
+int vf_ScaleFineToCoarseCompressible(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                     const vf_interpolation_cells* fineToCoarse,
+                                     const vf_interpolation_cell_neighbor* offsets, void* stream);
+
+/* ---------------------------------------------------------------------------------
  * Initialisation, output, conservation
  * --------------------------------------------------------------------------------- */
 

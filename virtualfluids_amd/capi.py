@@ -109,6 +109,8 @@ SIGNATURES = {
     "vf_CalculateMacroscopicQuantitiesCompressible": [_P, _vp],
     "vf_ConservationSums": [_P, _vp, _vp],
     "vf_MomentSums": [_P, _vp, _vp],
+    "vf_ScaleCoarseToFineCompressible": [_P, _P, _vp, _vp, _vp],
+    "vf_ScaleFineToCoarseCompressible": [_P, _P, _vp, _vp, _vp],
 }
 
 _lib = None

@@ -113,3 +113,53 @@ class Simulation:
     def run(self, steps: int, stream=None):
         for _ in range(steps):
             self.step(stream)
+
+
+class MultiLevelSimulation:
+    """Time stepping of a level hierarchy (level 0 coarsest) with refinement interfaces.
+
+    One ``step`` is one coarse time step, following UpdateGrid27::updateGrid
+    (src/gpu/core/Calculation/UpdateGrid27.cpp:54-96): a level first lets the next finer level
+    take two steps, then collides, applies its BCs and swaps parity, and finally refreshes the
+    interface to the finer level: fine -> coarse, then coarse -> fine
+    (Refinement_noExchange, src/gpu/core/GridScaling/RefinementStrategy.cpp:169-178). The interface kernels read and
+    write the finer level with the even parity, so every finer level starts with it.
+    """
+
+    def __init__(self, lattices, interfaces, bc_factories=None, scaling_factory=None):
+        from .refinement import DeviceInterfaceCells, GridScalingKernelManager
+
+        if len(interfaces) != len(lattices) - 1:
+            raise ValueError("one (coarse-to-fine, fine-to-coarse) cell pair is needed per level pair")
+        facs = bc_factories or [None] * len(lattices)
+        self.lattices = list(lattices)
+        self.levels = [Simulation(L, f) for L, f in zip(self.lattices, facs)]
+        for L in self.lattices[1:]:
+            if not L.even:
+                raise ValueError("finer levels must start with the even parity")
+        self.cells = [(DeviceInterfaceCells(cf, L.device), DeviceInterfaceCells(fc, L.device))
+                      for (cf, fc), L in zip(interfaces, self.lattices)]
+        n_fc = self.cells[0][1].n if self.cells else 0
+        n_cf = self.cells[0][0].n if self.cells else 0
+        self.manager = GridScalingKernelManager(len(self.lattices), scaling_factory, n_fc, n_cf)
+        self.t = 0
+
+    def _update(self, level: int, stream):
+        fine = len(self.lattices) - 1
+        if level != fine:
+            self._update(level + 1, stream)
+            self._update(level + 1, stream)
+        self.levels[level].step(stream)
+        if level != fine:
+            cf, fc = self.cells[level]
+            coarse_l, fine_l = self.lattices[level], self.lattices[level + 1]
+            self.manager.run_fine_to_coarse(coarse_l, fine_l, fc, stream)
+            self.manager.run_coarse_to_fine(coarse_l, fine_l, cf, stream)
+
+    def step(self, stream=None):
+        self._update(0, stream)
+        self.t += 1
+
+    def run(self, steps: int, stream=None):
+        for _ in range(steps):
+            self.step(stream)
