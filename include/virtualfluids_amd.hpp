@@ -148,7 +148,7 @@ public:
     DeviceLevel& operator=(const DeviceLevel&) = delete;
 
     void attach(LBMSimulationParameter& p) const; /* points p at this level's memory */
-    void setBC(QforBoundaryConditions& bc, const HostBC& host);
+    void setBC(vf_q_boundary_conditions& bc, const HostBC& host); /* plain or directional BC list */
     /* cells of the interface from this (coarse) level to a finer level of fineNumberOfNodes */
     void setCells(InterpolationCells& cells, ICellNeigh& neigh, bool& hasOffsets, const HostCells& host,
                   uint64_t fineNumberOfNodes);

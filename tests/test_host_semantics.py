@@ -1,0 +1,66 @@
+"""Factory and manager semantics of the Python host layer (CPU only), mirroring the reference's
+own unit tests: BoundaryConditionFactoryTest.cpp, BCKernelManagerTest.cpp and
+GridScalingKernelManagerTest.cpp (tests/unit-tests/gpu/core/...)."""
+import types
+
+import numpy as np
+import pytest
+
+from virtualfluids_amd.grid import BoundaryNodes
+from virtualfluids_amd.refinement import GridScalingFactory, GridScalingKernelManager
+from virtualfluids_amd.simulation import BoundaryConditionFactory, Simulation
+
+F = BoundaryConditionFactory
+
+
+def test_bc_factory_defaults():
+    f = F()
+    assert f.get_velocity_boundary_condition_post() is None
+    assert f.get_no_slip_boundary_condition_post() == ""  # NoSlipDelayBounceBack: nothing to launch
+    assert f.get_pressure_boundary_condition_pre() is None
+
+
+def test_bc_factory_maps_to_c_entry_points():
+    f = F()
+    f.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedCompressible)
+    f.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipBounceBack)
+    f.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    assert f.get_velocity_boundary_condition_post() == "vf_VelocityInterpolatedCompressible"
+    assert f.get_no_slip_boundary_condition_post() == "vf_NoSlipBounceBack"
+    assert f.get_pressure_boundary_condition_pre() == "vf_PressureNonEquilibriumCompressible"
+    f.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedIncompressible)
+    with pytest.raises(NotImplementedError):
+        f.get_velocity_boundary_condition_post()
+
+
+def _fake_lattice(**counts):
+    bcs = {name: BoundaryNodes(k=np.zeros(n, dtype=np.int32), q27=np.zeros(27 * n)) for name, n in counts.items()}
+    return types.SimpleNamespace(bcs=bcs)
+
+
+def test_bc_manager_checks():
+    Simulation(_fake_lattice(velocity=0, pressure=0), F())  # no nodes: no function needed
+    Simulation(_fake_lattice(noslip=4), F())  # the no-slip default is specified
+    with pytest.raises(RuntimeError, match="velocityBoundaryConditionPost was not set"):
+        Simulation(_fake_lattice(velocity=3), F())
+    with pytest.raises(RuntimeError, match="PressureBoundaryConditionPre was not set"):
+        Simulation(_fake_lattice(pressure=3), F())
+
+
+def test_grid_scaling_manager_checks():
+    f = GridScalingFactory()
+    GridScalingKernelManager(1, None)
+    GridScalingKernelManager(1, f)
+    g = GridScalingFactory()
+    g.set_scaling_factory(GridScalingFactory.GridScaling.ScaleCompressible)
+    GridScalingKernelManager(1, g)
+    GridScalingKernelManager(3, g)
+    with pytest.raises(RuntimeError, match="no scalingFactory"):
+        GridScalingKernelManager(3, None)
+    GridScalingKernelManager(3, f)  # no interface cells: no function needed
+    with pytest.raises(RuntimeError, match="scalingFineToCoarse was not set"):
+        GridScalingKernelManager(3, f, fc_cells=100)
+    with pytest.raises(RuntimeError, match="scalingCoarseToFine was not set"):
+        GridScalingKernelManager(3, f, cf_cells=100)
+    with pytest.raises(NotImplementedError):
+        g.get_grid_scaling_fc(True)

@@ -18,6 +18,9 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvfamd.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
+ROOT = os.path.dirname(HERE)
+HOST_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_host.cpp")
+HOST_TEST = os.path.join(ROOT, "tests", "cpp", "test_host")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_FLAGS = [
@@ -56,9 +59,22 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_host_test()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources", file=sys.stderr)
+        print(f"built {LIB} from {len(srcs)} sources (and {HOST_TEST})", file=sys.stderr)
     return LIB
+
+
+def build_host_test() -> str:
+    """The C++ host-layer test program, linked against the in-tree libvfamd.so."""
+    tmp = HOST_TEST + ".tmp"
+    cmd = [HIPCC, *DEVICE_FLAGS, "-x", "hip", HOST_TEST_SRC, "-o", tmp, f"-L{HERE}", "-lvfamd",
+           "-Wl,-rpath,$ORIGIN/../../virtualfluids_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {HOST_TEST_SRC}:\n{r.stderr}")
+    os.replace(tmp, HOST_TEST)
+    return HOST_TEST
 
 
 if __name__ == "__main__":

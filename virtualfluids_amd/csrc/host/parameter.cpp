@@ -134,10 +134,11 @@ void DeviceLevel::attach(LBMSimulationParameter& p) const
     p.numberOfFluidNodes = (uint32_t)shape.fluidNodeIndices.size();
 }
 
-void DeviceLevel::setBC(QforBoundaryConditions& bc, const HostBC& host)
+void DeviceLevel::setBC(vf_q_boundary_conditions& bc, const HostBC& host)
 {
     const size_t n = host.k.size();
     if (host.q27.size() != 27 * n) throw std::invalid_argument("setBC: q27 must hold 27 entries per BC node");
+    if (!host.kN.empty() && host.kN.size() != n) throw std::invalid_argument("setBC: kN must hold one node per BC node");
     for (int32_t k : host.k)
         if (k < 0 || (uint64_t)k >= shape.numberOfNodes) throw std::invalid_argument("setBC: BC node out of range");
     for (int32_t k : host.kN)
@@ -149,7 +150,6 @@ void DeviceLevel::setBC(QforBoundaryConditions& bc, const HostBC& host)
     };
     bc.k = (const int32_t*)upload(host.k.data(), n * 4);
     bc.kN = host.kN.empty() ? nullptr : (const int32_t*)upload(host.kN.data(), n * 4);
-    if (!host.kN.empty() && host.kN.size() != n) throw std::invalid_argument("setBC: kN must hold one node per BC node");
     bc.q27 = (const float*)upload(host.q27.data(), 27 * n * 4);
     bc.numberOfBCnodes = (uint32_t)n;
     bc.Vx = opt(host.Vx, "Vx");

@@ -92,10 +92,11 @@ class Simulation:
         self.pressure_fn = f.get_pressure_boundary_condition_pre()
         # BoundaryConditionKernelManager: BC nodes without a function is an error
         # (src/gpu/core/BoundaryConditions/BoundaryConditionKernelManager.cpp:53-83)
-        for name, fn in (("velocity", self.velocity_fn), ("pressure", self.pressure_fn)):
+        for name, fn, label in (("velocity", self.velocity_fn, "velocityBoundaryConditionPost"),
+                                ("pressure", self.pressure_fn, "directionalPressureBoundaryConditionPre")):
             bc = lattice.bcs.get(name)
             if bc is not None and bc.n > 0 and fn is None:
-                raise RuntimeError(f"The {name} boundary condition is not set in the BoundaryConditionFactory")
+                raise RuntimeError(f"The boundary condition {label} was not set!")
         self.t = 0
 
     def step(self, stream=None):
