@@ -7,8 +7,9 @@ place), u0 = 0.05, Re = 1600. A step is one launch of the dense-box K17 kernel o
 
 Prints ONE JSON line: MLUPS, the kernel's HBM roofline (algorithmic bytes 2*27*4 = 216 B
 per node update, divided by the event-timed average launch duration), the FP16/BF16
-storage variants measured the same way, a measured copy bandwidth, and the CPU oracle
-timed on the host cores on a bounded sample.
+storage variants measured the same way, the adjoint step (kernel alone, and a whole
+checkpointed forward + backward through k17_steps) against the forward step, a measured
+copy bandwidth, and the CPU oracle timed on the host cores on a bounded sample.
 
     python bench.py [--gpus 1] [--steps 2000] [--warmup 20]
 """
@@ -57,6 +58,55 @@ def time_steps(L, steps, warmup):
     stop.record()
     torch.cuda.synchronize()
     return start.elapsed_time(stop) / 1000.0
+
+
+def time_adjoint(L, steps, warmup):
+    """Event-timed vf_k17_adjoint_box launches (FP32 adjoint buffer, state fixed)."""
+    import ctypes
+
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import stream_handle
+
+    lam = torch.full_like(L.f, 1e-3, dtype=torch.float32)
+    lp = ctypes.c_void_p(lam.data_ptr())
+
+    def step():
+        capi.call("vf_k17_adjoint_box", L.pref(), lp, None, stream_handle())
+        L.swap()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        step()
+    stop.record()
+    torch.cuda.synchronize()
+    del lam
+    return start.elapsed_time(stop) / 1000.0
+
+
+def time_gradient(L, steps, checkpoint_every):
+    """Wall time of k17_steps forward + backward (checkpoints, replay, adjoint) over `steps`."""
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_steps
+
+    x = L.f.detach().clone().requires_grad_(True)
+    out = k17_steps(x, L, 2, checkpoint_every=1)
+    out.backward(torch.ones_like(out))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = k17_steps(x, L, steps, checkpoint_every=checkpoint_every)
+    out.backward(torch.ones_like(out))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    del out, x
+    torch.cuda.empty_cache()
+    return dt
 
 
 def copy_bandwidth():
@@ -129,6 +179,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--variant-steps", type=int, default=500, help="timed steps of the FP16/BF16 variants")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
+    ap.add_argument("--adjoint-steps", type=int, default=200, help="timed adjoint kernel steps")
     ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
                     help="time one storage type only (for profiling runs)")
     args = ap.parse_args()
@@ -152,6 +204,7 @@ def main():
     lim = (1e6, 1e6, 1e6)  # the Taylor-Green app switches the limiter off
 
     results = {}
+    adjoint = None
     for storage in (["fp32", "fp16", "bf16"] if args.only is None else [args.only]):
         L = Lattice(grid, omega, storage=storage, limiters=lim)
         L.init_equilibrium(*fields)
@@ -163,6 +216,15 @@ def main():
                             "bytes_per_node": bytes_per_node}
         s = L.moment_sums()
         results[storage]["mass_drift"] = float(s[0])
+        if storage == "fp32" and not args.no_adjoint:
+            ta = time_adjoint(L, args.adjoint_steps, 5)
+            tg = time_gradient(L, 40, 10)
+            adjoint = {"kernel_ms_per_step": round(1e3 * ta / args.adjoint_steps, 4),
+                       "kernel_vs_forward": round((ta / args.adjoint_steps) / (t / steps), 3),
+                       "kernel_GBps": round(3 * 27 * 4 * N * args.adjoint_steps / ta / 1e9, 1),
+                       "gradient_ms_per_step": round(1e3 * tg / 40, 4),
+                       "gradient_vs_forward": round((tg / 40) / (t / steps), 3),
+                       "gradient_run": "k17_steps 40 steps, checkpoint every 10: forward, replay, adjoint"}
         del L
         torch.cuda.empty_cache()
 
@@ -193,6 +255,7 @@ def main():
                                  "vs_fp32": (round(v["mlups"] / results["fp32"]["mlups"], 3)
                                              if "fp32" in results else None)}
                              for k, v in results.items()},
+        "adjoint": adjoint,
         "hbm_copy_GBps": round(copy_gbps, 1),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }

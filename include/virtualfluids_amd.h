@@ -170,6 +170,26 @@ int vf_PressureNonEquilibriumCompressible(const vf_lbm_parameter* para, const vf
                                           void* stream);
 
 /* ---------------------------------------------------------------------------------
+ * Adjoint of the collide + stream step (no counterpart in the reference: the reverse mode
+ * behind the differentiable time step of the Python layer).
+ *
+ * para->distributions holds the populations BEFORE the forward step and para->isEvenTimestep
+ * the parity that step used (the forward step's linearisation point). adjoint is an FP32
+ * buffer of 27*numberOfNodes values in the same slot layout: on entry the adjoint of the
+ * populations after the step, on exit (in place) the adjoint of the populations before it.
+ * velocityShiftAdjoint (optional, 3*numberOfNodes floats: all x, then all y, then all z): the
+ * adjoint of each node's collision velocity shift (forces * 1/2 * forceFactor) is ADDED, so
+ * per-node force or porosity gradients accumulate over steps. Nodes that the forward step does
+ * not collide (fluidMask == 0, or not listed) are left untouched, as in the forward step.
+ * --------------------------------------------------------------------------------- */
+
+/* Reverse of vf_k17_run_box (dense periodic box, optional fluidMask). */
+int vf_k17_adjoint_box(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint, void* stream);
+
+/* Reverse of vf_k17_run (every node of fluidNodeIndices, neighbour arrays). */
+int vf_k17_adjoint(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint, void* stream);
+
+/* ---------------------------------------------------------------------------------
  * Grid refinement: the launchers GridScalingFactory hands out for
  * GridScaling::ScaleCompressible (src/gpu/core/GridScaling/GridScalingFactory.cpp:43-69),
  * whose signature is gridScaling(LBMSimulationParameter* coarse, LBMSimulationParameter* fine,

@@ -25,9 +25,11 @@
 //   refinement cell gather      src/gpu/core/Utilities/ScalingUtilities.h:53-241
 //   coarse -> fine kernel       src/gpu/core/GridScaling/scaleCoarseToFineCompressible.cu:46-528
 //   fine -> coarse kernel       src/gpu/core/GridScaling/scaleFineToCoarseCompressible.cu:43-124
+//   adjoint (no reference counterpart): forward-mode dual numbers through k17_collide
 #include <cstdint>
 #include <cstring>
 
+#include "oracle_adjoint.h"
 #include "oracle_k17.h"
 #include "oracle_refine.h"
 
@@ -551,3 +553,41 @@ extern "C" void orc_moments_f64(const double* f, double* out8)
 
 // Eso-twist slot of direction d for a parity (KernelUtilities.h:66-125): returns the array index.
 extern "C" int orc_eso_slot(int d, int even) { return even ? d : INV[d]; }
+
+// ---- adjoint of the collide+stream step (forward-mode AD of k17_collide, oracle_adjoint.h) ----
+// Per node: g_post[e] = adjoint at the cell where post[e] was written; the node's adjoint of its
+// pre-collision values goes back to the cells they were read from (the step is block-diagonal
+// in memory, so the adjoint is in place). ub3n (optional): per-node adjoint of the velocity shift.
+extern "C" void orc_k17_vjp_f64(const double* fin, const double* g, long long n, double omega, const double* lim3,
+                                const double* force3, double* fb, double* ub3n)
+{
+    const K17Param<double> p = make_param<double>(omega, lim3, force3);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i)
+        k17_vjp_dual(fin + 27 * i, g + 27 * i, p, fb + 27 * i, ub3n ? ub3n + 3 * i : nullptr);
+}
+
+extern "C" void orc_adjoint_step_f64(const double* state, double* lam, unsigned long long N, const uint32_t* nbx,
+                                     const uint32_t* nby, const uint32_t* nbz, const uint32_t* fluid,
+                                     unsigned long long nfluid, double omega, const double* lim3,
+                                     const double* force3, int even, double* ub3n)
+{
+    const K17Param<double> p = make_param<double>(omega, lim3, force3);
+    Lattice<double> S{const_cast<double*>(state), N, nbx, nby, nbz};
+    Lattice<double> A{lam, N, nbx, nby, nbz};
+    double* PS[27];
+    double* PA[27];
+    slot_table(S, even != 0, PS);
+    slot_table(A, even != 0, PA);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)nfluid; ++i) {
+        const u32 k = fluid[i];
+        u32 nn[8];
+        cell_nodes(S, k, nn);
+        double f[27], g[27], fb[27];
+        read_pre(PS, nn, f);
+        read_post(PA, nn, g);
+        k17_vjp_dual(f, g, p, fb, ub3n ? ub3n + 3 * (u64)k : nullptr);
+        for (int d = 0; d < 27; ++d) write_pre(PA, nn, d, fb[d]);
+    }
+}

@@ -236,3 +236,36 @@ def scale(fine_to_coarse, coarse, fine, even_coarse, cells_coarse, cells_fine, o
     getattr(lib(), "orc_scale_" + _sfx(coarse.dtype))(
         ctypes.c_int(1 if fine_to_coarse else 0), *coarse._nb(), *fine._nb(), ctypes.c_int(1 if even_coarse else 0),
         _p(cc), _p(cf), ctypes.c_ulonglong(cc.size), _p(ox), _p(oy), _p(oz), T(omega_c), T(omega_f))
+
+
+# ---- adjoint (forward-mode dual numbers through the oracle's collision) ------------------------
+def k17_vjp(fin, g, omega, limiters=(0.01, 0.01, 0.01), force=None):
+    """(d post / d pre)^T g per node (FP64), plus the adjoint of the velocity shift (n, 3)."""
+    fin = np.ascontiguousarray(fin, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    n = fin.shape[0]
+    fb = np.empty_like(fin)
+    ub = np.empty((n, 3))
+    lim = np.asarray(limiters, dtype=np.float64)
+    frc = None if force is None else np.asarray(force, dtype=np.float64)
+    lib().orc_k17_vjp_f64(_p(fin), _p(g), ctypes.c_longlong(n), ctypes.c_double(omega), _p(lim), _p(frc), _p(fb),
+                          _p(ub))
+    return fb, ub
+
+
+def adjoint_step(lattice, lam, fluid, omega, limiters=(0.01, 0.01, 0.01), force=None, shift_adjoint=None):
+    """Reverse of one Lattice.collide(fluid, ...) step: lattice.f holds the state before the
+    step and lattice.even its parity; lam (27*N, FP64) is updated in place. shift_adjoint
+    (optional (3, N) array) accumulates the adjoint of each node's velocity shift."""
+    assert lattice.dtype == np.float64 and lam.dtype == np.float64 and lam.flags.c_contiguous
+    fluid = np.ascontiguousarray(fluid, dtype=np.uint32)
+    lim = np.asarray(limiters, dtype=np.float64)
+    frc = None if force is None else np.asarray(force, dtype=np.float64)
+    ub = None
+    if shift_adjoint is not None:
+        ub = np.zeros((lattice.N, 3))
+    lib().orc_adjoint_step_f64(_p(lattice.f), _p(lam), ctypes.c_ulonglong(lattice.N), _p(lattice.nbx), _p(lattice.nby),
+                               _p(lattice.nbz), _p(fluid), ctypes.c_ulonglong(fluid.size), ctypes.c_double(omega),
+                               _p(lim), _p(frc), ctypes.c_int(int(lattice.even)), _p(ub))
+    if shift_adjoint is not None:
+        shift_adjoint += ub.T

@@ -264,7 +264,10 @@ template <class T> void k17_collide(T* f, const K17Param<T>& p, Macro<T>* out)
     m110 += omega * (-m110);
 
     // limited relaxation of 3rd order (Eqs. 116-123)
-    auto limited = [](T rate, T x, T lim) { return rate + (T(1) - rate) * std::abs(x) / (std::abs(x) + lim); };
+    auto limited = [](T rate, T x, T lim) {
+        using std::abs;
+        return rate + (T(1) - rate) * abs(x) / (abs(x) + lim);
+    };
     T w = limited(Oxyz, m111, p.limD);
     m111 += w * (-m111);
     w = limited(OxyyPxzz, mxxyPyzz, p.limP);

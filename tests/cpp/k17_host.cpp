@@ -1,0 +1,47 @@
+// Host build of the gfx950 collision and its hand-written adjoint (vf_k17.h,
+// vf_k17_adjoint.h): the same source the kernels inline, compiled for the CPU so the tests can
+// compare it with the oracle without a GPU. Test infrastructure; not part of the library.
+#include <cstring>
+
+#include "vf_k17_adjoint.h"
+
+using namespace vfa;
+
+static K17Rates rates(float omega, const float* lim3, const float* force3)
+{
+    vf_lbm_parameter p{};
+    p.omega = omega;
+    for (int i = 0; i < 3; ++i) {
+        p.quadricLimiters[i] = lim3[i];
+        p.forces[i] = force3 ? force3[i] : 0.0f;
+    }
+    p.forceFactor = 1.0f;
+    return make_k17_rates(p);
+}
+
+// force3 is the body force; the collision shifts the velocity by half of it
+extern "C" void host_k17_collide(const float* fin, float* fout, long long n, float omega, const float* lim3,
+                                 const float* force3)
+{
+    const K17Rates r = rates(omega, lim3, force3);
+    for (long long i = 0; i < n; ++i) {
+        float f[27];
+        std::memcpy(f, fin + 27 * i, sizeof(f));
+        MacroOut mo;
+        k17_collide(f, r, mo);
+        std::memcpy(fout + 27 * i, f, sizeof(f));
+    }
+}
+
+extern "C" void host_k17_vjp(const float* fin, const float* g, long long n, float omega, const float* lim3,
+                             const float* force3, float* fb, float* ub3n)
+{
+    const K17Rates r = rates(omega, lim3, force3);
+    for (long long i = 0; i < n; ++i) {
+        float f[27], gg[27], out[27];
+        std::memcpy(f, fin + 27 * i, sizeof(f));
+        std::memcpy(gg, g + 27 * i, sizeof(gg));
+        k17_vjp(f, gg, r, out, ub3n ? ub3n + 3 * i : nullptr);
+        std::memcpy(fb + 27 * i, out, sizeof(out));
+    }
+}

@@ -1,0 +1,155 @@
+"""GPU adjoint of the K17 collide+stream step and the differentiable time step.
+
+* vf_k17_adjoint_box / vf_k17_adjoint (hand-written FP32 reverse mode) against the oracle's
+  FP64 reference adjoint (forward-mode dual numbers through its own collision) on the same
+  stored state: every storage type, both parities, masked and periodic lattices, and the
+  velocity-shift adjoint. Tolerance: 3e-5 of the largest adjoint entry (FP32 reverse mode; the
+  limiter's derivative amplifies rounding).
+* k17_steps + an FP64 kinetic-energy objective: the gradient matches FP64 central finite
+  differences of the oracle's forward run (relative error < 1e-4, the stated bound) and the
+  oracle's FP64 multi-step adjoint, and does not depend on the checkpoint interval.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import near_equilibrium, pack_pre
+from virtualfluids_amd.grid import periodic_box, walled_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(kind):
+    return periodic_box(24, 10, 8) if kind == "periodic" else walled_box((22, 8, 6), {})
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("path", ["box", "indexed"])
+@pytest.mark.parametrize("even", [True, False])
+@pytest.mark.parametrize("kind", ["periodic", "walled"])
+def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, storage, path, even, kind):
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import Lattice
+
+    grid = _grid(kind)
+    rng = np.random.default_rng(500 + 2 * even + (path == "box") + 4 * len(storage))
+    omega = 1.8
+    L = Lattice(grid, omega, storage=storage, box=(path == "box"))
+    L.even = even
+    L.set_populations(pack_pre(near_equilibrium(rng, grid.N, u_amp=0.05, noise=1e-2), grid, even))
+    lam0 = rng.normal(0, 1, 27 * grid.N)
+    lam = torch.from_numpy(lam0.astype(np.float32)).cuda()
+    shift = torch.zeros(3 * grid.N, dtype=torch.float32, device="cuda")
+    fn = "vf_k17_adjoint_box" if path == "box" else "vf_k17_adjoint"
+    print(f"{fn} {storage} even={even} {kind}", flush=True)
+    capi.call(fn, L.pref(), ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(shift.data_ptr()), None)
+    got = lam.cpu().numpy().astype(np.float64)
+    got_shift = shift.cpu().numpy().reshape(3, grid.N)
+
+    A = orc.Lattice(grid.nbx, grid.nby, grid.nbz)
+    A.f[:] = L.populations()  # the stored (rounded) state
+    A.even = even
+    ref = lam0.astype(np.float32).astype(np.float64)
+    ref_shift = np.zeros((3, grid.N))
+    orc.adjoint_step(A, ref, grid.fluid, omega, shift_adjoint=ref_shift)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    err_s = np.abs(got_shift - ref_shift).max() / np.abs(ref_shift).max()
+    print(f"  rel err adjoint {err:.2e} shift {err_s:.2e}")
+    assert err < 3e-5 and err_s < 3e-4
+    if kind == "walled":  # cells only solid nodes own keep their adjoint
+        assert np.any(got == lam0.astype(np.float32))
+
+
+def _objective_setup(orc, steps):
+    import torch
+
+    from virtualfluids_amd.adjoint import pre_collision_index
+    from virtualfluids_amd.lattice import Lattice
+
+    grid = periodic_box(16, 12, 10)
+    rng = np.random.default_rng(600)
+    f0 = pack_pre(near_equilibrium(rng, grid.N, u_amp=0.05, noise=2e-3), grid, True)
+    omega = 1.9
+    L = Lattice(grid, omega)
+    even_final = steps % 2 == 0
+    idx = torch.from_numpy(pre_collision_index(grid, even_final)).cuda()
+    return grid, f0, omega, L, idx, even_final
+
+
+def _gpu_gradient(L, f0, idx, steps, K):
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_steps, kinetic_energy
+
+    x = torch.from_numpy(f0.astype(np.float32)).cuda().requires_grad_(True)
+    J = kinetic_energy(k17_steps(x, L, steps, checkpoint_every=K, even=True), idx)
+    J.backward()
+    return float(J), x.grad.cpu().numpy().astype(np.float64)
+
+
+def _oracle_J(orc, grid, buf, omega, steps, even_final):
+    from virtualfluids_amd.adjoint import pre_collision_index
+
+    L = orc.Lattice(grid.nbx, grid.nby, grid.nbz)
+    L.f[:] = buf
+    states = []
+    for _ in range(steps):
+        states.append((L.f.copy(), L.even))
+        L.collide(grid.fluid, omega)
+        L.swap()
+    f = L.f[pre_collision_index(grid, even_final)]
+    drho = f.sum(0)
+    from virtualfluids_amd.grid import CX, CY, CZ
+
+    j = np.stack([CX, CY, CZ]).astype(np.float64) @ f
+    return 0.5 * ((j * j).sum(0) / (1 + drho)).sum(), states, f, drho, j
+
+
+def test_gradient_matches_fp64_finite_differences(gpu, orc):
+    steps = 24
+    grid, f0, omega, L, idx, even_final = _objective_setup(orc, steps)
+    J, grad = _gpu_gradient(L, f0, idx, steps, K=5)
+    rng = np.random.default_rng(601)
+    v = rng.normal(0, 1, f0.shape)
+    h = 1e-5
+    Jp = _oracle_J(orc, grid, f0 + h * v, omega, steps, even_final)[0]
+    Jm = _oracle_J(orc, grid, f0 - h * v, omega, steps, even_final)[0]
+    fd = (Jp - Jm) / (2 * h)
+    ad = float(grad @ v)
+    print(f"J={J:.6e} <grad, v>: adjoint {ad:.10e} FD {fd:.10e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) < 1e-4 * abs(fd)
+
+
+def test_gradient_matches_fp64_reference_adjoint(gpu, orc):
+    from virtualfluids_amd.grid import CX, CY, CZ
+
+    steps = 12
+    grid, f0, omega, L, idx, even_final = _objective_setup(orc, steps)
+    _, grad = _gpu_gradient(L, f0, idx, steps, K=4)
+    J, states, f, drho, j = _oracle_J(orc, grid, f0, omega, steps, even_final)
+    # dJ/df_d of 1/2 |j|^2 / (1 + drho) at every node, scattered to the cells
+    u = j / (1 + drho)
+    dJ = np.stack([CX, CY, CZ]).T.astype(np.float64) @ u - 0.5 * (u * u).sum(0)
+    from virtualfluids_amd.adjoint import pre_collision_index
+
+    lam = np.zeros(27 * grid.N)
+    lam[pre_collision_index(grid, even_final)] = dJ
+    A = orc.Lattice(grid.nbx, grid.nby, grid.nbz)
+    for state, even in reversed(states):
+        A.f[:] = state
+        A.even = even
+        orc.adjoint_step(A, lam, grid.fluid, omega)
+    err = np.abs(grad - lam).max() / np.abs(lam).max()
+    print(f"gradient vs FP64 reference adjoint: rel err {err:.2e}")
+    assert err < 1e-4
+
+
+def test_checkpoint_interval_does_not_change_gradient(gpu, orc):
+    steps = 9
+    grid, f0, omega, L, idx, _ = _objective_setup(orc, steps)
+    grads = [_gpu_gradient(L, f0, idx, steps, K)[1] for K in (1, 4, 9, 20)]
+    for g in grads[1:]:
+        assert np.array_equal(g, grads[0])

@@ -21,6 +21,8 @@ INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 ROOT = os.path.dirname(HERE)
 HOST_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_host.cpp")
 HOST_TEST = os.path.join(ROOT, "tests", "cpp", "test_host")
+K17_HOST_SRC = os.path.join(ROOT, "tests", "cpp", "k17_host.cpp")
+K17_HOST_LIB = os.path.join(ROOT, "tests", "cpp", "libk17host.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_FLAGS = [
@@ -74,6 +76,13 @@ def build_host_test() -> str:
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {HOST_TEST_SRC}:\n{r.stderr}")
     os.replace(tmp, HOST_TEST)
+    # the collision and its adjoint compiled for the host (CPU tests of the device source)
+    tmp = K17_HOST_LIB + ".tmp"
+    cmd = [HIPCC, *DEVICE_FLAGS, "-x", "hip", "-shared", K17_HOST_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {K17_HOST_SRC}:\n{r.stderr}")
+    os.replace(tmp, K17_HOST_LIB)
     return HOST_TEST
 
 

@@ -110,6 +110,8 @@ SIGNATURES = {
     "vf_ConservationSums": [_P, _vp, _vp],
     "vf_MomentSums": [_P, _vp, _vp],
     "vf_ScaleCoarseToFineCompressible": [_P, _P, _vp, _vp, _vp],
+    "vf_k17_adjoint_box": [_P, _vp, _vp, _vp],
+    "vf_k17_adjoint": [_P, _vp, _vp, _vp],
     "vf_ScaleFineToCoarseCompressible": [_P, _P, _vp, _vp, _vp],
 }
 

@@ -23,40 +23,6 @@
 
 namespace vfa {
 
-// ---- host: omega-derived constants ------------------------------------------------------
-// Relaxation rates of src/lbm/collision/K17CompressibleNavierStokes.h:251-273 evaluated
-// once per launch in double precision.
-K17Rates make_k17_rates(const vf_lbm_parameter& p)
-{
-    const double w = p.omega;
-    const double OxyyPxzz = 8.0 * (-2.0 + w) * (1.0 + 2.0 * w) / (-8.0 - 14.0 * w + 7.0 * w * w);
-    const double OxyyMxzz = 8.0 * (-2.0 + w) * (-7.0 + 4.0 * w) / (56.0 - 50.0 * w + 9.0 * w * w);
-    const double Oxyz = 24.0 * (-2.0 + w) * (-2.0 - 7.0 * w + 3.0 * w * w) /
-                        (48.0 + 152.0 * w - 130.0 * w * w + 29.0 * w * w * w);
-    const double factorA = (4.0 + 2.0 * w - 3.0 * w * w) / (2.0 - 7.0 * w + 5.0 * w * w);
-    const double factorB = (4.0 + 28.0 * w - 14.0 * w * w) / (6.0 - 21.0 * w + 15.0 * w * w);
-    const double nu = 1.0 / w - 0.5;
-    K17Rates r;
-    r.omega = (float)w;
-    r.oneMinusOmega = (float)(1.0 - w);
-    r.mHalfOmega = (float)(-0.5 * w);
-    r.threeHalfOmega = (float)(1.5 * w);
-    r.mThreeOmega = (float)(-3.0 * w);
-    r.galileo = (float)(3.0 * (1.0 - 0.5 * w));
-    r.limP = p.quadricLimiters[0];
-    r.limM = p.quadricLimiters[1];
-    r.limD = p.quadricLimiters[2];
-    r.limNumP = (float)((1.0 - OxyyPxzz) * (double)p.quadricLimiters[0]);
-    r.limNumM = (float)((1.0 - OxyyMxzz) * (double)p.quadricLimiters[1]);
-    r.limNumD = (float)((1.0 - Oxyz) * (double)p.quadricLimiters[2]);
-    r.A4 = (float)(-nu * (2.0 / 3.0) * factorA);
-    r.B4 = (float)(-nu * (1.0 / 3.0) * factorB * (-3.0 * w));
-    r.fx = p.forces[0] * 0.5f * p.forceFactor;
-    r.fy = p.forces[1] * 0.5f * p.forceFactor;
-    r.fz = p.forces[2] * 0.5f * p.forceFactor;
-    return r;
-}
-
 // ---- device: gather / scatter over the 8 cell nodes -------------------------------------
 template <class S, bool EVEN>
 __device__ __forceinline__ void gather(const S* __restrict__ f, uint64_t N, const uint32_t (&n)[8], float (&v)[27])

@@ -19,7 +19,7 @@
 
 namespace vfa {
 
-// Per-launch constants derived from omega on the host (see make_k17_rates in vf_collision.hip).
+// Per-launch constants derived from omega on the host (make_k17_rates below).
 struct K17Rates {
     float omega;         // w
     float oneMinusOmega; // 1 - w
@@ -34,11 +34,21 @@ struct K17Rates {
     float fx, fy, fz;    // half-step velocity shift (forces * 1/2 * forceFactor)
 };
 
-__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// The collision helpers compile for the host as well (the CPU tests evaluate the same code).
+#define VF_HD __host__ __device__ __forceinline__
+
+VF_HD float fast_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
 
 // forward chimera line: a,b,c -> zeroth, first, second central moment along the line
 template <int KNUM, int KDEN>
-__device__ __forceinline__ void fwd(float& a, float& b, float& c, float u, float u2, float m2u)
+VF_HD void fwd(float& a, float& b, float& c, float u, float u2, float m2u)
 {
     const float e = a + c;
     const float d = c - a;
@@ -50,7 +60,7 @@ __device__ __forceinline__ void fwd(float& a, float& b, float& c, float u, float
 }
 // forward line when only the zeroth and first moments are used later
 template <int KNUM, int KDEN>
-__device__ __forceinline__ void fwd01(float& a, float& b, float c, float u)
+VF_HD void fwd01(float& a, float& b, float c, float u)
 {
     const float d = c - a;
     const float s = (a + c) + b;
@@ -65,7 +75,7 @@ __device__ __forceinline__ void fwd01(float& a, float& b, float c, float u)
 struct BwdPass {
     float a0, b0, a2, b2;
 };
-__device__ __forceinline__ BwdPass bwd_pass(float u, float u2)
+VF_HD BwdPass bwd_pass(float u, float u2)
 {
     BwdPass p;
     p.a0 = 0.5f * (u2 - u);
@@ -75,7 +85,7 @@ __device__ __forceinline__ BwdPass bwd_pass(float u, float u2)
     return p;
 }
 template <int KNUM, int KDEN>
-__device__ __forceinline__ void bwd(float& a, float& b, float& c, const BwdPass& p)
+VF_HD void bwd(float& a, float& b, float& c, const BwdPass& p)
 {
     constexpr float K = float(KNUM) / float(KDEN);
     const float hc = 0.5f * c;
@@ -87,12 +97,45 @@ __device__ __forceinline__ void bwd(float& a, float& b, float& c, const BwdPass&
     c = hi;
 }
 
+// Per-launch constants from the level's omega, limiters and force (double precision on the
+// host; relaxation rates of src/lbm/collision/K17CompressibleNavierStokes.h:251-273).
+inline K17Rates make_k17_rates(const vf_lbm_parameter& p)
+{
+    const double w = p.omega;
+    const double OxyyPxzz = 8.0 * (-2.0 + w) * (1.0 + 2.0 * w) / (-8.0 - 14.0 * w + 7.0 * w * w);
+    const double OxyyMxzz = 8.0 * (-2.0 + w) * (-7.0 + 4.0 * w) / (56.0 - 50.0 * w + 9.0 * w * w);
+    const double Oxyz = 24.0 * (-2.0 + w) * (-2.0 - 7.0 * w + 3.0 * w * w) /
+                        (48.0 + 152.0 * w - 130.0 * w * w + 29.0 * w * w * w);
+    const double factorA = (4.0 + 2.0 * w - 3.0 * w * w) / (2.0 - 7.0 * w + 5.0 * w * w);
+    const double factorB = (4.0 + 28.0 * w - 14.0 * w * w) / (6.0 - 21.0 * w + 15.0 * w * w);
+    const double nu = 1.0 / w - 0.5;
+    K17Rates r;
+    r.omega = (float)w;
+    r.oneMinusOmega = (float)(1.0 - w);
+    r.mHalfOmega = (float)(-0.5 * w);
+    r.threeHalfOmega = (float)(1.5 * w);
+    r.mThreeOmega = (float)(-3.0 * w);
+    r.galileo = (float)(3.0 * (1.0 - 0.5 * w));
+    r.limP = p.quadricLimiters[0];
+    r.limM = p.quadricLimiters[1];
+    r.limD = p.quadricLimiters[2];
+    r.limNumP = (float)((1.0 - OxyyPxzz) * (double)p.quadricLimiters[0]);
+    r.limNumM = (float)((1.0 - OxyyMxzz) * (double)p.quadricLimiters[1]);
+    r.limNumD = (float)((1.0 - Oxyz) * (double)p.quadricLimiters[2]);
+    r.A4 = (float)(-nu * (2.0 / 3.0) * factorA);
+    r.B4 = (float)(-nu * (1.0 / 3.0) * factorB * (-3.0 * w));
+    r.fx = p.forces[0] * 0.5f * p.forceFactor;
+    r.fy = p.forces[1] * 0.5f * p.forceFactor;
+    r.fz = p.forces[2] * 0.5f * p.forceFactor;
+    return r;
+}
+
 struct MacroOut {
     float drho, vx, vy, vz;
 };
 
 // f: 27 shifted pre-collision populations in D3Q27 order; replaced by post-collision values.
-__device__ __forceinline__ void k17_collide(float (&f)[27], const K17Rates& r, MacroOut& mo)
+VF_HD void k17_collide(float (&f)[27], const K17Rates& r, MacroOut& mo)
 {
     float& fMMM = f[dMMM]; float& f0MM = f[d0MM]; float& fPMM = f[dPMM];
     float& fM0M = f[dM0M]; float& f00M = f[d00M]; float& fP0M = f[dP0M];
