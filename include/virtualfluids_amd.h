@@ -224,6 +224,15 @@ int vf_ScaleFineToCoarseCompressible(const vf_lbm_parameter* coarse, const vf_lb
  * VF_GEO_SOLID/VF_GEO_VOID (all nodes when typeOfGridNode is NULL). */
 int vf_InitNavierStokesCompressible(const vf_lbm_parameter* para, void* stream);
 
+/* InitNavierStokesCompressible::init with para->getUseInitNeq() (InitNavierStokesCompressible.cu:67-84,
+ * kernel InitNavierStokesCompressibleNonEquilibrium_Device, InitNavierStokesCompressible_Device.cu:216-501):
+ * the equilibrium plus (1 + drho) times the first-order non-equilibrium part from central
+ * differences of velocityX/Y/Z, relaxation rate para->omega. Solid and void nodes get 96 in their
+ * rest slot (the reference's marker). neighborInverse: the (-1,-1,-1) neighbour of every node
+ * (Parameter.h neighborInverse); may be NULL for a dense periodic box without neighbour arrays. */
+int vf_InitNavierStokesCompressibleNonEquilibrium(const vf_lbm_parameter* para, const uint32_t* neighborInverse,
+                                                  void* stream);
+
 /* calculateMacroscopicQuantitiesCompressible (src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237, 953):
  * fills para->rho, velocity*, pressure from the pre-collision state. */
 int vf_CalculateMacroscopicQuantitiesCompressible(const vf_lbm_parameter* para, void* stream);

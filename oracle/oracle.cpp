@@ -309,6 +309,92 @@ void init_eq(const Lattice<T>& L, const u32* geo, const T* rho, const T* ux, con
     }
 }
 
+// Non-equilibrium initialisation (InitNavierStokesCompressibleNonEquilibrium_Device,
+// InitNavierStokesCompressible_Device.cu:216-501): the equilibrium above plus (1 + drho) times a
+// first-order non-equilibrium part built from central differences of the input velocity
+// (neighbours +x/+y/+z and, through the (-1,-1,-1) neighbour nbinv, -x/-y/-z), with the
+// relaxation rate omega for the deviatoric part and 1 for the bulk part. Solid and void
+// nodes get 96 in their rest slot, as in the reference (a marker value).
+template <class T> struct NeqCoefficients {
+    T zero, e, n, t, ne, se, te, be, tn, bn, tne, tsw, tse, tnw;
+};
+
+// The closed forms of the reference (coefficients of the velocity-gradient entries
+// a = d(ux)/d(x,y,z), b = d(uy)/..., c = d(uz)/...), with eps = 1 and bulk rate 1.
+template <class T>
+NeqCoefficients<T> neq_coefficients(T ax, T ay, T az, T bx, T by, T bz, T cx, T cy, T cz, T o)
+{
+    NeqCoefficients<T> q;
+    const T div = ax + by + cz;
+    q.e = (T(5) * div * o - T(8) * ax + T(4) * by + T(4) * cz) / (T(54) * o);
+    q.n = q.e + T(2) * (ax - by) / (T(9) * o);
+    q.t = q.e + T(2) * (ax - cz) / (T(9) * o);
+    q.ne = -(T(5) * cz * o + T(3) * (ay + bx) - T(2) * cz + ax * (T(5) * o + T(1)) + by * (T(5) * o + T(1))) / (T(54) * o);
+    q.se = q.ne + (ay + bx) / (T(9) * o);
+    q.te = -(T(5) * cz * o + by * (T(5) * o - T(2)) + T(3) * (az + cx) + cz + ax * (T(5) * o + T(1))) / (T(54) * o);
+    q.be = q.te + (az + cx) / (T(9) * o);
+    q.tn = -(T(5) * div * o - T(2) * ax + by + T(3) * bz + T(3) * cy + cz) / (T(54) * o);
+    q.bn = q.tn + (bz + cy) / (T(9) * o);
+    q.zero = T(5) * div / T(9);
+    q.tne = -(ay + az + bx + bz + cx + cy) / (T(72) * o);
+    q.tsw = -q.tne - (ay + bx) / (T(36) * o);
+    q.tse = -q.tne - (az + cx) / (T(36) * o);
+    q.tnw = -q.tne - (bz + cy) / (T(36) * o);
+    return q;
+}
+
+// per direction: which coefficient (the pairs d, inv(d) share one)
+template <class T> inline T neq_of(const NeqCoefficients<T>& q, int d)
+{
+    switch (d) {
+    case d000: return q.zero;
+    case dP00: case dM00: return q.e;
+    case d0P0: case d0M0: return q.n;
+    case d00P: case d00M: return q.t;
+    case dPP0: case dMM0: return q.ne;
+    case dPM0: case dMP0: return q.se;
+    case dP0P: case dM0M: return q.te;
+    case dP0M: case dM0P: return q.be;
+    case d0PP: case d0MM: return q.tn;
+    case d0PM: case d0MP: return q.bn;
+    case dPPP: case dMMM: return q.tne;
+    case dPPM: case dMMP: return q.tsw;
+    case dPMP: case dMPM: return q.tse;
+    default: return q.tnw; // dPMM, dMPP
+    }
+}
+
+template <class T>
+void init_neq(const Lattice<T>& L, const u32* geo, const u32* nbinv, const T* rho, const T* ux, const T* uy,
+              const T* uz, T omega, bool even)
+{
+    T* P[27];
+    slot_table(L, even, P);
+#pragma omp parallel for schedule(static)
+    for (long long kk = 0; kk < (long long)L.N; ++kk) {
+        const u32 k = (u32)kk;
+        if (geo && (geo[k] == 15u || geo[k] == 16u)) {
+            L.buf[(u64)d000 * L.N + k] = T(96);
+            continue;
+        }
+        const u32 kPx = L.nbx[k], kPy = L.nby[k], kPz = L.nbz[k], kM = nbinv[k];
+        const u32 kMx = L.nbz[L.nby[kM]], kMy = L.nbz[L.nbx[kM]], kMz = L.nby[L.nbx[kM]];
+        const T h = T(2);
+        const NeqCoefficients<T> q = neq_coefficients<T>(
+            (ux[kPx] - ux[kMx]) / h, (ux[kPy] - ux[kMy]) / h, (ux[kPz] - ux[kMz]) / h, (uy[kPx] - uy[kMx]) / h,
+            (uy[kPy] - uy[kMy]) / h, (uy[kPz] - uy[kMz]) / h, (uz[kPx] - uz[kMx]) / h, (uz[kPy] - uz[kMy]) / h,
+            (uz[kPz] - uz[kMz]) / h, omega);
+        u32 n[8];
+        cell_nodes(L, k, n);
+        const T cu_sq = T(1.5) * (ux[k] * ux[k] + uy[k] * uy[k] + uz[k] * uz[k]);
+        for (int d = 0; d < 27; ++d) {
+            T v = feq_full(d, rho[k], ux[k], uy[k], uz[k], cu_sq);
+            v += (T(1) + rho[k]) * neq_of(q, d);
+            write_pre(P, n, d, v);
+        }
+    }
+}
+
 template <class T>
 void macroscopic(const Lattice<T>& L, const u32* geo, bool even, T* rho, T* vx, T* vy, T* vz, T* press)
 {
@@ -483,6 +569,13 @@ template <class T> K17Param<T> make_param(T omega, const T* lim3, const T* force
     {                                                                                                          \
         Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
         init_eq(L, geo, rho, ux, uy, uz, even != 0);                                                           \
+    }                                                                                                          \
+    extern "C" void orc_init_neq_##SFX(T* buf, unsigned long long N, const uint32_t* nbx, const uint32_t* nby, \
+                                       const uint32_t* nbz, const uint32_t* nbinv, const uint32_t* geo,        \
+                                       const T* rho, const T* ux, const T* uy, const T* uz, T omega, int even)  \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        init_neq(L, geo, nbinv, rho, ux, uy, uz, omega, even != 0);                                            \
     }                                                                                                          \
     extern "C" void orc_macroscopic_##SFX(T* buf, unsigned long long N, const uint32_t* nbx,                  \
                                           const uint32_t* nby, const uint32_t* nbz, const uint32_t* geo,      \

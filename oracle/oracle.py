@@ -139,6 +139,15 @@ class Lattice:
         g = None if geo is None else np.ascontiguousarray(geo, dtype=np.uint32)
         self._fn("orc_init_eq")(*self._nb(), _p(g), *(_p(a) for a in arrs), ctypes.c_int(int(self.even)))
 
+    def init_neq(self, rho, ux, uy, uz, nbinv, omega, geo=None):
+        """Equilibrium plus the velocity-gradient non-equilibrium part (needs the (-1,-1,-1)
+        neighbour array nbinv)."""
+        arrs = [np.ascontiguousarray(a, dtype=self.dtype) for a in (rho, ux, uy, uz)]
+        g = None if geo is None else np.ascontiguousarray(geo, dtype=np.uint32)
+        inv = np.ascontiguousarray(nbinv, dtype=np.uint32)
+        self._fn("orc_init_neq")(*self._nb(), _p(inv), _p(g), *(_p(a) for a in arrs), self._T(omega),
+                                 ctypes.c_int(int(self.even)))
+
     def macroscopic(self, geo=None):
         g = None if geo is None else np.ascontiguousarray(geo, dtype=np.uint32)
         out = [np.empty(self.N, dtype=self.dtype) for _ in range(5)]

@@ -106,6 +106,7 @@ SIGNATURES = {
     "vf_VelocityWithPressureInterpolatedCompressible": [_P, _Q, _vp],
     "vf_PressureNonEquilibriumCompressible": [_P, _Q, _vp],
     "vf_InitNavierStokesCompressible": [_P, _vp],
+    "vf_InitNavierStokesCompressibleNonEquilibrium": [_P, _vp, _vp],
     "vf_CalculateMacroscopicQuantitiesCompressible": [_P, _vp],
     "vf_ConservationSums": [_P, _vp, _vp],
     "vf_MomentSums": [_P, _vp, _vp],

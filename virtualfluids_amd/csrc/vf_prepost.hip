@@ -25,6 +25,101 @@ __global__ __launch_bounds__(256) void init_kernel(S* __restrict__ f, Addr a, co
     for (int d = 0; d < 27; ++d) write_pre(f, a, n, d, even_flag != 0, feq_shifted(d, r, vx, vy, vz, cu_sq));
 }
 
+// Non-equilibrium initialisation: the equilibrium plus (1 + drho) times the first-order
+// non-equilibrium part from central differences of the velocity field
+// (InitNavierStokesCompressibleNonEquilibrium_Device, InitNavierStokesCompressible_Device.cu:216-501;
+// eps = 1, bulk rate 1). The pairs d / inv(d) share one coefficient.
+struct NeqCoef {
+    float zero, e, n, t, ne, se, te, be, tn, bn, tne, tsw, tse, tnw;
+};
+
+__device__ __forceinline__ NeqCoef neq_coefficients(float ax, float ay, float az, float bx, float by, float bz, float cx_,
+                                                    float cy_, float cz_, float o)
+{
+    NeqCoef q;
+    const float div = ax + by + cz_;
+    const float i54 = 1.0f / (54.0f * o), i9 = 1.0f / (9.0f * o), i36 = 1.0f / (36.0f * o), i72 = 1.0f / (72.0f * o);
+    q.e = (5.0f * div * o - 8.0f * ax + 4.0f * by + 4.0f * cz_) * i54;
+    q.n = q.e + 2.0f * (ax - by) * i9;
+    q.t = q.e + 2.0f * (ax - cz_) * i9;
+    q.ne = -(5.0f * cz_ * o + 3.0f * (ay + bx) - 2.0f * cz_ + (ax + by) * (5.0f * o + 1.0f)) * i54;
+    q.se = q.ne + (ay + bx) * i9;
+    q.te = -(5.0f * cz_ * o + by * (5.0f * o - 2.0f) + 3.0f * (az + cx_) + cz_ + ax * (5.0f * o + 1.0f)) * i54;
+    q.be = q.te + (az + cx_) * i9;
+    q.tn = -(5.0f * div * o - 2.0f * ax + by + 3.0f * bz + 3.0f * cy_ + cz_) * i54;
+    q.bn = q.tn + (bz + cy_) * i9;
+    q.zero = (5.0f / 9.0f) * div;
+    q.tne = -(ay + az + bx + bz + cx_ + cy_) * i72;
+    q.tsw = -q.tne - (ay + bx) * i36;
+    q.tse = -q.tne - (az + cx_) * i36;
+    q.tnw = -q.tne - (bz + cy_) * i36;
+    return q;
+}
+
+__device__ __forceinline__ float neq_of(const NeqCoef& q, int d)
+{
+    switch (d) {
+    case d000: return q.zero;
+    case dP00: case dM00: return q.e;
+    case d0P0: case d0M0: return q.n;
+    case d00P: case d00M: return q.t;
+    case dPP0: case dMM0: return q.ne;
+    case dPM0: case dMP0: return q.se;
+    case dP0P: case dM0M: return q.te;
+    case dP0M: case dM0P: return q.be;
+    case d0PP: case d0MM: return q.tn;
+    case d0PM: case d0MP: return q.bn;
+    case dPPP: case dMMM: return q.tne;
+    case dPPM: case dMMP: return q.tsw;
+    case dPMP: case dMPM: return q.tse;
+    default: return q.tnw;
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(256) void init_neq_kernel(S* __restrict__ f, Addr a, const uint32_t* geo,
+                                                       const uint32_t* nbinv, const float* rho, const float* ux,
+                                                       const float* uy, const float* uz, float omega, int even_flag)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.N) return;
+    if (geo && (geo[k] == VF_GEO_SOLID || geo[k] == VF_GEO_VOID)) {
+        Codec<S>::store(f + (uint64_t)d000 * a.N + k, 96.0f); // the reference's marker value
+        return;
+    }
+    uint32_t kPx, kPy, kPz, kMx, kMy, kMz;
+    if (a.nbx) {
+        const uint32_t kM = nbinv[k];
+        kPx = a.nbx[k];
+        kPy = a.nby[k];
+        kPz = a.nbz[k];
+        kMx = a.nbz[a.nby[kM]];
+        kMy = a.nbz[a.nbx[kM]];
+        kMz = a.nby[a.nbx[kM]];
+    } else {
+        const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny, nz = (uint32_t)a.nz;
+        const uint32_t x = (uint32_t)k % nx, t = (uint32_t)k / nx, y = t % ny, z = t / ny;
+        const uint32_t row = nx * (y + ny * z);
+        kPx = row + (x + 1 == nx ? 0 : x + 1);
+        kMx = row + (x == 0 ? nx - 1 : x - 1);
+        kPy = nx * ((y + 1 == ny ? 0 : y + 1) + ny * z) + x;
+        kMy = nx * ((y == 0 ? ny - 1 : y - 1) + ny * z) + x;
+        kPz = nx * (y + ny * (z + 1 == nz ? 0 : z + 1)) + x;
+        kMz = nx * (y + ny * (z == 0 ? nz - 1 : z - 1)) + x;
+    }
+    const NeqCoef q = neq_coefficients(0.5f * (ux[kPx] - ux[kMx]), 0.5f * (ux[kPy] - ux[kMy]), 0.5f * (ux[kPz] - ux[kMz]),
+                                       0.5f * (uy[kPx] - uy[kMx]), 0.5f * (uy[kPy] - uy[kMy]), 0.5f * (uy[kPz] - uy[kMz]),
+                                       0.5f * (uz[kPx] - uz[kMx]), 0.5f * (uz[kPy] - uz[kMy]), 0.5f * (uz[kPz] - uz[kMz]),
+                                       omega);
+    uint32_t n[8];
+    cell_nodes(a, (uint32_t)k, n);
+    const float r = rho[k], vx = ux[k], vy = uy[k], vz = uz[k];
+    const float cu_sq = 1.5f * (vx * vx + vy * vy + vz * vz);
+#pragma unroll
+    for (int d = 0; d < 27; ++d)
+        write_pre(f, a, n, d, even_flag != 0, feq_shifted(d, r, vx, vy, vz, cu_sq) + (1.0f + r) * neq_of(q, d));
+}
+
 __device__ __forceinline__ bool is_valid_fluid(uint32_t t)
 {
     return t == VF_GEO_FLUID || t == VF_GEO_PM_0 || t == VF_GEO_PM_1 || t == VF_GEO_PM_2;
@@ -149,6 +244,15 @@ template <class S> struct InitL {
                            p.rho, p.velocityX, p.velocityY, p.velocityZ, p.isEvenTimestep);
     }
 };
+template <class S> struct InitNeqL {
+    static void run(const vf_lbm_parameter& p, const uint32_t* nbinv, hipStream_t s)
+    {
+        const dim3 grid((unsigned)((p.numberOfNodes + 255) / 256));
+        hipLaunchKernelGGL(init_neq_kernel<S>, grid, dim3(256), 0, s, (S*)p.distributions, make_addr(p),
+                           p.typeOfGridNode, nbinv, p.rho, p.velocityX, p.velocityY, p.velocityZ, p.omega,
+                           p.isEvenTimestep);
+    }
+};
 template <class S> struct MacroL {
     static void run(const vf_lbm_parameter& p, hipStream_t s)
     {
@@ -199,6 +303,27 @@ extern "C" int vf_InitNavierStokesCompressible(const vf_lbm_parameter* p, void* 
     }
     rc = by_storage<InitL>(p->storage, *p, (hipStream_t)stream);
     return rc ? rc : check_launch("InitNavierStokesCompressible_Device execution failed");
+}
+
+extern "C" int vf_InitNavierStokesCompressibleNonEquilibrium(const vf_lbm_parameter* p, const uint32_t* neighborInverse,
+                                                             void* stream)
+{
+    int rc = check_lattice(p);
+    if (rc) return rc;
+    if (!p->rho || !p->velocityX || !p->velocityY || !p->velocityZ) {
+        set_last_error("init needs rho and velocityX/Y/Z");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (p->neighborX && !neighborInverse) {
+        set_last_error("the non-equilibrium init on neighbour arrays needs neighborInverse");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (!(p->omega > 0.0f && p->omega < 2.0f)) {
+        set_last_error("omega must lie in (0, 2)");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    rc = by_storage<InitNeqL>(p->storage, *p, neighborInverse, (hipStream_t)stream);
+    return rc ? rc : check_launch("InitNavierStokesCompressibleNonEquilibrium_Device execution failed");
 }
 
 extern "C" int vf_CalculateMacroscopicQuantitiesCompressible(const vf_lbm_parameter* p, void* stream)

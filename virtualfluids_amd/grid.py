@@ -69,6 +69,11 @@ class Grid:
     def index(self, x, y, z):
         return np.asarray(x) + self.nx * (np.asarray(y) + self.ny * np.asarray(z))
 
+    def neighbor_inverse(self) -> np.ndarray:
+        """The (-1,-1,-1) neighbour of every node (neighborInverse), periodic wrap."""
+        x, y, z = self.coords()
+        return self.index((x - 1) % self.nx, (y - 1) % self.ny, (z - 1) % self.nz).astype(np.uint32)
+
     @property
     def fluid_mask(self) -> np.ndarray:
         return (self.geo == GEO_FLUID).astype(np.uint8)

@@ -157,6 +157,17 @@ class Lattice:
                                       dtype=torch.float32))
         capi.call("vf_InitNavierStokesCompressible", self.pref(), stream_handle(stream))
 
+    def init_non_equilibrium(self, rho, ux, uy, uz, stream=None):
+        """Equilibrium plus the velocity-gradient non-equilibrium part (InitNeq), from drho and
+        velocity (numpy or torch, N values each)."""
+        for dst, src in ((self.rho, rho), (self.vx, ux), (self.vy, uy), (self.vz, uz)):
+            dst.copy_(torch.as_tensor(np.asarray(src, dtype=np.float32) if not torch.is_tensor(src) else src,
+                                      dtype=torch.float32))
+        if not hasattr(self, "nbinv"):
+            self.nbinv = _u32(self.grid.neighbor_inverse(), self.device)
+        capi.call("vf_InitNavierStokesCompressibleNonEquilibrium", self.pref(), _ptr(self.nbinv),
+                  stream_handle(stream))
+
     def macroscopic(self, stream=None):
         """drho, vx, vy, vz, p from the pre-collision state (host numpy arrays)."""
         capi.call("vf_CalculateMacroscopicQuantitiesCompressible", self.pref(), stream_handle(stream))
