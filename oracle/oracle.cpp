@@ -28,6 +28,7 @@
 //   adjoint (no reference counterpart): forward-mode dual numbers through k17_collide
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "oracle_adjoint.h"
 #include "oracle_k17.h"
@@ -196,6 +197,43 @@ enum BcKind : int {
     BC_VELOCITY_PRESSURE_INTERPOLATED = 4,
 };
 
+// One BC node: from its post-collision values f, the reflected pre-collision value of the next
+// step for every active link d (q_d in [0, 1]) goes to out[inv(d)]; act[inv(d)] marks it.
+template <class T, class Q>
+void bc_node(int kind, const T* f, const Q* q27, u64 nbc, u64 i, T Ux, T Uy, T Uz, T omega, T* out, bool* act)
+{
+    T drho = T(0), ux = T(0), uy = T(0), uz = T(0), cu_sq = T(0);
+    if (kind >= BC_NOSLIP_INTERPOLATED) {
+        post_macro(f, drho, ux, uy, uz);
+        cu_sq = T(1.5) * (ux * ux + uy * uy + uz * uz) * (T(1) + drho);
+    }
+    for (int d = 0; d < 27; ++d) act[d] = false;
+    for (int d = 1; d < 27; ++d) {
+        const Q qraw = q27[(u64)d * nbc + i];
+        if (!active(qraw)) continue;
+        const T qd = T(qraw);
+        const int r = INV[d];
+        const T w = weight<T>(d);
+        T v;
+        switch (kind) {
+        case BC_NOSLIP_BOUNCEBACK:
+            v = f[d];
+            break;
+        case BC_VELOCITY_BOUNCEBACK:
+            v = f[d] - T(6) * w * c_dot(d, Ux, Uy, Uz);
+            break;
+        default: {
+            const T feq = feq_even(drho, c_dot(d, ux, uy, uz), cu_sq, w);
+            const T wall = (kind == BC_NOSLIP_INTERPOLATED) ? T(0) : T(6) * w * c_dot(d, Ux, Uy, Uz);
+            const T pres = (kind == BC_VELOCITY_PRESSURE_INTERPOLATED) ? w * drho : T(0);
+            v = interp_link(qd, f[d], f[r], feq, omega, wall, pres);
+        }
+        }
+        out[r] = v;
+        act[r] = true;
+    }
+}
+
 // Post-collision BC: read post-collision values with the current parity, write the
 // reflected pre-collision values of the next step with the flipped parity.
 template <class T>
@@ -210,36 +248,13 @@ void bc_post(int kind, const Lattice<T>& L, const int* k, const T* q, u64 nbc, c
     for (long long i = 0; i < (long long)nbc; ++i) {
         u32 n[8];
         cell_nodes(L, (u32)k[i], n);
-        T f[27];
+        T f[27], out[27];
+        bool act[27];
         read_post(P, n, f);
-        T drho = 0, ux = 0, uy = 0, uz = 0, cu_sq = 0;
-        if (kind >= BC_NOSLIP_INTERPOLATED) {
-            post_macro(f, drho, ux, uy, uz);
-            cu_sq = T(1.5) * (ux * ux + uy * uy + uz * uz) * (T(1) + drho);
-        }
         const T Ux = vxbc ? vxbc[i] : T(0), Uy = vybc ? vybc[i] : T(0), Uz = vzbc ? vzbc[i] : T(0);
-        for (int d = 1; d < 27; ++d) {
-            const T qd = q[(u64)d * nbc + i];
-            if (!active(qd)) continue;
-            const int r = INV[d];
-            const T w = weight<T>(d);
-            T v;
-            switch (kind) {
-            case BC_NOSLIP_BOUNCEBACK:
-                v = f[d];
-                break;
-            case BC_VELOCITY_BOUNCEBACK:
-                v = f[d] - T(6) * w * c_dot(d, Ux, Uy, Uz);
-                break;
-            default: {
-                const T feq = feq_even(drho, c_dot(d, ux, uy, uz), cu_sq, w);
-                const T wall = (kind == BC_NOSLIP_INTERPOLATED) ? T(0) : T(6) * w * c_dot(d, Ux, Uy, Uz);
-                const T pres = (kind == BC_VELOCITY_PRESSURE_INTERPOLATED) ? w * drho : T(0);
-                v = interp_link(qd, f[d], f[r], feq, omega, wall, pres);
-            }
-            }
-            write_pre(Pn, n, r, v);
-        }
+        bc_node(kind, f, q, nbc, (u64)i, Ux, Uy, Uz, omega, out, act);
+        for (int r = 0; r < 27; ++r)
+            if (act[r]) write_pre(Pn, n, r, out[r]);
     }
 }
 
@@ -249,6 +264,20 @@ template <class T> inline T feq_full(int d, T drho, T ux, T uy, T uz, T cu_sq)
     if (d == d000) return T(8) / T(27) * (drho - cu_sq * (T(1) + drho));
     const T cu = c_dot(d, ux, uy, uz);
     return weight<T>(d) * (drho + (T(1) + drho) * (T(3) * cu + T(4.5) * cu * cu - cu_sq));
+}
+
+// One pressure-BC node: from the neighbour's pre-collision values fn, the values written to
+// the BC node for the directions the face selects (act[d]).
+template <class T> void pressure_node(const T* fn, T rbc, int ax, int sgn, T* out, bool* act)
+{
+    T drho, inv_rho, ux, uy, uz;
+    compressible_macro(fn, drho, inv_rho, ux, uy, uz);
+    const T cu_sq = T(1.5) * (ux * ux + uy * uy + uz * uz);
+    for (int d = 0; d < 27; ++d) {
+        const int c = ax == 0 ? CX[d] : (ax == 1 ? CY[d] : CZ[d]);
+        act[d] = c * sgn <= 0;
+        if (act[d]) out[d] = fn[d] - feq_full(d, drho, ux, uy, uz, cu_sq) + feq_full(d, rbc, ux, uy, uz, cu_sq);
+    }
 }
 
 // Pre-collision pressure BC (runs after the parity swap). direction selects the face:
@@ -682,5 +711,108 @@ extern "C" void orc_adjoint_step_f64(const double* state, double* lam, unsigned 
         read_post(PA, nn, g);
         k17_vjp_dual(f, g, p, fb, ub3n ? ub3n + 3 * (u64)k : nullptr);
         for (int d = 0; d < 27; ++d) write_pre(PA, nn, d, fb[d]);
+    }
+}
+
+// ---- adjoints of the BC kernels (dual numbers through bc_node / pressure_node) -----------------
+// Post-collision BC of one step: `state` holds the post-collision populations (the collision
+// of the step done, parity `even` not yet swapped). The written cells (flipped parity) carry the
+// adjoint of the values the BC wrote: it goes to the BC node's post-collision cells and the
+// written cells' adjoint becomes zero (their previous content is overwritten). Ubar (optional,
+// 3*nbc, added): adjoint of the wall velocity.
+extern "C" void orc_bc_post_adjoint_f64(int kind, const double* state, double* lam, unsigned long long N,
+                                        const uint32_t* nbx, const uint32_t* nby, const uint32_t* nbz, const int* k,
+                                        const double* q, unsigned long long nbc, const double* vx, const double* vy,
+                                        const double* vz, double omega, int even, double* Ubar)
+{
+    Lattice<double> S{const_cast<double*>(state), N, nbx, nby, nbz};
+    Lattice<double> A{lam, N, nbx, nby, nbz};
+    double *PS[27], *PA[27], *PAn[27];
+    slot_table(S, even != 0, PS);
+    slot_table(A, even != 0, PA);
+    slot_table(A, even == 0, PAn);
+    for (long long i = 0; i < (long long)nbc; ++i) {
+        u32 n[8];
+        cell_nodes(S, (u32)k[i], n);
+        double f[27];
+        read_post(PS, n, f);
+        double g[27];
+        bool act[27];
+        {
+            double out[27];
+            bc_node<double, double>(kind, f, q, nbc, (u64)i, vx ? vx[i] : 0.0, vy ? vy[i] : 0.0, vz ? vz[i] : 0.0,
+                                    omega, out, act);
+        }
+        for (int r = 0; r < 27; ++r) g[r] = act[r] ? PAn[r][n[node_sel(r)]] : 0.0;
+        double fb[27] = {0};
+        for (int col = 0; col < 30; ++col) {
+            Dual fd[27], out[27];
+            bool a2[27];
+            for (int e = 0; e < 27; ++e) fd[e] = Dual(f[e], col == e ? 1.0 : 0.0);
+            Dual Ux(vx ? vx[i] : 0.0, col == 27 ? 1.0 : 0.0), Uy(vy ? vy[i] : 0.0, col == 28 ? 1.0 : 0.0),
+                Uz(vz ? vz[i] : 0.0, col == 29 ? 1.0 : 0.0);
+            bc_node<Dual, double>(kind, fd, q, nbc, (u64)i, Ux, Uy, Uz, Dual(omega), out, a2);
+            double sum = 0.0;
+            for (int r = 0; r < 27; ++r)
+                if (act[r]) sum += out[r].d * g[r];
+            if (col < 27) fb[col] = sum;
+            else if (Ubar) Ubar[3 * i + (col - 27)] += sum;
+        }
+        for (int r = 0; r < 27; ++r)
+            if (act[r]) PAn[r][n[node_sel(r)]] = 0.0;
+        for (int e = 0; e < 27; ++e) PA[INV[e]][n[node_sel(INV[e])]] += fb[e];
+    }
+}
+
+// Pressure BC of one step: `state` holds the populations after the step (parity `even` is the
+// one the BC used, after the swap). rhoBar (optional, nbc, added): adjoint of RhoBC.
+extern "C" void orc_bc_pressure_adjoint_f64(const double* state, double* lam, unsigned long long N,
+                                            const uint32_t* nbx, const uint32_t* nby, const uint32_t* nbz,
+                                            const int* k, const int* kN, const double* rhoBC, unsigned long long nbc,
+                                            int even, int direction, double* rhoBar)
+{
+    if (direction < 1 || direction > 6) return;
+    const int ax = (direction - 1) / 2, sgn = (direction % 2 == 1) ? 1 : -1;
+    Lattice<double> S{const_cast<double*>(state), N, nbx, nby, nbz};
+    Lattice<double> A{lam, N, nbx, nby, nbz};
+    double *PS[27], *PA[27];
+    slot_table(S, even != 0, PS);
+    slot_table(A, even != 0, PA);
+    // gather every written cell's adjoint first, then clear, then scatter (a launch reads before it writes)
+    std::vector<double> G(27 * nbc, 0.0);
+    std::vector<double> FN(27 * nbc, 0.0);
+    for (long long i = 0; i < (long long)nbc; ++i) {
+        u32 n[8], nn[8];
+        cell_nodes(S, (u32)k[i], n);
+        cell_nodes(S, (u32)kN[i], nn);
+        read_pre(PS, nn, &FN[27 * i]);
+        for (int d = 0; d < 27; ++d) {
+            const int c = ax == 0 ? CX[d] : (ax == 1 ? CY[d] : CZ[d]);
+            if (c * sgn <= 0) G[27 * i + d] = PA[d][n[node_sel(d)]];
+        }
+    }
+    for (long long i = 0; i < (long long)nbc; ++i) {
+        u32 n[8];
+        cell_nodes(S, (u32)k[i], n);
+        for (int d = 0; d < 27; ++d) {
+            const int c = ax == 0 ? CX[d] : (ax == 1 ? CY[d] : CZ[d]);
+            if (c * sgn <= 0) PA[d][n[node_sel(d)]] = 0.0;
+        }
+    }
+    for (long long i = 0; i < (long long)nbc; ++i) {
+        u32 nn[8];
+        cell_nodes(S, (u32)kN[i], nn);
+        const double* f = &FN[27 * i];
+        for (int col = 0; col < 28; ++col) {
+            Dual fd[27], out[27];
+            bool act[27];
+            for (int e = 0; e < 27; ++e) fd[e] = Dual(f[e], col == e ? 1.0 : 0.0);
+            pressure_node<Dual>(fd, Dual(rhoBC[i], col == 27 ? 1.0 : 0.0), ax, sgn, out, act);
+            double sum = 0.0;
+            for (int d = 0; d < 27; ++d)
+                if (act[d]) sum += out[d].d * G[27 * i + d];
+            if (col < 27) PA[col][nn[node_sel(col)]] += sum;
+            else if (rhoBar) rhoBar[i] += sum;
+        }
     }
 }

@@ -278,3 +278,31 @@ def adjoint_step(lattice, lam, fluid, omega, limiters=(0.01, 0.01, 0.01), force=
                                _p(lim), _p(frc), ctypes.c_int(int(lattice.even)), _p(ub))
     if shift_adjoint is not None:
         shift_adjoint += ub.T
+
+
+def bc_post_adjoint(kind, post_lattice, lam, bc_k, q27, omega, velocity=None, ubar=None):
+    """Reverse of Lattice.bc_post: post_lattice.f holds the post-collision state (its parity the
+    one the BC used); lam (27*N, FP64) is updated in place; ubar ((n, 3), optional) accumulates the
+    wall-velocity adjoint."""
+    L = post_lattice
+    assert L.dtype == np.float64 and lam.dtype == np.float64
+    k = np.ascontiguousarray(bc_k, dtype=np.int32)
+    q = np.ascontiguousarray(q27, dtype=np.float64)
+    vx = vy = vz = None
+    if velocity is not None:
+        vx, vy, vz = (np.ascontiguousarray(v, dtype=np.float64) for v in velocity)
+    lib().orc_bc_post_adjoint_f64(ctypes.c_int(kind), _p(L.f), _p(lam), ctypes.c_ulonglong(L.N), _p(L.nbx), _p(L.nby),
+                                  _p(L.nbz), _p(k), _p(q), ctypes.c_ulonglong(k.size), _p(vx), _p(vy), _p(vz),
+                                  ctypes.c_double(omega), ctypes.c_int(int(L.even)), _p(ubar))
+
+
+def bc_pressure_adjoint(lattice, lam, bc_k, kN, rho_bc, direction, rhobar=None):
+    """Reverse of Lattice.bc_pressure: lattice.f holds the state after the step (parity the BC
+    used); lam updated in place; rhobar (optional) accumulates the RhoBC adjoint."""
+    L = lattice
+    k = np.ascontiguousarray(bc_k, dtype=np.int32)
+    kn = np.ascontiguousarray(kN, dtype=np.int32)
+    rb = np.ascontiguousarray(rho_bc, dtype=np.float64)
+    lib().orc_bc_pressure_adjoint_f64(_p(L.f), _p(lam), ctypes.c_ulonglong(L.N), _p(L.nbx), _p(L.nby), _p(L.nbz),
+                                      _p(k), _p(kn), _p(rb), ctypes.c_ulonglong(k.size), ctypes.c_int(int(L.even)),
+                                      ctypes.c_int(direction), _p(rhobar))
