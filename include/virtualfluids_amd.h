@@ -147,6 +147,12 @@ int vf_k17_run_on_indices(const vf_lbm_parameter* para, const uint32_t* indices,
  * boxes; optional fluidMask skips solid nodes (NoSlipDelayBounceBack). */
 int vf_k17_run_box(const vf_lbm_parameter* para, int32_t collisionTemplate, void* stream);
 
+/* The same step out of place: reads para->distributions, writes the stepped populations to
+ * `out` (27*numberOfNodes values of the same storage type, a different buffer); masked nodes'
+ * cells are copied unchanged, so `out` equals what the in-place step leaves. No counterpart in
+ * the reference: it lets the adjoint's replay keep every intermediate state without copies. */
+int vf_k17_run_box_into(const vf_lbm_parameter* para, void* out, void* stream);
+
 /* ---------------------------------------------------------------------------------
  * Boundary conditions: the free launchers BoundaryConditionFactory hands out
  * (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.cpp:49-217).
@@ -188,6 +194,29 @@ int vf_k17_adjoint_box(const vf_lbm_parameter* para, float* adjoint, float* velo
 
 /* Reverse of vf_k17_run (every node of fluidNodeIndices, neighbour arrays). */
 int vf_k17_adjoint(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint, void* stream);
+
+/* Adjoints of the BC kernels, run in reverse order within a step (pressure BC first, then the
+ * post-collision BCs in reverse launch order, then the collision adjoint). For the post BCs
+ * para->distributions holds the populations BEFORE the step (each BC node's collision is
+ * recomputed from them with para's omega, limiters and force); for the pressure BC it holds the
+ * populations after the step. para->isEvenTimestep is the parity the forward kernel used.
+ * The adjoint of every written cell moves to the cells the value was computed from and the
+ * written cell's adjoint is cleared. velocityAdjoint (optional, 3*numberOfBCnodes floats: Vx,
+ * Vy, Vz blocks) and rhoBCAdjoint (optional, numberOfBCnodes) accumulate parameter gradients.
+ * No counterpart in the reference. */
+int vf_NoSlipBounceBackAdjoint(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, float* adjoint,
+                               float* velocityAdjoint, void* stream);
+int vf_NoSlipInterpolatedCompressibleAdjoint(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc,
+                                             float* adjoint, float* velocityAdjoint, void* stream);
+int vf_VelocityBounceBackAdjoint(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, float* adjoint,
+                                 float* velocityAdjoint, void* stream);
+int vf_VelocityInterpolatedCompressibleAdjoint(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc,
+                                               float* adjoint, float* velocityAdjoint, void* stream);
+int vf_VelocityWithPressureInterpolatedCompressibleAdjoint(const vf_lbm_parameter* para,
+                                                           const vf_q_boundary_conditions* bc, float* adjoint,
+                                                           float* velocityAdjoint, void* stream);
+int vf_PressureNonEquilibriumCompressibleAdjoint(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc,
+                                                 float* adjoint, float* rhoBCAdjoint, void* stream);
 
 /* ---------------------------------------------------------------------------------
  * Grid refinement: the launchers GridScalingFactory hands out for

@@ -816,3 +816,39 @@ extern "C" void orc_bc_pressure_adjoint_f64(const double* state, double* lam, un
         }
     }
 }
+
+// Per-node reverse mode of the BC formulas (dual numbers), for checking the GPU's hand-written
+// node functions: G[r] is the adjoint of the value written for incoming direction r.
+extern "C" void orc_bc_node_vjp_f64(int kind, const double* f, const double* q27, const double* U3, double omega,
+                                    const double* G, double* fb, double* Ub)
+{
+    for (int col = 0; col < 30; ++col) {
+        Dual fd[27], out[27];
+        bool act[27];
+        for (int e = 0; e < 27; ++e) fd[e] = Dual(f[e], col == e ? 1.0 : 0.0);
+        const Dual Ux(U3[0], col == 27 ? 1.0 : 0.0), Uy(U3[1], col == 28 ? 1.0 : 0.0), Uz(U3[2], col == 29 ? 1.0 : 0.0);
+        bc_node<Dual, double>(kind, fd, q27, 1, 0, Ux, Uy, Uz, Dual(omega), out, act);
+        double s = 0.0;
+        for (int r = 0; r < 27; ++r)
+            if (act[r]) s += out[r].d * G[r];
+        if (col < 27) fb[col] = s;
+        else Ub[col - 27] = s;
+    }
+}
+
+extern "C" void orc_pressure_node_vjp_f64(const double* fn, double rhoBC, int direction, const double* G, double* fnb,
+                                          double* rb)
+{
+    const int ax = (direction - 1) / 2, sgn = (direction % 2 == 1) ? 1 : -1;
+    for (int col = 0; col < 28; ++col) {
+        Dual fd[27], out[27];
+        bool act[27];
+        for (int e = 0; e < 27; ++e) fd[e] = Dual(fn[e], col == e ? 1.0 : 0.0);
+        pressure_node<Dual>(fd, Dual(rhoBC, col == 27 ? 1.0 : 0.0), ax, sgn, out, act);
+        double s = 0.0;
+        for (int d = 0; d < 27; ++d)
+            if (act[d]) s += out[d].d * G[d];
+        if (col < 27) fnb[col] = s;
+        else *rb = s;
+    }
+}

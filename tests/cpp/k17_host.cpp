@@ -3,6 +3,7 @@
 // compare it with the oracle without a GPU. Test infrastructure; not part of the library.
 #include <cstring>
 
+#include "vf_bc_adjoint.h"
 #include "vf_k17_adjoint.h"
 
 using namespace vfa;
@@ -44,4 +45,33 @@ extern "C" void host_k17_vjp(const float* fin, const float* g, long long n, floa
         k17_vjp(f, gg, r, out, ub3n ? ub3n + 3 * i : nullptr);
         std::memcpy(fb + 27 * i, out, sizeof(out));
     }
+}
+
+// one BC node: post (27), q (27, d-indexed, inactive < 0), G (27, adjoint of the written value of
+// each incoming direction); fb (27) and Ub (3) out
+extern "C" void host_bc_post_vjp(int kind, const float* post, const float* q, float omega, const float* G, float* fb,
+                                 float* Ub)
+{
+    float p[27], qq[27], g[27], out[27], ub[3];
+    std::memcpy(p, post, sizeof(p));
+    std::memcpy(qq, q, sizeof(qq));
+    std::memcpy(g, G, sizeof(g));
+    switch (kind) {
+    case kNoSlipBB: bc_post_vjp<kNoSlipBB>(p, qq, omega, g, out, ub); break;
+    case kVelocityBB: bc_post_vjp<kVelocityBB>(p, qq, omega, g, out, ub); break;
+    case kNoSlipInterp: bc_post_vjp<kNoSlipInterp>(p, qq, omega, g, out, ub); break;
+    case kVelocityInterp: bc_post_vjp<kVelocityInterp>(p, qq, omega, g, out, ub); break;
+    default: bc_post_vjp<kVelocityPressureInterp>(p, qq, omega, g, out, ub); break;
+    }
+    std::memcpy(fb, out, sizeof(out));
+    std::memcpy(Ub, ub, sizeof(ub));
+}
+
+extern "C" void host_pressure_vjp(const float* fn, float rhoBC, const float* G, float* fnb, float* rb)
+{
+    float f[27], g[27], out[27];
+    std::memcpy(f, fn, sizeof(f));
+    std::memcpy(g, G, sizeof(g));
+    pressure_vjp(f, rhoBC, g, out, *rb);
+    std::memcpy(fnb, out, sizeof(out));
 }

@@ -197,3 +197,48 @@ def test_step_with_bcs_adjoint_dot_product(orc, velocity_kind, noslip_kind, q_ra
     assert float(w @ jv) == pytest.approx(float(lam @ v), rel=1e-6)
     assert float(w @ jU) == pytest.approx(float((ubar.T * vU).sum()), rel=1e-6)
     assert float(w @ jR) == pytest.approx(float(rbar @ vR), rel=1e-6)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
+def test_host_build_of_gpu_bc_adjoint_matches_reference(orc, host, kind):
+    """The GPU's per-node reverse mode of the five post-collision BC formulas against the
+    oracle's dual-number reverse mode of the same formulas (random links and distances)."""
+    rng = np.random.default_rng(60 + kind)
+    for trial in range(20):
+        post = near_equilibrium(rng, 1, u_amp=0.05, noise=5e-3)[0]
+        q = np.where(rng.random(27) < 0.4, rng.uniform(0.05, 0.95, 27), -1.0)
+        q[0] = -1.0
+        U = rng.normal(0, 0.03, 3)
+        G = rng.normal(0, 1, 27)
+        omega = 1.85
+        fb_ref, ub_ref = np.empty(27), np.empty(3)
+        orc.lib().orc_bc_node_vjp_f64(ctypes.c_int(kind), _p(post), _p(q), _p(U), ctypes.c_double(omega), _p(G),
+                                      _p(fb_ref), _p(ub_ref))
+        fb, ub = np.empty(27, np.float32), np.empty(3, np.float32)
+        host.host_bc_post_vjp(ctypes.c_int(kind), _p(post.astype(np.float32)), _p(q.astype(np.float32)),
+                              ctypes.c_float(omega), _p(G.astype(np.float32)), _p(fb), _p(ub))
+        scale = max(np.abs(fb_ref).max(), 1e-30)
+        assert np.abs(fb - fb_ref).max() < 1e-5 * scale, (trial, np.abs(fb - fb_ref).max(), scale)
+        if kind in (1, 3, 4):
+            assert np.abs(ub - ub_ref).max() < 1e-5 * max(np.abs(ub_ref).max(), 1e-30)
+
+
+@pytest.mark.parametrize("direction", [1, 2, 3, 4, 5, 6])
+def test_host_build_of_gpu_pressure_adjoint_matches_reference(orc, host, direction):
+    from virtualfluids_amd.grid import CX, CY, CZ
+
+    rng = np.random.default_rng(70 + direction)
+    ax, sgn = (direction - 1) // 2, (1 if direction % 2 == 1 else -1)
+    c = [CX, CY, CZ][ax]
+    for trial in range(10):
+        fn = near_equilibrium(rng, 1, u_amp=0.05, noise=5e-3)[0]
+        rbc = rng.normal(0, 0.01)
+        G = np.where(c * sgn <= 0, rng.normal(0, 1, 27), 0.0)
+        fnb_ref, rb_ref = np.empty(27), np.empty(1)
+        orc.lib().orc_pressure_node_vjp_f64(_p(fn), ctypes.c_double(rbc), ctypes.c_int(direction), _p(G),
+                                            _p(fnb_ref), _p(rb_ref))
+        fnb, rb = np.empty(27, np.float32), np.empty(1, np.float32)
+        host.host_pressure_vjp(_p(fn.astype(np.float32)), ctypes.c_float(rbc), _p(G.astype(np.float32)), _p(fnb),
+                               _p(rb))
+        assert np.abs(fnb - fnb_ref).max() < 1e-5 * np.abs(fnb_ref).max()
+        assert abs(rb[0] - rb_ref[0]) < 1e-5 * abs(rb_ref[0])

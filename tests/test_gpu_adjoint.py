@@ -153,3 +153,163 @@ def test_checkpoint_interval_does_not_change_gradient(gpu, orc):
     grads = [_gpu_gradient(L, f0, idx, steps, K)[1] for K in (1, 4, 9, 20)]
     for g in grads[1:]:
         assert np.array_equal(g, grads[0])
+
+
+# ---- boundary conditions -------------------------------------------------------------------------
+POST_KINDS = {  # C entry point -> oracle kind
+    "vf_NoSlipBounceBack": 0, "vf_VelocityBounceBack": 1, "vf_NoSlipInterpolatedCompressible": 2,
+    "vf_VelocityInterpolatedCompressible": 3, "vf_VelocityWithPressureInterpolatedCompressible": 4,
+}
+
+
+def _duct(q_random, seed=47):
+    from virtualfluids_amd.grid import channel
+
+    g = channel((14, 7, 6), inflow=0.03)
+    if q_random:
+        rng = np.random.default_rng(seed)
+        for bc in g.bcs.values():
+            bc.q27 = np.where(bc.q27 >= 0, rng.uniform(0.15, 0.85, bc.q27.shape), bc.q27)
+    return g
+
+
+@pytest.mark.parametrize("fn", sorted(POST_KINDS))
+@pytest.mark.parametrize("even", [True, False])
+def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even):
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import Lattice
+
+    kind = POST_KINDS[fn]
+    g = _duct(q_random=kind >= 2)
+    bc_name = "velocity" if kind in (1, 3, 4) else "noslip"
+    rng = np.random.default_rng(700 + kind + 10 * even)
+    omega = 1.8
+    L = Lattice(g, omega)
+    L.even = even
+    L.set_populations(pack_pre(near_equilibrium(rng, g.N, u_amp=0.04, noise=5e-3), g, even))
+    lam0 = rng.normal(0, 1, 27 * g.N).astype(np.float32)
+    lam = torch.from_numpy(lam0).cuda()
+    bc = L.bcs[bc_name]
+    ubar = torch.zeros(3 * bc.n, dtype=torch.float32, device="cuda")
+    print(f"{fn}Adjoint even={even} nodes={bc.n}", flush=True)
+    capi.call(fn + "Adjoint", L.pref(), ctypes.byref(bc.struct), ctypes.c_void_p(lam.data_ptr()),
+              ctypes.c_void_p(ubar.data_ptr()), None)
+    got = lam.cpu().numpy().astype(np.float64)
+
+    P = orc.Lattice(g.nbx, g.nby, g.nbz)
+    P.f[:] = L.populations()
+    P.even = even
+    P.collide(g.fluid, omega)  # the post-collision state the forward BC kernel reads
+    ref = lam0.astype(np.float64)
+    ref_u = np.zeros((bc.n, 3))
+    nodes = g.bcs[bc_name]
+    orc.bc_post_adjoint(kind, P, ref, nodes.k, nodes.q27, omega, velocity=nodes.velocity,
+                        ubar=ref_u if kind in (1, 3, 4) else None)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"  rel err {err:.2e}")
+    assert err < 2e-5
+    if kind in (1, 3, 4):
+        gu = ubar.cpu().numpy().reshape(3, bc.n).T
+        assert np.abs(gu - ref_u).max() < 1e-4 * np.abs(ref_u).max()
+
+
+@pytest.mark.parametrize("even", [True, False])
+def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even):
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import Lattice
+
+    g = _duct(False)
+    rng = np.random.default_rng(720 + even)
+    L = Lattice(g, 1.8)
+    L.even = even
+    L.set_populations(pack_pre(near_equilibrium(rng, g.N, u_amp=0.04, noise=5e-3), g, even))
+    lam0 = rng.normal(0, 1, 27 * g.N).astype(np.float32)
+    lam = torch.from_numpy(lam0).cuda()
+    bc = L.bcs["pressure"]
+    rbar = torch.zeros(bc.n, dtype=torch.float32, device="cuda")
+    capi.call("vf_PressureNonEquilibriumCompressibleAdjoint", L.pref(), ctypes.byref(bc.struct),
+              ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(rbar.data_ptr()), None)
+    A = orc.Lattice(g.nbx, g.nby, g.nbz)
+    A.f[:] = L.populations()
+    A.even = even
+    ref = lam0.astype(np.float64)
+    ref_r = np.zeros(bc.n)
+    pr = g.bcs["pressure"]
+    orc.bc_pressure_adjoint(A, ref, pr.k, pr.kN, pr.rho_bc, pr.direction, rhobar=ref_r)
+    err = np.abs(lam.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print(f"pressure adjoint even={even}: rel err {err:.2e}")
+    assert err < 2e-5
+    assert np.abs(rbar.cpu().numpy() - ref_r).max() < 1e-4 * np.abs(ref_r).max()
+
+
+@pytest.mark.parametrize("velocity_bc,noslip_bc", [
+    ("VelocityBounceBack", "NoSlipBounceBack"),
+    ("VelocityInterpolatedCompressible", "NoSlipInterpolatedCompressible"),
+])
+def test_duct_gradients_match_fp64_finite_differences(gpu, orc, velocity_bc, noslip_bc):
+    """20 steps of a duct with velocity inflow, pressure outlet and walls: the gradient of the
+    FP64 kinetic energy with respect to the initial populations, the inflow wall velocities and
+    the outlet densities, against FP64 central differences of the oracle's forward run."""
+    import torch
+
+    from helpers import ORACLE_POST_BC
+    from virtualfluids_amd.adjoint import k17_steps, kinetic_energy, pre_collision_index
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory
+
+    g = _duct(q_random=velocity_bc.startswith("VelocityInterp"))
+    vel, ns, pr = g.bcs["velocity"], g.bcs["noslip"], g.bcs["pressure"]
+    omega, steps = 1.8, 20
+    rng = np.random.default_rng(730)
+    f0 = pack_pre(near_equilibrium(rng, g.N, u_amp=0.03, noise=2e-3), g, True)
+    U0 = np.array(vel.velocity, dtype=np.float32).astype(np.float64)
+    R0 = np.full(pr.n, 0.002, dtype=np.float32).astype(np.float64)
+    fluid = g.geo == 19
+
+    fac = BoundaryConditionFactory()
+    fac.set_velocity_boundary_condition(BoundaryConditionFactory.VelocityBC[velocity_bc])
+    fac.set_no_slip_boundary_condition(BoundaryConditionFactory.NoSlipBC[noslip_bc])
+    fac.set_pressure_boundary_condition(BoundaryConditionFactory.PressureBC.PressureNonEquilibriumCompressible)
+    L = Lattice(g, omega)
+    idx = torch.from_numpy(pre_collision_index(g, steps % 2 == 0)).cuda()
+    nodes = torch.from_numpy(np.nonzero(fluid)[0]).cuda()
+    x = torch.from_numpy(f0.astype(np.float32)).cuda().requires_grad_(True)
+    U = torch.from_numpy(U0.astype(np.float32)).cuda().requires_grad_(True)
+    R = torch.from_numpy(R0.astype(np.float32)).cuda().requires_grad_(True)
+    J = kinetic_energy(k17_steps(x, L, steps, checkpoint_every=6, even=True, bc_factory=fac, velocity=U, rho_bc=R),
+                       idx, nodes)
+    J.backward()
+
+    def oracle_J(buf, Uv, Rv):
+        A = orc.Lattice(g.nbx, g.nby, g.nbz)
+        A.f[:] = buf
+        for _ in range(steps):
+            A.collide(g.fluid, omega)
+            A.bc_post(ORACLE_POST_BC[velocity_bc], vel.k, vel.q27, omega, tuple(Uv))
+            A.bc_post(ORACLE_POST_BC[noslip_bc], ns.k, ns.q27, omega)
+            A.swap()
+            A.bc_pressure(pr.k, pr.kN, Rv, pr.direction)
+        f = A.f[pre_collision_index(g, A.even)]
+        from virtualfluids_amd.grid import CX, CY, CZ
+
+        j = np.stack([CX, CY, CZ]).astype(np.float64) @ f
+        ke = 0.5 * (j * j).sum(0) / (1 + f.sum(0))
+        return ke[fluid].sum()
+
+    h = 1e-7
+    for name, grad, base in (("f0", x.grad, f0), ("U", U.grad, U0), ("rhoBC", R.grad, R0)):
+        v = rng.normal(0, 1, base.shape)
+        if name == "f0":
+            Jp, Jm = oracle_J(f0 + h * v, U0, R0), oracle_J(f0 - h * v, U0, R0)
+        elif name == "U":
+            Jp, Jm = oracle_J(f0, U0 + h * v, R0), oracle_J(f0, U0 - h * v, R0)
+        else:
+            Jp, Jm = oracle_J(f0, U0, R0 + h * v), oracle_J(f0, U0, R0 - h * v)
+        fd = (Jp - Jm) / (2 * h)
+        ad = float((grad.cpu().numpy().astype(np.float64) * v).sum())
+        print(f"{velocity_bc}: d J / d {name}: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
+        assert abs(ad - fd) < 1e-4 * abs(fd)

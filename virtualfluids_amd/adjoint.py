@@ -1,20 +1,20 @@
-"""Differentiable K17 time stepping: the hand-written adjoint kernel behind torch.autograd.
+"""Differentiable K17 time stepping: the hand-written adjoint kernels behind torch.autograd.
 
-``k17_steps(f0, lattice, steps, checkpoint_every)`` runs `steps` collide+stream steps of one
-Lattice starting from the population buffer f0 and returns the final buffer. Its backward runs
-the reverse sweep with vf_k17_adjoint_box / vf_k17_adjoint (hand-written reverse mode of the
-FP32 collision): the forward pass keeps a copy of the buffer every `checkpoint_every` steps; the
-backward replays one segment at a time from its checkpoint, keeps the segment's states, and
-applies the adjoint steps in reverse order with the parity each forward step used. Memory:
-(steps / K + K) buffers of the storage type. The adjoint itself is FP32.
+``k17_steps(f0, lattice, steps, checkpoint_every, bc_factory, velocity, rho_bc)`` runs
+`steps` time steps of one Lattice from the population buffer f0 with the order of
+UpdateGrid27 (collision, velocity BC, no-slip BC, parity swap, pressure BC;
+src/gpu/core/Calculation/UpdateGrid27.cpp:54-96) and returns the final buffer. Its backward
+sweeps the steps in reverse with the adjoint kernels (vf_k17_adjoint[_box] and the BC
+adjoints): pressure BC, then no-slip, then velocity, then the collision. The forward keeps a
+copy of the buffer every `checkpoint_every` steps; the backward replays one segment at a time
+from its checkpoint (out of place on the dense box, so the replay writes each intermediate
+state once and copies nothing) and walks the segment backwards. Memory: steps/K + K + 1
+buffers of the storage type. Gradients flow to f0 and, when given, to the wall velocities of
+the velocity BC (3, n) and the outlet densities of the pressure BC (n,).
 
 Objectives are ordinary torch functions of the returned buffer; ``pre_collision_index`` gives
 the gather that turns a buffer into the (27, N) pre-collision populations of every node, and
-``kinetic_energy`` is the FP64 sum of 1/2 |j|^2 / rho over the fluid nodes
-(src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:99-153).
-
-The differentiable step covers the collision, streaming and solid nodes (no-slip delay
-bounce-back); lattices with BC kernels (velocity, pressure, explicit no-slip) are rejected.
+``kinetic_energy`` is the FP64 sum of 1/2 |j|^2 / rho (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:99-153).
 No counterpart in the reference.
 """
 from __future__ import annotations
@@ -27,89 +27,157 @@ import torch
 from . import capi
 from .grid import CX, CY, CZ, INV
 from .lattice import Lattice, stream_handle
+from .simulation import BoundaryConditionFactory
 
 
-def _param_copy(L: Lattice, state: torch.Tensor, even: bool) -> capi.LBMParameter:
+def _param(L: Lattice, state: torch.Tensor, even: bool) -> capi.LBMParameter:
     p = capi.LBMParameter.from_buffer_copy(L.param)
     p.distributions = ctypes.c_void_p(state.data_ptr())
     p.isEvenTimestep = 1 if even else 0
     return p
 
 
-def _check_differentiable(L: Lattice):
-    for name, bc in L.bcs.items():
-        if bc.n > 0:
-            raise NotImplementedError(f"the differentiable step has no adjoint of the {name} BC kernel")
+def _vp(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
-def _forward_step(L: Lattice, buf: torch.Tensor, even: bool, stream):
-    p = _param_copy(L, buf, even)
-    if L.box:
-        capi.call("vf_k17_run_box", ctypes.byref(p), capi.COLLISION_DEFAULT, stream_handle(stream))
-    else:
-        capi.call("vf_k17_run_on_indices", ctypes.byref(p), ctypes.c_void_p(L.fluid.data_ptr()),
-                  int(L.grid.fluid.size), capi.COLLISION_DEFAULT, stream_handle(stream))
+class Stepper:
+    """One time step of a Lattice on any buffer, and its adjoint, with the BC functions a
+    BoundaryConditionFactory selects (the same selection and checks as Simulation)."""
 
+    def __init__(self, L: Lattice, bc_factory: BoundaryConditionFactory | None = None):
+        f = bc_factory or BoundaryConditionFactory()
+        self.L = L
+        self.velocity = f.get_velocity_boundary_condition_post()
+        self.noslip = f.get_no_slip_boundary_condition_post()
+        self.pressure = f.get_pressure_boundary_condition_pre()
+        for name, fn, label in (("velocity", self.velocity, "velocityBoundaryConditionPost"),
+                                ("pressure", self.pressure, "directionalPressureBoundaryConditionPre")):
+            bc = L.bcs.get(name)
+            if bc is not None and bc.n > 0 and fn is None:
+                raise RuntimeError(f"The boundary condition {label} was not set!")
 
-def _adjoint_step(L: Lattice, state: torch.Tensor, lam: torch.Tensor, even: bool, shift_adj, stream):
-    p = _param_copy(L, state, even)
-    sa = None if shift_adj is None else ctypes.c_void_p(shift_adj.data_ptr())
-    fn = "vf_k17_adjoint_box" if L.box else "vf_k17_adjoint"
-    capi.call(fn, ctypes.byref(p), ctypes.c_void_p(lam.data_ptr()), sa, stream_handle(stream))
+    def _bc(self, name):
+        bc = self.L.bcs.get(name)
+        return bc if bc is not None and bc.n > 0 else None
+
+    def forward(self, buf: torch.Tensor, even: bool, out: torch.Tensor | None, stream) -> torch.Tensor:
+        """One step from `buf` (parity `even`); into `out` when given, else in place. Returns
+        the buffer holding the result."""
+        L, s = self.L, stream_handle(stream)
+        if out is not None:
+            if L.box:
+                capi.call("vf_k17_run_box_into", ctypes.byref(_param(L, buf, even)), _vp(out), s)
+            else:
+                out.copy_(buf)
+                capi.call("vf_k17_run_on_indices", ctypes.byref(_param(L, out, even)), _vp(L.fluid),
+                          int(L.grid.fluid.size), capi.COLLISION_DEFAULT, s)
+            target = out
+        else:
+            target = buf
+            p = _param(L, buf, even)
+            if L.box:
+                capi.call("vf_k17_run_box", ctypes.byref(p), capi.COLLISION_DEFAULT, s)
+            else:
+                capi.call("vf_k17_run_on_indices", ctypes.byref(p), _vp(L.fluid), int(L.grid.fluid.size),
+                          capi.COLLISION_DEFAULT, s)
+        p = _param(L, target, even)
+        for name, fn in (("velocity", self.velocity), ("noslip", self.noslip)):
+            bc = self._bc(name)
+            if fn and bc is not None:
+                capi.call(fn, ctypes.byref(p), ctypes.byref(bc.struct), s)
+        bc = self._bc("pressure")
+        if self.pressure and bc is not None:
+            capi.call(self.pressure, ctypes.byref(_param(L, target, not even)), ctypes.byref(bc.struct), s)
+        return target
+
+    def adjoint(self, before: torch.Tensor, after: torch.Tensor, lam: torch.Tensor, even: bool, ubar, rbar, stream):
+        """Reverse of one step: `before`/`after` are the states around it, `even` its parity;
+        lam (FP32) is updated in place; ubar (3*n) / rbar (n) accumulate BC parameter adjoints."""
+        L, s = self.L, stream_handle(stream)
+        bc = self._bc("pressure")
+        if self.pressure and bc is not None:
+            capi.call(self.pressure + "Adjoint", ctypes.byref(_param(L, after, not even)), ctypes.byref(bc.struct),
+                      _vp(lam), _vp(rbar), s)
+        p = _param(L, before, even)
+        for name, fn, grad in (("noslip", self.noslip, None), ("velocity", self.velocity, ubar)):
+            bc = self._bc(name)
+            if fn and bc is not None:
+                capi.call(fn + "Adjoint", ctypes.byref(p), ctypes.byref(bc.struct), _vp(lam), _vp(grad), s)
+        fn = "vf_k17_adjoint_box" if L.box else "vf_k17_adjoint"
+        capi.call(fn, ctypes.byref(p), _vp(lam), None, s)
 
 
 class _K17Steps(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, f0, L, steps, checkpoint_every, even0):
+    def forward(ctx, f0, velocity, rho_bc, stepper, steps, checkpoint_every, even0):
+        L = stepper.L
         if f0.dtype != L.f.dtype or f0.numel() != L.f.numel() or not f0.is_cuda:
             raise ValueError("f0 must be a device buffer of the lattice's storage type and size")
         stream = torch.cuda.current_stream()
-        buf = f0.detach().clone()
+        if velocity is not None:
+            bc = L.bcs["velocity"]
+            for i in range(3):
+                bc.vel[i].copy_(velocity[i].detach().to(torch.float32))
+        if rho_bc is not None:
+            L.bcs["pressure"].rho_bc.copy_(rho_bc.detach().to(torch.float32))
+        bufs = [f0.detach().clone(), torch.empty_like(f0)]
+        cur = 0
         checkpoints = []
         even = even0
         for i in range(steps):
             if i % checkpoint_every == 0:
-                checkpoints.append((i, even, buf.clone()))
-            _forward_step(L, buf, even, stream)
+                checkpoints.append((i, even, bufs[cur].clone()))
+            res = stepper.forward(bufs[cur], even, bufs[1 - cur], stream)
+            cur = 1 - cur if res is bufs[1 - cur] else cur
             even = not even
-        ctx.L, ctx.steps, ctx.K, ctx.even0 = L, steps, checkpoint_every, even0
+        ctx.stepper, ctx.steps, ctx.K = stepper, steps, checkpoint_every
         ctx.checkpoints = checkpoints
-        return buf
+        ctx.has_velocity, ctx.has_rho = velocity is not None, rho_bc is not None
+        return bufs[cur]
 
     @staticmethod
     def backward(ctx, grad_out):
-        L, steps, K = ctx.L, ctx.steps, ctx.K
+        stepper, steps, K = ctx.stepper, ctx.steps, ctx.K
+        L = stepper.L
         stream = torch.cuda.current_stream()
         lam = grad_out.detach().to(torch.float32).contiguous().clone()
+        vel_bc, pr_bc = L.bcs.get("velocity"), L.bcs.get("pressure")
+        ubar = torch.zeros(3 * vel_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_velocity else None
+        rbar = torch.zeros(pr_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_rho else None
         for start, even_start, ckpt in reversed(ctx.checkpoints):
             n = min(K, steps - start)
             states = [ckpt]
-            work = ckpt.clone()
             even = even_start
-            for j in range(n - 1):
-                _forward_step(L, work, even, stream)
+            for _ in range(n):
+                states.append(stepper.forward(states[-1], even, torch.empty_like(ckpt), stream))
                 even = not even
-                states.append(work.clone())
             for j in reversed(range(n)):
-                _adjoint_step(L, states[j], lam, even_start ^ (j % 2 == 1), None, stream)
-            del states, work
+                stepper.adjoint(states[j], states[j + 1], lam, even_start ^ (j % 2 == 1), ubar, rbar, stream)
+            del states
         ctx.checkpoints = None
-        return lam.to(grad_out.dtype), None, None, None, None
+        gv = ubar.view(3, -1) if ubar is not None else None
+        return lam.to(grad_out.dtype), gv, rbar, None, None, None, None
 
 
-def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: int = 10,
-              even: bool | None = None) -> torch.Tensor:
-    """`steps` collide+stream steps of `lattice` from buffer f0 (differentiable in f0).
+def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: int = 10, even: bool | None = None,
+              bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
+              rho_bc: torch.Tensor | None = None) -> torch.Tensor:
+    """`steps` time steps of `lattice` from buffer f0, differentiable in f0 and, when given, in
+    the velocity BC's wall velocities `velocity` (3, n) and the pressure BC's densities `rho_bc`.
 
-    The lattice's own buffer and parity are not touched; `even` is the parity of the first
-    step (default: the lattice's current parity). The result is a new buffer whose parity is
-    flipped `steps` times.
+    The lattice's own buffer and parity are not touched (its BC parameter arrays take the given
+    values); `even` is the parity of the first step (default: the lattice's current parity).
     """
-    _check_differentiable(lattice)
     if steps < 0 or checkpoint_every < 1:
         raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
+    stepper = Stepper(lattice, bc_factory)
+    if velocity is not None and lattice.bcs.get("velocity") is None:
+        raise ValueError("the lattice has no velocity BC nodes")
+    if rho_bc is not None and lattice.bcs.get("pressure") is None:
+        raise ValueError("the lattice has no pressure BC nodes")
     even0 = lattice.even if even is None else bool(even)
-    return _K17Steps.apply(f0, lattice, steps, checkpoint_every, even0)
+    return _K17Steps.apply(f0, velocity, rho_bc, stepper, steps, checkpoint_every, even0)
 
 
 def pre_collision_index(grid, even: bool) -> np.ndarray:
@@ -130,24 +198,22 @@ def pre_collision_index(grid, even: bool) -> np.ndarray:
     return out
 
 
-_CXYZ = None
+_CXYZ = {}
 
 
 def node_moments(buf: torch.Tensor, index: torch.Tensor):
     """drho (N) and momentum j (3, N) in FP64 from a buffer and its pre-collision index."""
-    global _CXYZ
     f = buf.reshape(-1)[index].to(torch.float64)  # (27, N)
-    if _CXYZ is None or _CXYZ.device != f.device:
-        _CXYZ = torch.tensor(np.stack([CX, CY, CZ]), dtype=torch.float64, device=f.device)
-    drho = f.sum(0)
-    j = _CXYZ @ f
-    return drho, j
+    c = _CXYZ.get(f.device)
+    if c is None:
+        c = _CXYZ[f.device] = torch.tensor(np.stack([CX, CY, CZ]), dtype=torch.float64, device=f.device)
+    return f.sum(0), c @ f
 
 
-def kinetic_energy(buf: torch.Tensor, index: torch.Tensor, fluid: torch.Tensor | None = None) -> torch.Tensor:
-    """FP64 sum over nodes (or the fluid subset) of 1/2 |j|^2 / (1 + drho)."""
+def kinetic_energy(buf: torch.Tensor, index: torch.Tensor, nodes: torch.Tensor | None = None) -> torch.Tensor:
+    """FP64 sum over all nodes (or the given node subset) of 1/2 |j|^2 / (1 + drho)."""
     drho, j = node_moments(buf, index)
     ke = 0.5 * (j * j).sum(0) / (1.0 + drho)
-    if fluid is not None:
-        ke = ke[fluid]
+    if nodes is not None:
+        ke = ke[nodes]
     return ke.sum()

@@ -99,6 +99,7 @@ SIGNATURES = {
     "vf_k17_run": [_P, _vp],
     "vf_k17_run_on_indices": [_P, _vp, ctypes.c_uint32, ctypes.c_int32, _vp],
     "vf_k17_run_box": [_P, ctypes.c_int32, _vp],
+    "vf_k17_run_box_into": [_P, _vp, _vp],
     "vf_NoSlipBounceBack": [_P, _Q, _vp],
     "vf_NoSlipInterpolatedCompressible": [_P, _Q, _vp],
     "vf_VelocityBounceBack": [_P, _Q, _vp],
@@ -106,13 +107,19 @@ SIGNATURES = {
     "vf_VelocityWithPressureInterpolatedCompressible": [_P, _Q, _vp],
     "vf_PressureNonEquilibriumCompressible": [_P, _Q, _vp],
     "vf_InitNavierStokesCompressible": [_P, _vp],
-    "vf_InitNavierStokesCompressibleNonEquilibrium": [_P, _vp, _vp],
     "vf_CalculateMacroscopicQuantitiesCompressible": [_P, _vp],
     "vf_ConservationSums": [_P, _vp, _vp],
     "vf_MomentSums": [_P, _vp, _vp],
     "vf_ScaleCoarseToFineCompressible": [_P, _P, _vp, _vp, _vp],
     "vf_k17_adjoint_box": [_P, _vp, _vp, _vp],
     "vf_k17_adjoint": [_P, _vp, _vp, _vp],
+    "vf_NoSlipBounceBackAdjoint": [_P, _Q, _vp, _vp, _vp],
+    "vf_NoSlipInterpolatedCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
+    "vf_VelocityBounceBackAdjoint": [_P, _Q, _vp, _vp, _vp],
+    "vf_VelocityInterpolatedCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
+    "vf_VelocityWithPressureInterpolatedCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
+    "vf_PressureNonEquilibriumCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
+    "vf_InitNavierStokesCompressibleNonEquilibrium": [_P, _vp, _vp],
     "vf_ScaleFineToCoarseCompressible": [_P, _P, _vp, _vp, _vp],
 }
 

@@ -10,11 +10,10 @@
 //   VelocityBounceBack        f_inv(d) = f_d - 6 w_d c_d.U                      (VelocityBounceBack.cu:150-185)
 //   *InterpolatedCompressible q-interpolated bounce-back with pre-collision reconstruction
 // The pressure BC is a pre-collision kernel run after the swap (PressureNonEquilibriumCompressible.cu).
+#include "vf_bc_adjoint.h" // BcKind
 #include "vf_lattice.h"
 
 namespace vfa {
-
-enum BcKind : int { kNoSlipBB = 0, kVelocityBB = 1, kNoSlipInterp = 2, kVelocityInterp = 3, kVelocityPressureInterp = 4 };
 
 struct BcArgs {
     const int32_t* k;

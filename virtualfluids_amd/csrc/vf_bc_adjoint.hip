@@ -1,0 +1,239 @@
+// Adjoints of the boundary-condition kernels (no counterpart in the reference: the reverse
+// mode behind the differentiable time step).
+//
+// A post-collision BC kernel writes, for every active link d of a BC node, one cell W (the
+// node's pre-collision inv(d) cell of the next step) with a function of the node's
+// post-collision values. Its adjoint reads the adjoint at W, adds (d value / d post)^T to the
+// adjoint of the node's post-collision cells, and clears W (its previous content was
+// overwritten). The pressure BC (pre-collision, after the swap) writes the BC node's cells from
+// the neighbour kN; its adjoint moves the written cells' adjoint to kN's cells. Linearisation
+// points: the pre-collision state of the step for the post BCs (each BC node's collision is
+// recomputed from it, so no post-collision copy of the lattice is kept), the state after the
+// step for the pressure BC, with the parity each kernel used. Written cells are assumed to belong to nodes that no
+// other BC node of the same list reads (links end outside the fluid; kN is not itself a
+// pressure-BC node), which is how the grid builders lay boundaries out.
+#include "vf_bc_adjoint.h"
+#include "vf_lattice.h"
+
+namespace vfa {
+
+namespace {
+
+struct BcAdjArgs {
+    const int32_t* k;
+    const float* q;
+    uint32_t nbc;
+    const float* vx;
+    const float* vy;
+    const float* vz;
+    float omega;
+    float* ubar; // 3*nbc (Vx block, Vy block, Vz block) or null
+    K17Rates rates;
+};
+
+__device__ __forceinline__ float* lam_cell(float* lam, const Addr& a, const uint32_t (&n)[8], int d, bool even)
+{
+    return slot_ptr(lam, a.N, d, even) + n[node_sel(d)];
+}
+
+template <class S, int KIND>
+__global__ __launch_bounds__(256) void bc_post_adjoint_kernel(const S* __restrict__ f, float* __restrict__ lam, Addr a,
+                                                              BcAdjArgs b, int even_flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.nbc) return;
+    const bool even = even_flag != 0;
+    uint32_t n[8];
+    cell_nodes(a, (uint32_t)b.k[i], n);
+    // the node's post-collision values, recomputed from its pre-collision values
+    float post[27], q[27], G[27], fb[27], Ub[3];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        post[d] = read_pre(f, a, n, d, even);
+        q[d] = d == 0 ? -1.0f : b.q[(uint64_t)d * b.nbc + i];
+    }
+    MacroOut mo;
+    k17_collide(post, b.rates, mo);
+    // adjoint of the written cells (incoming direction inv(d), next parity), then cleared
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        G[inv(d)] = 0.0f;
+        if (!(q[d] >= 0.0f && q[d] <= 1.0f)) continue;
+        float* W = lam_cell(lam, a, n, inv(d), !even);
+        G[inv(d)] = *W;
+        *W = 0.0f;
+    }
+    bc_post_vjp<KIND>(post, q, b.omega, G, fb, Ub);
+    // the post-collision value e sits in the cell of pre(inv(e))
+#pragma unroll
+    for (int e = 0; e < 27; ++e) *lam_cell(lam, a, n, inv(e), even) += fb[e];
+    if (b.ubar) {
+        b.ubar[i] += Ub[0];
+        b.ubar[b.nbc + i] += Ub[1];
+        b.ubar[2 * b.nbc + i] += Ub[2];
+    }
+}
+
+struct PressureAdjArgs {
+    const int32_t* k;
+    const int32_t* kN;
+    const float* rhoBC;
+    uint32_t nbc;
+    int axis, sign;
+    float* rhobar; // nbc or null
+};
+
+template <class S>
+__global__ __launch_bounds__(256) void pressure_adjoint_kernel(const S* __restrict__ f, float* __restrict__ lam, Addr a,
+                                                               PressureAdjArgs b, int even_flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.nbc) return;
+    const bool even = even_flag != 0;
+    uint32_t nn[8], n[8];
+    cell_nodes(a, (uint32_t)b.kN[i], nn);
+    cell_nodes(a, (uint32_t)b.k[i], n);
+    float fn[27], G[27], fnb[27], rb;
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        fn[d] = read_pre(f, a, nn, d, even);
+        const int c = b.axis == 0 ? cx(d) : (b.axis == 1 ? cy(d) : cz(d));
+        G[d] = 0.0f;
+        if (c * b.sign > 0) continue;
+        float* cell = lam_cell(lam, a, n, d, even);
+        G[d] = *cell;
+        *cell = 0.0f;
+    }
+    pressure_vjp(fn, b.rhoBC[i], G, fnb, rb);
+#pragma unroll
+    for (int e = 0; e < 27; ++e) atomicAdd(lam_cell(lam, a, nn, e, even), fnb[e]);
+    if (b.rhobar) b.rhobar[i] += rb;
+}
+
+template <template <class> class K, class... A> int by_storage(int storage, A&&... args)
+{
+    switch (storage) {
+    case VF_STORAGE_FP32: K<float>::run(args...); return VF_OK;
+    case VF_STORAGE_FP16: K<__half>::run(args...); return VF_OK;
+    case VF_STORAGE_BF16: K<__hip_bfloat16>::run(args...); return VF_OK;
+    default: set_last_error("unknown storage type"); return VF_ERR_INVALID_ARGUMENT;
+    }
+}
+
+template <int KIND> struct PostAdjL {
+    template <class S> struct L {
+        static void run(const vf_lbm_parameter& p, const BcAdjArgs& b, float* lam, hipStream_t s)
+        {
+            const dim3 grid((b.nbc + 255) / 256);
+            hipLaunchKernelGGL((bc_post_adjoint_kernel<S, KIND>), grid, dim3(256), 0, s, (const S*)p.distributions, lam,
+                               make_addr(p), b, p.isEvenTimestep);
+        }
+    };
+};
+
+template <class S> struct PressAdjL {
+    static void run(const vf_lbm_parameter& p, const PressureAdjArgs& b, float* lam, hipStream_t s)
+    {
+        const dim3 grid((b.nbc + 255) / 256);
+        hipLaunchKernelGGL(pressure_adjoint_kernel<S>, grid, dim3(256), 0, s, (const S*)p.distributions, lam,
+                           make_addr(p), b, p.isEvenTimestep);
+    }
+};
+
+int check_common(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc, const float* lam)
+{
+    if (!p || !bc || !p->distributions || !lam) {
+        set_last_error("adjoint BC: null parameter, boundary condition or adjoint buffer");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (!p->neighborX && (uint64_t)p->nx * p->ny * p->nz != p->numberOfNodes) {
+        set_last_error("adjoint BC: needs neighbour arrays or a dense box");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    return VF_OK;
+}
+
+template <int KIND>
+int launch_post_adjoint(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc, float* lam, float* ubar,
+                        void* stream, const char* name)
+{
+    int rc = check_common(p, bc, lam);
+    if (rc) return rc;
+    if (bc->numberOfBCnodes == 0) return VF_OK;
+    if (!bc->k || !bc->q27) {
+        set_last_error("adjoint BC: needs k and q27");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    constexpr bool velocity = KIND == kVelocityBB || KIND == kVelocityInterp || KIND == kVelocityPressureInterp;
+    if (velocity && (!bc->Vx || !bc->Vy || !bc->Vz)) {
+        set_last_error("adjoint velocity BC: needs Vx, Vy, Vz");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (KIND >= kNoSlipInterp && !(p->omega > 0.0f && p->omega < 2.0f && p->omega != 1.0f)) {
+        set_last_error("adjoint interpolated BC: omega must lie in (0, 2) and differ from 1");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    const BcAdjArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega, velocity ? ubar : nullptr,
+                      make_k17_rates(*p)};
+    rc = by_storage<PostAdjL<KIND>::template L>(p->storage, *p, b, lam, (hipStream_t)stream);
+    return rc ? rc : check_launch(name);
+}
+
+} // namespace
+} // namespace vfa
+
+using namespace vfa;
+
+extern "C" int vf_NoSlipBounceBackAdjoint(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc, float* adjoint,
+                                          float* velocityAdjoint, void* stream)
+{
+    return launch_post_adjoint<kNoSlipBB>(p, bc, adjoint, velocityAdjoint, stream, "NoSlipBounceBack adjoint");
+}
+extern "C" int vf_NoSlipInterpolatedCompressibleAdjoint(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc,
+                                                        float* adjoint, float* velocityAdjoint, void* stream)
+{
+    return launch_post_adjoint<kNoSlipInterp>(p, bc, adjoint, velocityAdjoint, stream,
+                                              "NoSlipInterpolatedCompressible adjoint");
+}
+extern "C" int vf_VelocityBounceBackAdjoint(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc, float* adjoint,
+                                            float* velocityAdjoint, void* stream)
+{
+    return launch_post_adjoint<kVelocityBB>(p, bc, adjoint, velocityAdjoint, stream, "VelocityBounceBack adjoint");
+}
+extern "C" int vf_VelocityInterpolatedCompressibleAdjoint(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc,
+                                                          float* adjoint, float* velocityAdjoint, void* stream)
+{
+    return launch_post_adjoint<kVelocityInterp>(p, bc, adjoint, velocityAdjoint, stream,
+                                                "VelocityInterpolatedCompressible adjoint");
+}
+extern "C" int vf_VelocityWithPressureInterpolatedCompressibleAdjoint(const vf_lbm_parameter* p,
+                                                                      const vf_q_boundary_conditions* bc, float* adjoint,
+                                                                      float* velocityAdjoint, void* stream)
+{
+    return launch_post_adjoint<kVelocityPressureInterp>(p, bc, adjoint, velocityAdjoint, stream,
+                                                        "VelocityWithPressureInterpolatedCompressible adjoint");
+}
+
+extern "C" int vf_PressureNonEquilibriumCompressibleAdjoint(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc,
+                                                            float* adjoint, float* rhoBCAdjoint, void* stream)
+{
+    int rc = check_common(p, bc, adjoint);
+    if (rc) return rc;
+    if (bc->numberOfBCnodes == 0) return VF_OK;
+    if (!bc->k || !bc->kN || !bc->RhoBC) {
+        set_last_error("adjoint pressure BC: needs k, kN and RhoBC");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    PressureAdjArgs b{bc->k, bc->kN, bc->RhoBC, bc->numberOfBCnodes, 0, 0, rhoBCAdjoint};
+    switch (bc->direction) {
+    case dP00: b.axis = 0; b.sign = 1; break;
+    case dM00: b.axis = 0; b.sign = -1; break;
+    case d0P0: b.axis = 1; b.sign = 1; break;
+    case d0M0: b.axis = 1; b.sign = -1; break;
+    case d00P: b.axis = 2; b.sign = 1; break;
+    case d00M: b.axis = 2; b.sign = -1; break;
+    default: return VF_OK; // the forward kernel writes nothing for other directions
+    }
+    rc = by_storage<PressAdjL>(p->storage, *p, b, adjoint, (hipStream_t)stream);
+    return rc ? rc : check_launch("PressureNonEquilibriumCompressible adjoint");
+}

@@ -86,6 +86,40 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, B
     scatter<S, EVEN>(f, a.N, n, v);
 }
 
+// Out-of-place variant: reads `in`, writes the stepped state to `out` (same layout). Masked
+// nodes copy their 27 cells unchanged, so `out` is the complete state the in-place step
+// would leave. Used to replay a segment of steps while keeping every intermediate state.
+template <class S, bool EVEN, bool MASK>
+__global__ __launch_bounds__(kBoxBlock) void k17_box_into_kernel(const S* __restrict__ in, S* __restrict__ out,
+                                                                 BoxArgs a, K17Rates r)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int z = blockIdx.z;
+    if (x >= a.nx) return;
+    const int xp = (x + 1 == a.nx) ? 0 : x + 1;
+    const int yp = (y + 1 == a.ny) ? 0 : y + 1;
+    const int zp = (z + 1 == a.nz) ? 0 : z + 1;
+    const uint32_t row00 = (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)z);
+    const uint32_t row10 = (uint32_t)a.nx * ((uint32_t)yp + (uint32_t)a.ny * (uint32_t)z);
+    const uint32_t row01 = (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)zp);
+    const uint32_t row11 = (uint32_t)a.nx * ((uint32_t)yp + (uint32_t)a.ny * (uint32_t)zp);
+    const uint32_t n[8] = {row00 + x, row00 + xp, row10 + x, row10 + xp, row01 + x, row01 + xp, row11 + x, row11 + xp};
+    float v[27];
+    gather<S, EVEN>(in, a.N, n, v);
+    if (MASK && a.mask[n[0]] == 0) {
+        // unchanged cells: value of pre(d) back into its own cell (scatter writes v[inv(d)])
+        float w[27];
+#pragma unroll
+        for (int d = 0; d < 27; ++d) w[inv(d)] = v[d];
+        scatter<S, EVEN>(out, a.N, n, w);
+        return;
+    }
+    MacroOut mo;
+    k17_collide(v, r, mo);
+    scatter<S, EVEN>(out, a.N, n, v);
+}
+
 // ---- row kernel: one workgroup per x-row, V consecutive x-nodes per lane ----------------
 // Every global access of a wave is an aligned, contiguous, non-temporal vector access.
 // The nine directions with c_x = -1 live one node further in +x (node_sel bit 0); a lane
@@ -353,6 +387,20 @@ static int dispatch_storage(const vf_lbm_parameter& p, A&&... args)
 template <class S, bool EVEN> struct BoxL {
     static void run(const vf_lbm_parameter& p, bool macro, hipStream_t s) { launch_box_t<S, EVEN>(p, macro, s); }
 };
+template <class S, bool EVEN> struct BoxIntoL {
+    static void run(const vf_lbm_parameter& p, void* out, hipStream_t s)
+    {
+        BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, nullptr, nullptr, nullptr, nullptr};
+        const K17Rates r = make_k17_rates(p);
+        const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
+        const dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
+        const S* in = static_cast<const S*>(p.distributions);
+        if (p.fluidMask)
+            hipLaunchKernelGGL((k17_box_into_kernel<S, EVEN, true>), grid, dim3(bx), 0, s, in, (S*)out, a, r);
+        else
+            hipLaunchKernelGGL((k17_box_into_kernel<S, EVEN, false>), grid, dim3(bx), 0, s, in, (S*)out, a, r);
+    }
+};
 template <class S, bool EVEN> struct IndexedL {
     static void run(const vf_lbm_parameter& p, const uint32_t* idx, uint32_t n, bool macro, bool force, hipStream_t s)
     {
@@ -423,6 +471,24 @@ extern "C" int vf_k17_run(const vf_lbm_parameter* p, void* stream)
         return VF_ERR_INVALID_ARGUMENT;
     }
     return vf_k17_run_on_indices(p, p->fluidNodeIndices, p->numberOfFluidNodes, VF_COLLISION_DEFAULT, stream);
+}
+
+extern "C" int vf_k17_run_box_into(const vf_lbm_parameter* p, void* out, void* stream)
+{
+    int rc = check_common(p);
+    if (rc) return rc;
+    if (!out || out == p->distributions) {
+        set_last_error("out-of-place step needs a separate output buffer");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 ||
+        (uint64_t)p->nx * (uint64_t)p->ny * (uint64_t)p->nz != p->numberOfNodes || p->ny > 65535 || p->nz > 65535) {
+        set_last_error("box path needs nx*ny*nz == numberOfNodes (ny, nz <= 65535)");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    rc = dispatch_storage<BoxIntoL>(*p, out, (hipStream_t)stream);
+    if (rc) return rc;
+    return check_launch("K17CompressibleNavierStokes (box, out of place)");
 }
 
 extern "C" int vf_k17_run_box(const vf_lbm_parameter* p, int32_t collisionTemplate, void* stream)
