@@ -73,7 +73,7 @@ def time_adjoint(L, steps, warmup):
     lp = ctypes.c_void_p(lam.data_ptr())
 
     def step():
-        capi.call("vf_k17_adjoint_box", L.pref(), lp, None, stream_handle())
+        capi.call("vf_k17_adjoint_box", L.pref(), lp, None, None, stream_handle())
         L.swap()
 
     for _ in range(warmup):

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define VF_ABI_VERSION 1
+#define VF_ABI_VERSION 2
 
 enum vf_status {
     VF_OK = 0,
@@ -91,6 +91,11 @@ typedef struct vf_lbm_parameter {
     float* pressure;
     const uint32_t* fluidNodeIndices; /* taggedFluidNodeIndices[Default] (Parameter.h:245-256) */
     uint32_t numberOfFluidNodes;
+    /* Brinkman porosity per node (NULL: none; no counterpart in the reference): a drag
+     * acceleration -porosity * u applied through the collision's half-step velocity shift,
+     * so the collision velocity becomes u (1 - porosity/2). Honoured by every collision and
+     * adjoint entry point (the dense-box FP32 row kernel falls back to the box kernel). */
+    const float* porosity;
 } vf_lbm_parameter;
 
 /*
@@ -189,11 +194,15 @@ int vf_PressureNonEquilibriumCompressible(const vf_lbm_parameter* para, const vf
  * not collide (fluidMask == 0, or not listed) are left untouched, as in the forward step.
  * --------------------------------------------------------------------------------- */
 
-/* Reverse of vf_k17_run_box (dense periodic box, optional fluidMask). */
-int vf_k17_adjoint_box(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint, void* stream);
+/* Reverse of vf_k17_run_box (dense periodic box, optional fluidMask). porosityAdjoint
+ * (optional, numberOfNodes floats): with para->porosity set, the adjoint of each node's
+ * porosity is ADDED. */
+int vf_k17_adjoint_box(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint,
+                       float* porosityAdjoint, void* stream);
 
 /* Reverse of vf_k17_run (every node of fluidNodeIndices, neighbour arrays). */
-int vf_k17_adjoint(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint, void* stream);
+int vf_k17_adjoint(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint, float* porosityAdjoint,
+                   void* stream);
 
 /* Adjoints of the BC kernels, run in reverse order within a step (pressure BC first, then the
  * post-collision BCs in reverse launch order, then the collision adjoint). For the post BCs

@@ -45,7 +45,7 @@ def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, storage, path, even,
     shift = torch.zeros(3 * grid.N, dtype=torch.float32, device="cuda")
     fn = "vf_k17_adjoint_box" if path == "box" else "vf_k17_adjoint"
     print(f"{fn} {storage} even={even} {kind}", flush=True)
-    capi.call(fn, L.pref(), ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(shift.data_ptr()), None)
+    capi.call(fn, L.pref(), ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(shift.data_ptr()), None, None)
     got = lam.cpu().numpy().astype(np.float64)
     got_shift = shift.cpu().numpy().reshape(3, grid.N)
 

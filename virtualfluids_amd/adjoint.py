@@ -10,7 +10,8 @@ copy of the buffer every `checkpoint_every` steps; the backward replays one segm
 from its checkpoint (out of place on the dense box, so the replay writes each intermediate
 state once and copies nothing) and walks the segment backwards. Memory: steps/K + K + 1
 buffers of the storage type. Gradients flow to f0 and, when given, to the wall velocities of
-the velocity BC (3, n) and the outlet densities of the pressure BC (n,).
+the velocity BC (3, n), the outlet densities of the pressure BC (n,) and the Brinkman porosity
+of every node (N,).
 
 Objectives are ordinary torch functions of the returned buffer; ``pre_collision_index`` gives
 the gather that turns a buffer into the (27, N) pre-collision populations of every node, and
@@ -91,9 +92,11 @@ class Stepper:
             capi.call(self.pressure, ctypes.byref(_param(L, target, not even)), ctypes.byref(bc.struct), s)
         return target
 
-    def adjoint(self, before: torch.Tensor, after: torch.Tensor, lam: torch.Tensor, even: bool, ubar, rbar, stream):
+    def adjoint(self, before: torch.Tensor, after: torch.Tensor, lam: torch.Tensor, even: bool, ubar, rbar, padj,
+                stream):
         """Reverse of one step: `before`/`after` are the states around it, `even` its parity;
-        lam (FP32) is updated in place; ubar (3*n) / rbar (n) accumulate BC parameter adjoints."""
+        lam (FP32) is updated in place; ubar (3*n) / rbar (n) / padj (N) accumulate the adjoints
+        of the wall velocities, outlet densities and porosity."""
         L, s = self.L, stream_handle(stream)
         bc = self._bc("pressure")
         if self.pressure and bc is not None:
@@ -105,12 +108,12 @@ class Stepper:
             if fn and bc is not None:
                 capi.call(fn + "Adjoint", ctypes.byref(p), ctypes.byref(bc.struct), _vp(lam), _vp(grad), s)
         fn = "vf_k17_adjoint_box" if L.box else "vf_k17_adjoint"
-        capi.call(fn, ctypes.byref(p), _vp(lam), None, s)
+        capi.call(fn, ctypes.byref(p), _vp(lam), None, _vp(padj), s)
 
 
 class _K17Steps(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, f0, velocity, rho_bc, stepper, steps, checkpoint_every, even0):
+    def forward(ctx, f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0):
         L = stepper.L
         if f0.dtype != L.f.dtype or f0.numel() != L.f.numel() or not f0.is_cuda:
             raise ValueError("f0 must be a device buffer of the lattice's storage type and size")
@@ -121,6 +124,8 @@ class _K17Steps(torch.autograd.Function):
                 bc.vel[i].copy_(velocity[i].detach().to(torch.float32))
         if rho_bc is not None:
             L.bcs["pressure"].rho_bc.copy_(rho_bc.detach().to(torch.float32))
+        if porosity is not None:
+            L.set_porosity(porosity.detach())
         bufs = [f0.detach().clone(), torch.empty_like(f0)]
         cur = 0
         checkpoints = []
@@ -134,6 +139,7 @@ class _K17Steps(torch.autograd.Function):
         ctx.stepper, ctx.steps, ctx.K = stepper, steps, checkpoint_every
         ctx.checkpoints = checkpoints
         ctx.has_velocity, ctx.has_rho = velocity is not None, rho_bc is not None
+        ctx.has_porosity = porosity is not None
         return bufs[cur]
 
     @staticmethod
@@ -145,6 +151,7 @@ class _K17Steps(torch.autograd.Function):
         vel_bc, pr_bc = L.bcs.get("velocity"), L.bcs.get("pressure")
         ubar = torch.zeros(3 * vel_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_velocity else None
         rbar = torch.zeros(pr_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_rho else None
+        padj = torch.zeros(L.N, dtype=torch.float32, device=lam.device) if ctx.has_porosity else None
         for start, even_start, ckpt in reversed(ctx.checkpoints):
             n = min(K, steps - start)
             states = [ckpt]
@@ -153,18 +160,19 @@ class _K17Steps(torch.autograd.Function):
                 states.append(stepper.forward(states[-1], even, torch.empty_like(ckpt), stream))
                 even = not even
             for j in reversed(range(n)):
-                stepper.adjoint(states[j], states[j + 1], lam, even_start ^ (j % 2 == 1), ubar, rbar, stream)
+                stepper.adjoint(states[j], states[j + 1], lam, even_start ^ (j % 2 == 1), ubar, rbar, padj, stream)
             del states
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
-        return lam.to(grad_out.dtype), gv, rbar, None, None, None, None
+        return lam.to(grad_out.dtype), gv, rbar, padj, None, None, None, None
 
 
 def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: int = 10, even: bool | None = None,
               bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
-              rho_bc: torch.Tensor | None = None) -> torch.Tensor:
+              rho_bc: torch.Tensor | None = None, porosity: torch.Tensor | None = None) -> torch.Tensor:
     """`steps` time steps of `lattice` from buffer f0, differentiable in f0 and, when given, in
-    the velocity BC's wall velocities `velocity` (3, n) and the pressure BC's densities `rho_bc`.
+    the velocity BC's wall velocities `velocity` (3, n), the pressure BC's densities `rho_bc`
+    and the Brinkman porosity per node `porosity` (N).
 
     The lattice's own buffer and parity are not touched (its BC parameter arrays take the given
     values); `even` is the parity of the first step (default: the lattice's current parity).
@@ -177,7 +185,7 @@ def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: 
     if rho_bc is not None and lattice.bcs.get("pressure") is None:
         raise ValueError("the lattice has no pressure BC nodes")
     even0 = lattice.even if even is None else bool(even)
-    return _K17Steps.apply(f0, velocity, rho_bc, stepper, steps, checkpoint_every, even0)
+    return _K17Steps.apply(f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0)
 
 
 def pre_collision_index(grid, even: bool) -> np.ndarray:

@@ -63,6 +63,7 @@ class LBMParameter(ctypes.Structure):
         ("pressure", _vp),
         ("fluidNodeIndices", _vp),
         ("numberOfFluidNodes", ctypes.c_uint32),
+        ("porosity", _vp),
     ]
 
 
@@ -111,8 +112,8 @@ SIGNATURES = {
     "vf_ConservationSums": [_P, _vp, _vp],
     "vf_MomentSums": [_P, _vp, _vp],
     "vf_ScaleCoarseToFineCompressible": [_P, _P, _vp, _vp, _vp],
-    "vf_k17_adjoint_box": [_P, _vp, _vp, _vp],
-    "vf_k17_adjoint": [_P, _vp, _vp, _vp],
+    "vf_k17_adjoint_box": [_P, _vp, _vp, _vp, _vp],
+    "vf_k17_adjoint": [_P, _vp, _vp, _vp, _vp],
     "vf_NoSlipBounceBackAdjoint": [_P, _Q, _vp, _vp, _vp],
     "vf_NoSlipInterpolatedCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
     "vf_VelocityBounceBackAdjoint": [_P, _Q, _vp, _vp, _vp],

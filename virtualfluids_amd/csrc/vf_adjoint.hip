@@ -45,8 +45,16 @@ struct AdjBoxArgs {
     uint64_t N;
     int nx, ny, nz;
     const uint8_t* mask;
-    float* shiftAdj; // 3*N or null
+    float* shiftAdj;       // 3*N or null
+    const float* porosity; // Brinkman porosity or null
+    float* porosityAdj;    // N or null
 };
+
+// adjoint of the velocity scale s = 1 - porosity/2 goes to the porosity with the factor -1/2
+__device__ __forceinline__ void brinkman_adjoint(const float* porosity, float* porosityAdj, uint32_t k, float sb)
+{
+    if (porosity && porosityAdj) porosityAdj[k] += -0.5f * sb;
+}
 
 template <class S, bool EVEN, bool MASK, bool SHIFT>
 __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __restrict__ f, float* __restrict__ lam,
@@ -65,15 +73,17 @@ __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __r
     const uint32_t row11 = (uint32_t)a.nx * ((uint32_t)yp + (uint32_t)a.ny * (uint32_t)zp);
     const uint32_t n[8] = {row00 + x, row00 + xp, row10 + x, row10 + xp, row01 + x, row01 + xp, row11 + x, row11 + xp};
     if (MASK && a.mask[n[0]] == 0) return;
-    float v[27], g[27], fb[27], ub[3];
+    float v[27], g[27], fb[27], ub[3], sb = 0.0f;
     adj_gather<S, EVEN>(f, lam, a.N, n, v, g);
-    k17_vjp(v, g, r, fb, SHIFT ? ub : nullptr);
+    const float vs = a.porosity ? 1.0f - 0.5f * a.porosity[n[0]] : 1.0f;
+    k17_vjp(v, g, r, fb, SHIFT ? ub : nullptr, vs, a.porosity ? &sb : nullptr);
     adj_scatter<EVEN>(lam, a.N, n, fb);
     if (SHIFT) {
         a.shiftAdj[n[0]] += ub[0];
         a.shiftAdj[a.N + n[0]] += ub[1];
         a.shiftAdj[2 * a.N + n[0]] += ub[2];
     }
+    brinkman_adjoint(a.porosity, a.porosityAdj, n[0], sb);
 }
 
 struct AdjIndexedArgs {
@@ -84,6 +94,8 @@ struct AdjIndexedArgs {
     const uint32_t* indices;
     uint32_t count;
     float* shiftAdj;
+    const float* porosity;
+    float* porosityAdj;
 };
 
 template <class S, bool EVEN, bool SHIFT>
@@ -103,15 +115,17 @@ __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_indexed_kernel(const S*
     n[5] = a.nbz[n[1]];
     n[6] = a.nbz[n[2]];
     n[7] = a.nbz[n[3]];
-    float v[27], g[27], fb[27], ub[3];
+    float v[27], g[27], fb[27], ub[3], sb = 0.0f;
     adj_gather<S, EVEN>(f, lam, a.N, n, v, g);
-    k17_vjp(v, g, r, fb, SHIFT ? ub : nullptr);
+    const float vs = a.porosity ? 1.0f - 0.5f * a.porosity[k] : 1.0f;
+    k17_vjp(v, g, r, fb, SHIFT ? ub : nullptr, vs, a.porosity ? &sb : nullptr);
     adj_scatter<EVEN>(lam, a.N, n, fb);
     if (SHIFT) {
         a.shiftAdj[k] += ub[0];
         a.shiftAdj[a.N + k] += ub[1];
         a.shiftAdj[2 * a.N + k] += ub[2];
     }
+    brinkman_adjoint(a.porosity, a.porosityAdj, k, sb);
 }
 
 int check_adjoint_args(const vf_lbm_parameter* p, const float* adjoint)
@@ -136,9 +150,9 @@ int check_adjoint_args(const vf_lbm_parameter* p, const float* adjoint)
 }
 
 template <class S, bool EVEN>
-void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, hipStream_t s)
+void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj, hipStream_t s)
 {
-    AdjBoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, shift};
+    AdjBoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, shift, p.porosity, padj};
     const K17Rates r = make_k17_rates(p);
     const int bx = p.nx >= kAdjBlock ? kAdjBlock : ((p.nx + 63) / 64) * 64;
     const dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
@@ -151,10 +165,10 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, hipStream_t
 }
 
 template <class S, bool EVEN>
-void launch_indexed(const vf_lbm_parameter& p, float* lam, float* shift, hipStream_t s)
+void launch_indexed(const vf_lbm_parameter& p, float* lam, float* shift, float* padj, hipStream_t s)
 {
     AdjIndexedArgs a{p.numberOfNodes, p.neighborX, p.neighborY, p.neighborZ, p.fluidNodeIndices,
-                     p.numberOfFluidNodes, shift};
+                     p.numberOfFluidNodes, shift, p.porosity, padj};
     const K17Rates r = make_k17_rates(p);
     const dim3 grid((p.numberOfFluidNodes + kAdjBlock - 1) / kAdjBlock);
     const S* f = static_cast<const S*>(p.distributions);
@@ -178,7 +192,8 @@ void dispatch(const vf_lbm_parameter& p, F&& launch)
 
 using namespace vfa;
 
-extern "C" int vf_k17_adjoint_box(const vf_lbm_parameter* p, float* adjoint, float* velocityShiftAdjoint, void* stream)
+extern "C" int vf_k17_adjoint_box(const vf_lbm_parameter* p, float* adjoint, float* velocityShiftAdjoint,
+                                  float* porosityAdjoint, void* stream)
 {
     const int st = check_adjoint_args(p, adjoint);
     if (st != VF_OK) return st;
@@ -190,12 +205,13 @@ extern "C" int vf_k17_adjoint_box(const vf_lbm_parameter* p, float* adjoint, flo
     hipStream_t s = (hipStream_t)stream;
     dispatch(*p, [&](auto tag, auto even) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        launch_box<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, s);
+        launch_box<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, porosityAdjoint, s);
     });
     return check_launch("k17_adjoint_box_kernel execution failed");
 }
 
-extern "C" int vf_k17_adjoint(const vf_lbm_parameter* p, float* adjoint, float* velocityShiftAdjoint, void* stream)
+extern "C" int vf_k17_adjoint(const vf_lbm_parameter* p, float* adjoint, float* velocityShiftAdjoint,
+                              float* porosityAdjoint, void* stream)
 {
     const int st = check_adjoint_args(p, adjoint);
     if (st != VF_OK) return st;
@@ -207,7 +223,7 @@ extern "C" int vf_k17_adjoint(const vf_lbm_parameter* p, float* adjoint, float* 
     hipStream_t s = (hipStream_t)stream;
     dispatch(*p, [&](auto tag, auto even) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        launch_indexed<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, s);
+        launch_indexed<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, porosityAdjoint, s);
     });
     return check_launch("k17_adjoint_indexed_kernel execution failed");
 }

@@ -29,6 +29,7 @@ struct BcAdjArgs {
     float omega;
     float* ubar; // 3*nbc (Vx block, Vy block, Vz block) or null
     K17Rates rates;
+    const float* porosity;
 };
 
 __device__ __forceinline__ float* lam_cell(float* lam, const Addr& a, const uint32_t (&n)[8], int d, bool even)
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(256) void bc_post_adjoint_kernel(const S* __restric
         q[d] = d == 0 ? -1.0f : b.q[(uint64_t)d * b.nbc + i];
     }
     MacroOut mo;
-    k17_collide(post, b.rates, mo);
+    k17_collide(post, b.rates, mo, b.porosity ? 1.0f - 0.5f * b.porosity[b.k[i]] : 1.0f);
     // adjoint of the written cells (incoming direction inv(d), next parity), then cleared
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
@@ -174,7 +175,7 @@ int launch_post_adjoint(const vf_lbm_parameter* p, const vf_q_boundary_condition
         return VF_ERR_INVALID_ARGUMENT;
     }
     const BcAdjArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega, velocity ? ubar : nullptr,
-                      make_k17_rates(*p)};
+                      make_k17_rates(*p), p->porosity};
     rc = by_storage<PostAdjL<KIND>::template L>(p->storage, *p, b, lam, (hipStream_t)stream);
     return rc ? rc : check_launch(name);
 }

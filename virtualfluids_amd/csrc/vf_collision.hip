@@ -52,7 +52,14 @@ struct BoxArgs {
     float* vx;
     float* vy;
     float* vz;
+    const float* porosity; // Brinkman porosity per node, or null
 };
+
+// Brinkman drag a = -porosity u through the half-step velocity shift: u (1 - porosity/2)
+__device__ __forceinline__ float brinkman_scale(const float* porosity, uint32_t k)
+{
+    return porosity ? 1.0f - 0.5f * porosity[k] : 1.0f;
+}
 
 constexpr int kBoxBlock = 256;
 
@@ -76,7 +83,7 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, B
     float v[27];
     gather<S, EVEN>(f, a.N, n, v);
     MacroOut mo;
-    k17_collide(v, r, mo);
+    k17_collide(v, r, mo, brinkman_scale(a.porosity, n[0]));
     if (MACRO) {
         a.rho[n[0]] = mo.drho;
         a.vx[n[0]] = mo.vx;
@@ -116,7 +123,7 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_into_kernel(const S* __rest
         return;
     }
     MacroOut mo;
-    k17_collide(v, r, mo);
+    k17_collide(v, r, mo, brinkman_scale(a.porosity, n[0]));
     scatter<S, EVEN>(out, a.N, n, v);
 }
 
@@ -260,6 +267,7 @@ struct IndexedArgs {
     float* bfy;
     float* bfz;
     float forceScale; // 1/2 * forceFactor, applied to the per-node body force
+    const float* porosity;
 };
 
 constexpr int kIndexedBlock = 256;
@@ -292,7 +300,7 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
     float v[27];
     gather<S, EVEN>(f, a.N, n, v);
     MacroOut mo;
-    k17_collide(v, rr, mo);
+    k17_collide(v, rr, mo, brinkman_scale(a.porosity, k));
     if (MACRO) {
         a.rho[k] = mo.drho;
         a.vx[k] = mo.vx;
@@ -329,13 +337,15 @@ static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
 template <class S, bool EVEN>
 static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
 {
-    BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, p.rho, p.velocityX, p.velocityY, p.velocityZ};
+    BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, p.rho, p.velocityX, p.velocityY, p.velocityZ,
+              p.porosity};
     const K17Rates r = make_k17_rates(p);
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
     // row kernel: a row of nx nodes is nx/V lanes, whole waves, at most one workgroup
+    // (porous media take the box kernel)
     const int V = row_width<S>();
-    if (V > 0 && p.nx % (64 * V) == 0 && p.nx / V <= kRowMaxThreads) {
+    if (!p.porosity && V > 0 && p.nx % (64 * V) == 0 && p.nx / V <= kRowMaxThreads) {
         if (V == 1 && sizeof(S) == 4) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, s); return; }
         if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, s); return; }
         if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, s); return; }
@@ -354,7 +364,7 @@ static void launch_indexed_t(const vf_lbm_parameter& p, const uint32_t* idx, uin
 {
     IndexedArgs a{p.numberOfNodes, p.neighborX, p.neighborY, p.neighborZ, idx, n,
                   p.rho, p.velocityX, p.velocityY, p.velocityZ, p.bodyForceX, p.bodyForceY, p.bodyForceZ,
-                  0.5f * p.forceFactor};
+                  0.5f * p.forceFactor, p.porosity};
     const K17Rates r = make_k17_rates(p);
     dim3 grid((n + kIndexedBlock - 1) / kIndexedBlock);
     S* f = static_cast<S*>(p.distributions);
@@ -390,7 +400,7 @@ template <class S, bool EVEN> struct BoxL {
 template <class S, bool EVEN> struct BoxIntoL {
     static void run(const vf_lbm_parameter& p, void* out, hipStream_t s)
     {
-        BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, nullptr, nullptr, nullptr, nullptr};
+        BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, nullptr, nullptr, nullptr, nullptr, p.porosity};
         const K17Rates r = make_k17_rates(p);
         const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
         const dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);

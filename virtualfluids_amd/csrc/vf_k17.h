@@ -135,7 +135,8 @@ struct MacroOut {
 };
 
 // f: 27 shifted pre-collision populations in D3Q27 order; replaced by post-collision values.
-VF_HD void k17_collide(float (&f)[27], const K17Rates& r, MacroOut& mo)
+// vscale: the Brinkman factor 1 - porosity/2 that scales the collision velocity (1: none)
+VF_HD void k17_collide(float (&f)[27], const K17Rates& r, MacroOut& mo, float vscale = 1.0f)
 {
     float& fMMM = f[dMMM]; float& f0MM = f[d0MM]; float& fPMM = f[dPMM];
     float& fM0M = f[dM0M]; float& f00M = f[d00M]; float& fP0M = f[dP0M];
@@ -174,9 +175,10 @@ VF_HD void k17_collide(float (&f)[27], const K17Rates& r, MacroOut& mo)
     const float jz = (((dMM + dMP) + (dPM + dPP)) + ((dM0 + dP0) + (d0M + d0P))) + d00;
 
     const float oneOverRho = fast_rcp(drho + 1.0f);
-    const float vx = __builtin_fmaf(jx, oneOverRho, r.fx);
-    const float vy = __builtin_fmaf(jy, oneOverRho, r.fy);
-    const float vz = __builtin_fmaf(jz, oneOverRho, r.fz);
+    const float rs = oneOverRho * vscale;
+    const float vx = __builtin_fmaf(jx, rs, r.fx);
+    const float vy = __builtin_fmaf(jy, rs, r.fy);
+    const float vz = __builtin_fmaf(jz, rs, r.fz);
     const float vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
 
     // ---- z pass, velocity half: b = d - (s+K) u, c = e - 2 u d + (s+K) u^2 ------------------

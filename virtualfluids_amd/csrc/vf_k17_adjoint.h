@@ -102,7 +102,7 @@ struct Macro {
     float drho, R, jx, jy, jz, vx, vy, vz;
 };
 
-VF_HD void macro_of(const float (&p)[27], const K17Rates& r, Macro& m)
+VF_HD void macro_of(const float (&p)[27], const K17Rates& r, float vscale, Macro& m)
 {
     float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
 #pragma unroll
@@ -121,9 +121,10 @@ VF_HD void macro_of(const float (&p)[27], const K17Rates& r, Macro& m)
     m.jx = jx;
     m.jy = jy;
     m.jz = jz;
-    m.vx = jx * m.R + r.fx;
-    m.vy = jy * m.R + r.fy;
-    m.vz = jz * m.R + r.fz;
+    const float rs = m.R * vscale;
+    m.vx = jx * rs + r.fx;
+    m.vy = jy * rs + r.fy;
+    m.vz = jz * rs + r.fz;
 }
 
 // z pass (all 27 outputs are used downstream)
@@ -488,15 +489,16 @@ VF_HD void mid_reverse(const Mid& s, const Macro& mc, const K17Rates& r, const f
 // Reverse-mode of k17_collide for one node. f: shifted pre-collision populations (direction
 // order); g: adjoint of the post-collision populations (direction order). fb receives the
 // adjoint of f (direction order); ub (optional) receives the adjoint of the velocity shift
-// (r.fx, r.fy, r.fz).
-VF_HD void k17_vjp(const float (&f)[27], const float (&g)[27], const K17Rates& r, float (&fb)[27], float* ub = nullptr)
+// (r.fx, r.fy, r.fz); sb (optional) the adjoint of the velocity scale vscale.
+VF_HD void k17_vjp(const float (&f)[27], const float (&g)[27], const K17Rates& r, float (&fb)[27], float* ub = nullptr,
+                   float vscale = 1.0f, float* sb = nullptr)
 {
     using namespace adj;
     float p[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) p[i] = f[dir_of_idx(i)];
     Macro mc;
-    macro_of(p, r, mc);
+    macro_of(p, r, vscale, mc);
 
     // forward to the moments m, and the cumulant stage
     float m[27];
@@ -597,8 +599,11 @@ VF_HD void k17_vjp(const float (&f)[27], const float (&g)[27], const K17Rates& r
         ub[1] = vyb;
         ub[2] = vzb;
     }
-    const float jxb = vxb * mc.R, jyb = vyb * mc.R, jzb = vzb * mc.R;
-    Rb += vxb * mc.jx + vyb * mc.jy + vzb * mc.jz;
+    const float rs = mc.R * vscale;
+    const float jxb = vxb * rs, jyb = vyb * rs, jzb = vzb * rs;
+    const float vj = vxb * mc.jx + vyb * mc.jy + vzb * mc.jz;
+    Rb += vscale * vj;
+    if (sb) *sb = mc.R * vj;
     drhob += -Rb * mc.R * mc.R;
 #pragma unroll
     for (int i = 0; i < 3; ++i)

@@ -115,6 +115,21 @@ class Lattice:
         p.numberOfFluidNodes = int(grid.fluid.size)
         self.param = p
 
+    def set_porosity(self, porosity):
+        """Brinkman porosity per node (numpy or torch, N values; None removes it): a drag
+        acceleration -porosity * u applied through the collision's velocity shift."""
+        if porosity is None:
+            self.porosity = None
+            self.param.porosity = None
+            return
+        t = torch.as_tensor(porosity, dtype=torch.float32).reshape(-1)
+        if t.numel() != self.N:
+            raise ValueError("porosity needs one value per node")
+        if getattr(self, "porosity", None) is None:
+            self.porosity = torch.zeros(self.N, dtype=torch.float32, device=self.device)
+        self.porosity.copy_(t)
+        self.param.porosity = _ptr(self.porosity)
+
     # ---- parity -------------------------------------------------------------------------
     @property
     def even(self) -> bool:
