@@ -89,8 +89,8 @@ template <class T> inline void write_pre(T* const P[27], const u32 n[8], int d, 
 
 // ---- collision over an index list (runCollision) -------------------------------------------
 template <class T>
-void collide_indexed(const Lattice<T>& L, const u32* fluid, u64 nfluid, const K17Param<T>& prm, bool even, T* rho,
-                     T* vx, T* vy, T* vz)
+void collide_indexed(const Lattice<T>& L, const u32* fluid, u64 nfluid, const K17Param<T>& prm0, bool even, T* rho,
+                     T* vx, T* vy, T* vz, const T* porosity = nullptr)
 {
     T* P[27];
     slot_table(L, even, P);
@@ -102,6 +102,8 @@ void collide_indexed(const Lattice<T>& L, const u32* fluid, u64 nfluid, const K1
         T f[27];
         read_pre(P, n, f);
         Macro<T> m;
+        K17Param<T> prm = prm0;
+        if (porosity) prm.vscale = T(1) - T(0.5) * porosity[k];
         k17_collide(f, prm, &m);
         if (rho) {
             rho[k] = m.drho;
@@ -571,6 +573,15 @@ template <class T> K17Param<T> make_param(T omega, const T* lim3, const T* force
         Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
         collide_indexed(L, fluid, nfluid, make_param<T>(omega, lim3, force3), even != 0, rho, vx, vy, vz);    \
     }                                                                                                          \
+    extern "C" void orc_collide_porous_##SFX(T* buf, unsigned long long N, const uint32_t* nbx,               \
+                                             const uint32_t* nby, const uint32_t* nbz, const uint32_t* fluid,  \
+                                             unsigned long long nfluid, T omega, const T* lim3,              \
+                                             const T* force3, const T* porosity, int even)                   \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        collide_indexed(L, fluid, nfluid, make_param<T>(omega, lim3, force3), even != 0, (T*)nullptr,         \
+                        (T*)nullptr, (T*)nullptr, (T*)nullptr, porosity);                                      \
+    }                                                                                                          \
     extern "C" void orc_collide_box_##SFX(T* buf, int nx, int ny, int nz, const uint8_t* mask, T omega,        \
                                           const T* lim3, const T* force3, int even)                            \
     {                                                                                                          \
@@ -692,7 +703,8 @@ extern "C" void orc_k17_vjp_f64(const double* fin, const double* g, long long n,
 extern "C" void orc_adjoint_step_f64(const double* state, double* lam, unsigned long long N, const uint32_t* nbx,
                                      const uint32_t* nby, const uint32_t* nbz, const uint32_t* fluid,
                                      unsigned long long nfluid, double omega, const double* lim3,
-                                     const double* force3, int even, double* ub3n)
+                                     const double* force3, int even, double* ub3n, const double* porosity,
+                                     double* porosityAdj)
 {
     const K17Param<double> p = make_param<double>(omega, lim3, force3);
     Lattice<double> S{const_cast<double*>(state), N, nbx, nby, nbz};
@@ -706,10 +718,13 @@ extern "C" void orc_adjoint_step_f64(const double* state, double* lam, unsigned 
         const u32 k = fluid[i];
         u32 nn[8];
         cell_nodes(S, k, nn);
-        double f[27], g[27], fb[27];
+        double f[27], g[27], fb[27], sb = 0.0;
         read_pre(PS, nn, f);
         read_post(PA, nn, g);
-        k17_vjp_dual(f, g, p, fb, ub3n ? ub3n + 3 * (u64)k : nullptr);
+        K17Param<double> pk = p;
+        if (porosity) pk.vscale = 1.0 - 0.5 * porosity[k];
+        k17_vjp_dual(f, g, pk, fb, ub3n ? ub3n + 3 * (u64)k : nullptr, porosity ? &sb : nullptr);
+        if (porosity && porosityAdj) porosityAdj[k] += -0.5 * sb;
         for (int d = 0; d < 27; ++d) write_pre(PA, nn, d, fb[d]);
     }
 }
@@ -850,5 +865,17 @@ extern "C" void orc_pressure_node_vjp_f64(const double* fn, double rhoBC, int di
             if (act[d]) s += out[d].d * G[d];
         if (col < 27) fnb[col] = s;
         else *rb = s;
+    }
+}
+
+extern "C" void orc_k17_vjp_scaled_f64(const double* fin, const double* g, long long n, double omega,
+                                       const double* lim3, const double* vscale, double* fb, double* sb)
+{
+    const K17Param<double> p0 = make_param<double>(omega, lim3, nullptr);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) {
+        K17Param<double> p = p0;
+        p.vscale = vscale[i];
+        k17_vjp_dual(fin + 27 * i, g + 27 * i, p, fb + 27 * i, nullptr, sb + i);
     }
 }

@@ -105,10 +105,16 @@ class Lattice:
     def _nb(self):
         return (_p(self.f), ctypes.c_ulonglong(self.N), _p(self.nbx), _p(self.nby), _p(self.nbz))
 
-    def collide(self, fluid, omega, limiters=(0.01, 0.01, 0.01), force=(0.0, 0.0, 0.0), macro=None):
+    def collide(self, fluid, omega, limiters=(0.01, 0.01, 0.01), force=(0.0, 0.0, 0.0), macro=None, porosity=None):
         fluid = np.ascontiguousarray(fluid, dtype=np.uint32)
         lim = np.asarray(limiters, dtype=self.dtype)
         frc = np.asarray(force, dtype=self.dtype)
+        if porosity is not None:
+            assert macro is None
+            por = np.ascontiguousarray(porosity, dtype=self.dtype)
+            self._fn("orc_collide_porous")(*self._nb(), _p(fluid), ctypes.c_ulonglong(fluid.size), self._T(omega),
+                                           _p(lim), _p(frc), _p(por), ctypes.c_int(int(self.even)))
+            return
         r = vx = vy = vz = None
         if macro is not None:
             r, vx, vy, vz = macro
@@ -262,10 +268,12 @@ def k17_vjp(fin, g, omega, limiters=(0.01, 0.01, 0.01), force=None):
     return fb, ub
 
 
-def adjoint_step(lattice, lam, fluid, omega, limiters=(0.01, 0.01, 0.01), force=None, shift_adjoint=None):
+def adjoint_step(lattice, lam, fluid, omega, limiters=(0.01, 0.01, 0.01), force=None, shift_adjoint=None,
+                 porosity=None, porosity_adjoint=None):
     """Reverse of one Lattice.collide(fluid, ...) step: lattice.f holds the state before the
     step and lattice.even its parity; lam (27*N, FP64) is updated in place. shift_adjoint
-    (optional (3, N) array) accumulates the adjoint of each node's velocity shift."""
+    (optional (3, N) array) accumulates the adjoint of each node's velocity shift;
+    porosity_adjoint (N, with porosity given) that of the Brinkman porosity."""
     assert lattice.dtype == np.float64 and lam.dtype == np.float64 and lam.flags.c_contiguous
     fluid = np.ascontiguousarray(fluid, dtype=np.uint32)
     lim = np.asarray(limiters, dtype=np.float64)
@@ -273,9 +281,13 @@ def adjoint_step(lattice, lam, fluid, omega, limiters=(0.01, 0.01, 0.01), force=
     ub = None
     if shift_adjoint is not None:
         ub = np.zeros((lattice.N, 3))
+    por = None if porosity is None else np.ascontiguousarray(porosity, dtype=np.float64)
+    if porosity_adjoint is not None:
+        assert porosity_adjoint.dtype == np.float64 and porosity_adjoint.flags.c_contiguous
     lib().orc_adjoint_step_f64(_p(lattice.f), _p(lam), ctypes.c_ulonglong(lattice.N), _p(lattice.nbx), _p(lattice.nby),
                                _p(lattice.nbz), _p(fluid), ctypes.c_ulonglong(fluid.size), ctypes.c_double(omega),
-                               _p(lim), _p(frc), ctypes.c_int(int(lattice.even)), _p(ub))
+                               _p(lim), _p(frc), ctypes.c_int(int(lattice.even)), _p(ub), _p(por),
+                               _p(porosity_adjoint))
     if shift_adjoint is not None:
         shift_adjoint += ub.T
 

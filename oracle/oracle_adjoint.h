@@ -35,21 +35,27 @@ inline bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
 inline Dual abs(const Dual& a) { return a.v < 0.0 ? -a : a; }
 
 // (d post / d f)^T g for one node; ub (optional): adjoint of the half-step velocity shift.
-inline void k17_vjp_dual(const double* fin, const double* g, const K17Param<double>& p, double* fb, double* ub)
+// sb (optional): adjoint of the velocity scale p.vscale.
+inline void k17_vjp_dual(const double* fin, const double* g, const K17Param<double>& p, double* fb, double* ub,
+                         double* sb = nullptr)
 {
-    K17Param<Dual> q{p.omega, p.limP, p.limM, p.limD, p.fx, p.fy, p.fz};
-    for (int col = 0; col < 27 + (ub ? 3 : 0); ++col) {
+    K17Param<Dual> q{p.omega, p.limP, p.limM, p.limD, p.fx, p.fy, p.fz, p.vscale};
+    for (int col = 0; col < 31; ++col) {
+        if (col >= 27 && col < 30 && !ub) continue;
+        if (col == 30 && !sb) continue;
         Dual f[27];
         for (int e = 0; e < 27; ++e) f[e] = Dual(fin[e], col == e ? 1.0 : 0.0);
         K17Param<Dual> qq = q;
         if (col == 27) qq.fx = Dual(p.fx, 1.0);
         if (col == 28) qq.fy = Dual(p.fy, 1.0);
         if (col == 29) qq.fz = Dual(p.fz, 1.0);
+        if (col == 30) qq.vscale = Dual(p.vscale, 1.0);
         k17_collide(f, qq, static_cast<Macro<Dual>*>(nullptr));
         double s = 0.0;
         for (int e = 0; e < 27; ++e) s += f[e].d * g[e];
         if (col < 27) fb[col] = s;
-        else ub[col - 27] = s;
+        else if (col < 30) ub[col - 27] = s;
+        else *sb = s;
     }
 }
 

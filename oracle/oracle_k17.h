@@ -24,6 +24,7 @@ template <class T> struct K17Param {
     T omega;
     T limP, limM, limD; // quadric limiters P, M, D (src/lbm/collision/K17CompressibleNavierStokes.h:177-179)
     T fx, fy, fz;       // half-step velocity shift (body force), already scaled by the caller
+    T vscale = T(1);    // Brinkman factor 1 - porosity/2 on the velocity (no reference counterpart; 1: none)
 };
 
 template <class T> struct Macro {
@@ -181,9 +182,9 @@ template <class T> void k17_collide(T* f, const K17Param<T>& p, Macro<T>* out)
 {
     T drho, oneOverRho, vvx, vvy, vvz;
     compressible_macro(f, drho, oneOverRho, vvx, vvy, vvz);
-    vvx += p.fx;
-    vvy += p.fy;
-    vvz += p.fz;
+    vvx = vvx * p.vscale + p.fx; // vscale = 1 leaves the value unchanged bit for bit
+    vvy = vvy * p.vscale + p.fy;
+    vvz = vvz * p.vscale + p.fz;
     const T vx2 = vvx * vvx, vy2 = vvy * vvy, vz2 = vvz * vvz;
 
     pass_fwd(f, kPassZ, vvz, vz2);

@@ -75,3 +75,17 @@ extern "C" void host_pressure_vjp(const float* fn, float rhoBC, const float* G, 
     pressure_vjp(f, rhoBC, g, out, *rb);
     std::memcpy(fnb, out, sizeof(out));
 }
+
+// the same with a Brinkman velocity scale per node; sb (n) receives its adjoint
+extern "C" void host_k17_vjp_scaled(const float* fin, const float* g, long long n, float omega, const float* lim3,
+                                    const float* vscale, float* fb, float* sb)
+{
+    const K17Rates r = rates(omega, lim3, nullptr);
+    for (long long i = 0; i < n; ++i) {
+        float f[27], gg[27], out[27];
+        std::memcpy(f, fin + 27 * i, sizeof(f));
+        std::memcpy(gg, g + 27 * i, sizeof(gg));
+        k17_vjp(f, gg, r, out, nullptr, vscale[i], sb + i);
+        std::memcpy(fb + 27 * i, out, sizeof(out));
+    }
+}

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -375,9 +376,46 @@ static int run_case(const std::string& dir, unsigned steps)
     return 0;
 }
 
+// the C structs' field offsets, for the ctypes binding's layout check
+static int print_layout()
+{
+#define VF_FIELD(S, F) std::printf("%s.%s %zu\n", #S, #F, offsetof(S, F))
+    VF_FIELD(vf_lbm_parameter, distributions);
+    VF_FIELD(vf_lbm_parameter, storage);
+    VF_FIELD(vf_lbm_parameter, isEvenTimestep);
+    VF_FIELD(vf_lbm_parameter, numberOfNodes);
+    VF_FIELD(vf_lbm_parameter, neighborX);
+    VF_FIELD(vf_lbm_parameter, typeOfGridNode);
+    VF_FIELD(vf_lbm_parameter, nx);
+    VF_FIELD(vf_lbm_parameter, fluidMask);
+    VF_FIELD(vf_lbm_parameter, omega);
+    VF_FIELD(vf_lbm_parameter, quadricLimiters);
+    VF_FIELD(vf_lbm_parameter, forces);
+    VF_FIELD(vf_lbm_parameter, forceFactor);
+    VF_FIELD(vf_lbm_parameter, bodyForceX);
+    VF_FIELD(vf_lbm_parameter, rho);
+    VF_FIELD(vf_lbm_parameter, pressure);
+    VF_FIELD(vf_lbm_parameter, fluidNodeIndices);
+    VF_FIELD(vf_lbm_parameter, numberOfFluidNodes);
+    VF_FIELD(vf_lbm_parameter, porosity);
+    VF_FIELD(vf_q_boundary_conditions, k);
+    VF_FIELD(vf_q_boundary_conditions, q27);
+    VF_FIELD(vf_q_boundary_conditions, numberOfBCnodes);
+    VF_FIELD(vf_q_boundary_conditions, Vx);
+    VF_FIELD(vf_q_boundary_conditions, RhoBC);
+    VF_FIELD(vf_q_boundary_conditions, direction);
+    VF_FIELD(vf_interpolation_cells, numberOfCells);
+    VF_FIELD(vf_interpolation_cell_neighbor, z);
+#undef VF_FIELD
+    std::printf("sizeof.vf_lbm_parameter %zu\n", sizeof(vf_lbm_parameter));
+    std::printf("sizeof.vf_q_boundary_conditions %zu\n", sizeof(vf_q_boundary_conditions));
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "factories";
+    if (mode == "layout") return print_layout();
     if (mode == "factories") {
         bc_factory_tests();
         bc_manager_tests();

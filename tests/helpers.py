@@ -81,8 +81,8 @@ ORACLE_POST_BC = {
 
 
 def oracle_run(orc, grid, omega, steps, velocity_bc=None, noslip_bc=None, pressure=False, init=None,
-               limiters=(0.01, 0.01, 0.01), dtype=np.float64):
-    """collision -> velocity BC -> no-slip BC -> swap -> pressure BC, `steps` times."""
+               limiters=(0.01, 0.01, 0.01), dtype=np.float64, geometry_bc=None, porosity=None):
+    """collision -> velocity BC -> no-slip BC -> geometry BC -> swap -> pressure BC, `steps` times."""
     L = orc.Lattice(grid.nbx, grid.nby, grid.nbz, dtype=dtype)
     N = grid.N
     if init is None:
@@ -92,12 +92,15 @@ def oracle_run(orc, grid, omega, steps, velocity_bc=None, noslip_bc=None, pressu
     vel = grid.bcs.get("velocity")
     ns = grid.bcs.get("noslip")
     pr = grid.bcs.get("pressure")
+    geo = grid.bcs.get("geometry")
     for _ in range(steps):
-        L.collide(grid.fluid, omega, limiters)
+        L.collide(grid.fluid, omega, limiters, porosity=porosity)
         if velocity_bc and vel is not None:
             L.bc_post(ORACLE_POST_BC[velocity_bc], vel.k, vel.q27, omega, vel.velocity)
         if noslip_bc and ns is not None:
             L.bc_post(ORACLE_POST_BC[noslip_bc], ns.k, ns.q27, omega)
+        if geometry_bc and geo is not None:
+            L.bc_post(ORACLE_POST_BC[geometry_bc], geo.k, geo.q27, omega)
         L.swap()
         if pressure and pr is not None:
             L.bc_pressure(pr.k, pr.kN, pr.rho_bc, pr.direction)

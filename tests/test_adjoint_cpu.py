@@ -242,3 +242,59 @@ def test_host_build_of_gpu_pressure_adjoint_matches_reference(orc, host, directi
                                _p(rb))
         assert np.abs(fnb - fnb_ref).max() < 1e-5 * np.abs(fnb_ref).max()
         assert abs(rb[0] - rb_ref[0]) < 1e-5 * abs(rb_ref[0])
+
+
+def test_host_build_of_brinkman_adjoint_matches_reference(orc, host):
+    """The velocity-scale (Brinkman) term of the GPU reverse mode against the dual numbers."""
+    rng = np.random.default_rng(80)
+    n = 64
+    fin = near_equilibrium(rng, n, u_amp=0.05, noise=5e-3)
+    g = rng.normal(0, 1, (n, 27))
+    vs = 1.0 - 0.5 * rng.uniform(0.0, 0.1, n)
+    lim = np.full(3, 0.01)
+    fb_ref, sb_ref = np.empty_like(fin), np.empty(n)
+    orc.lib().orc_k17_vjp_scaled_f64(_p(fin), _p(g), ctypes.c_longlong(n), ctypes.c_double(1.8), _p(lim), _p(vs),
+                                     _p(fb_ref), _p(sb_ref))
+    f32, g32 = fin.astype(np.float32), g.astype(np.float32)
+    fb, sb = np.empty_like(f32), np.empty(n, dtype=np.float32)
+    host.host_k17_vjp_scaled(_p(f32), _p(g32), ctypes.c_longlong(n), ctypes.c_float(1.8), _p(lim.astype(np.float32)),
+                             _p(vs.astype(np.float32)), _p(fb), _p(sb))
+    assert np.abs(fb - fb_ref).max() < 2e-6 * np.abs(fb_ref).max()
+    assert np.abs(sb - sb_ref).max() < 2e-5 * np.abs(sb_ref).max()
+
+
+def test_porosity_gradient_dot_product(orc):
+    """Three steps of a periodic box with a Brinkman porosity field: the porosity adjoint
+    accumulated by the oracle's adjoint steps equals FP64 central differences of <w, f(phi)>."""
+    from virtualfluids_amd.grid import periodic_box
+
+    g = periodic_box(6, 5, 4)
+    rng = np.random.default_rng(81)
+    f0 = pack_pre(near_equilibrium(rng, g.N, u_amp=0.05, noise=3e-3), g, True)
+    phi = rng.uniform(0.0, 0.1, g.N)
+    w = rng.normal(0, 1, f0.shape)
+    v = rng.normal(0, 1, g.N)
+    omega, steps = 1.8, 3
+
+    def run(ph, keep=False):
+        L = orc.Lattice(g.nbx, g.nby, g.nbz)
+        L.f[:] = f0
+        rec = []
+        for _ in range(steps):
+            if keep:
+                rec.append((L.f.copy(), L.even))
+            L.collide(g.fluid, omega, porosity=ph)
+            L.swap()
+        return L.f.copy(), rec
+
+    h = 1e-7
+    fd = float(w @ ((run(phi + h * v)[0] - run(phi - h * v)[0]) / (2 * h)))
+    _, rec = run(phi, keep=True)
+    lam = w.copy()
+    padj = np.zeros(g.N)
+    A = orc.Lattice(g.nbx, g.nby, g.nbz)
+    for state, even in reversed(rec):
+        A.f[:] = state
+        A.even = even
+        orc.adjoint_step(A, lam, g.fluid, omega, porosity=phi, porosity_adjoint=padj)
+    assert float(padj @ v) == pytest.approx(fd, rel=1e-6)

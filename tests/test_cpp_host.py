@@ -6,6 +6,7 @@ GPU: cases run through UpdateGrid27 / KernelFactory / the BC and scaling factori
 bit-identical fields to the Python driver (same kernels, same order), which the other GPU tests
 hold to the oracle.
 """
+import ctypes
 import os
 import subprocess
 
@@ -19,6 +20,26 @@ def _binary():
     if not os.path.exists(HOST_TEST):
         pytest.fail(f"{HOST_TEST} is missing: build with virtualfluids_amd/build.py")
     return HOST_TEST
+
+
+def test_ctypes_struct_layout_matches_c():
+    """Every field offset and struct size the ctypes binding assumes equals the C compiler's."""
+    from virtualfluids_amd import capi
+
+    r = subprocess.run([_binary(), "layout"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    structs = {"vf_lbm_parameter": capi.LBMParameter, "vf_q_boundary_conditions": capi.QBoundaryConditions,
+               "vf_interpolation_cells": capi.InterpolationCells,
+               "vf_interpolation_cell_neighbor": capi.InterpolationCellNeighbor}
+    for line in r.stdout.split("\n"):
+        if not line.strip():
+            continue
+        key, value = line.split()
+        head, field = key.split(".")
+        if head == "sizeof":
+            assert ctypes.sizeof(structs[field]) == int(value), key
+        else:
+            assert getattr(structs[head], field).offset == int(value), key
 
 
 def test_factories_and_managers():

@@ -2,10 +2,10 @@
 
 ``k17_steps(f0, lattice, steps, checkpoint_every, bc_factory, velocity, rho_bc)`` runs
 `steps` time steps of one Lattice from the population buffer f0 with the order of
-UpdateGrid27 (collision, velocity BC, no-slip BC, parity swap, pressure BC;
+UpdateGrid27 (collision, velocity BC, no-slip BC, geometry BC, parity swap, pressure BC;
 src/gpu/core/Calculation/UpdateGrid27.cpp:54-96) and returns the final buffer. Its backward
 sweeps the steps in reverse with the adjoint kernels (vf_k17_adjoint[_box] and the BC
-adjoints): pressure BC, then no-slip, then velocity, then the collision. The forward keeps a
+adjoints): pressure BC, then geometry, no-slip and velocity, then the collision. The forward keeps a
 copy of the buffer every `checkpoint_every` steps; the backward replays one segment at a time
 from its checkpoint (out of place on the dense box, so the replay writes each intermediate
 state once and copies nothing) and walks the segment backwards. Memory: steps/K + K + 1
@@ -52,6 +52,7 @@ class Stepper:
         self.velocity = f.get_velocity_boundary_condition_post()
         self.noslip = f.get_no_slip_boundary_condition_post()
         self.pressure = f.get_pressure_boundary_condition_pre()
+        self.geometry = f.get_geometry_boundary_condition_post()
         for name, fn, label in (("velocity", self.velocity, "velocityBoundaryConditionPost"),
                                 ("pressure", self.pressure, "directionalPressureBoundaryConditionPre")):
             bc = L.bcs.get(name)
@@ -83,7 +84,7 @@ class Stepper:
                 capi.call("vf_k17_run_on_indices", ctypes.byref(p), _vp(L.fluid), int(L.grid.fluid.size),
                           capi.COLLISION_DEFAULT, s)
         p = _param(L, target, even)
-        for name, fn in (("velocity", self.velocity), ("noslip", self.noslip)):
+        for name, fn in (("velocity", self.velocity), ("noslip", self.noslip), ("geometry", self.geometry)):
             bc = self._bc(name)
             if fn and bc is not None:
                 capi.call(fn, ctypes.byref(p), ctypes.byref(bc.struct), s)
@@ -103,7 +104,8 @@ class Stepper:
             capi.call(self.pressure + "Adjoint", ctypes.byref(_param(L, after, not even)), ctypes.byref(bc.struct),
                       _vp(lam), _vp(rbar), s)
         p = _param(L, before, even)
-        for name, fn, grad in (("noslip", self.noslip, None), ("velocity", self.velocity, ubar)):
+        for name, fn, grad in (("geometry", self.geometry, None), ("noslip", self.noslip, None),
+                               ("velocity", self.velocity, ubar)):
             bc = self._bc(name)
             if fn and bc is not None:
                 capi.call(fn + "Adjoint", ctypes.byref(p), ctypes.byref(bc.struct), _vp(lam), _vp(grad), s)
@@ -225,3 +227,45 @@ def kinetic_energy(buf: torch.Tensor, index: torch.Tensor, nodes: torch.Tensor |
     if nodes is not None:
         ke = ke[nodes]
     return ke.sum()
+
+
+def _cell_index(grid, k: np.ndarray, d: int, even: bool) -> np.ndarray:
+    """Flat buffer index of the cell holding pre-collision direction d of nodes k."""
+    loc = k.astype(np.int64)
+    if CX[d] < 0:
+        loc = grid.nbx[loc].astype(np.int64)
+    if CY[d] < 0:
+        loc = grid.nby[loc].astype(np.int64)
+    if CZ[d] < 0:
+        loc = grid.nbz[loc].astype(np.int64)
+    slot = d if even else INV[d]
+    return slot * grid.N + loc
+
+
+def momentum_exchange_index(grid, bc_name: str, even: bool):
+    """Cells of every active link of a BC list in a state of parity `even` taken right after a
+    step: for link d of node n, the population that left n towards the wall (now in the
+    pre-collision d cell of the node across the link) and the one the BC sent back (n's
+    pre-collision inv(d) cell). Returns (out_idx, in_idx, c) with c the link directions (3, M)."""
+    bc = grid.bcs[bc_name]
+    q = bc.q27.reshape(27, bc.n)
+    x, y, z = grid.coords()
+    outs, ins, cs = [], [], []
+    for d in range(1, 27):
+        act = (q[d] >= 0) & (q[d] <= 1)
+        if not np.any(act):
+            continue
+        n = bc.k[act].astype(np.int64)
+        m = grid.index((x[n] + CX[d]) % grid.nx, (y[n] + CY[d]) % grid.ny, (z[n] + CZ[d]) % grid.nz)
+        outs.append(_cell_index(grid, m, d, even))
+        ins.append(_cell_index(grid, n, INV[d], even))
+        cs.append(np.repeat(np.array([[CX[d]], [CY[d]], [CZ[d]]]), n.size, axis=1))
+    return np.concatenate(outs), np.concatenate(ins), np.concatenate(cs, axis=1).astype(np.float64)
+
+
+def momentum_exchange(buf: torch.Tensor, out_idx: torch.Tensor, in_idx: torch.Tensor, c: torch.Tensor) -> torch.Tensor:
+    """Force on the boundary (3,) in FP64 from the momentum its links exchanged in the last step:
+    sum over links of c_d (f_d out + f_inv(d) back), the momentum-exchange form of the drag and
+    lift sums of src/gpu/core/PostProcessor/DragLift.cu:50-300."""
+    flat = buf.reshape(-1)
+    return (c * (flat[out_idx].to(torch.float64) + flat[in_idx].to(torch.float64))).sum(1)

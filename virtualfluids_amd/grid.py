@@ -181,3 +181,60 @@ def cavity(n: int, lid_velocity: float = 0.05) -> Grid:
     walls = {s: ("noslip",) for s in (dP00, dM00, d0P0, d0M0, d00M)}
     walls[d00P] = ("velocity", (lid_velocity, 0.0, 0.0))
     return walled_box((n, n, n), walls)
+
+
+def sphere_channel(n: tuple, center: tuple, radius: float, inflow: float, wall_q: float = 0.5) -> Grid:
+    """Channel along x (velocity inflow at x-min, pressure outlet at x-max, no-slip y/z walls)
+    with a solid sphere; the sphere's links form the "geometry" BC list with the sub-grid
+    distance q of each link from the exact ray-sphere intersection.
+
+    n: fluid extent (fx, fy, fz); center/radius in fluid-node coordinates (node (0,0,0) is the
+    first fluid node)."""
+    g = channel(n, inflow, q=wall_q)
+    x, y, z = g.coords()
+    # fluid coordinates: the walled box pads one solid layer on each non-periodic side
+    cx0, cy0, cz0 = (np.asarray(center, dtype=np.float64) + 1.0)
+    X = np.stack([x, y, z]).astype(np.float64)
+    C = np.array([cx0, cy0, cz0])
+    inside = ((X - C[:, None]) ** 2).sum(0) <= radius * radius
+    fluid = (g.geo == GEO_FLUID) & ~inside
+    g.geo = np.where(fluid, GEO_FLUID, GEO_SOLID).astype(np.uint32)
+    g.fluid = np.nonzero(fluid)[0].astype(np.uint32)
+    # drop sphere nodes from the other BC lists (a sphere touching a wall is not supported)
+    for bc in g.bcs.values():
+        if np.any(inside[bc.k]):
+            raise ValueError("the sphere overlaps a boundary layer")
+    kf = np.nonzero(fluid)[0]
+    q27 = np.full((27, kf.size), -1.0)
+    for d in range(1, 27):
+        c = np.array([CX[d], CY[d], CZ[d]], dtype=np.float64)
+        xn = (x[kf] + CX[d]) % g.nx, (y[kf] + CY[d]) % g.ny, (z[kf] + CZ[d]) % g.nz
+        hit = inside[g.index(*xn)]
+        if not np.any(hit):
+            continue
+        rel = X[:, kf[hit]] - C[:, None]
+        a = c @ c
+        b = 2.0 * (c @ rel)
+        cc = (rel * rel).sum(0) - radius * radius
+        t = (-b - np.sqrt(np.maximum(b * b - 4 * a * cc, 0.0))) / (2 * a)
+        q27[d, hit] = np.clip(t, 0.0, 1.0)
+    nodes = np.nonzero((q27 >= 0).any(axis=0))[0]
+    g.bcs["geometry"] = BoundaryNodes(k=kf[nodes].astype(np.int32), q27=q27[:, nodes].ravel())
+    return g
+
+
+def gaussian_random_field(shape: tuple, correlation_length: float, seed: int, low: float = 0.0,
+                          high: float = 0.1) -> np.ndarray:
+    """Seeded Gaussian random field on a periodic (nz, ny, nx) grid: white noise filtered in
+    Fourier space with exp(-k^2 l^2 / 2), mapped linearly onto [low, high]; returned flattened
+    in node order k = x + nx (y + ny z)."""
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    noise = rng.normal(0.0, 1.0, (nz, ny, nx))
+    kz = np.fft.fftfreq(nz) * 2 * np.pi
+    ky = np.fft.fftfreq(ny) * 2 * np.pi
+    kx = np.fft.fftfreq(nx) * 2 * np.pi
+    K2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    field = np.real(np.fft.ifftn(np.fft.fftn(noise) * np.exp(-0.5 * K2 * correlation_length ** 2)))
+    lo, hi = field.min(), field.max()
+    return (low + (high - low) * (field - lo) / (hi - lo)).ravel()

@@ -4,7 +4,7 @@
 (same enum names and defaults: velocity/pressure NotSpecified -> no function, no-slip
 NoSlipDelayBounceBack -> an empty function). ``Simulation.step`` mirrors
 UpdateGrid27::updateGrid for one level (src/gpu/core/Calculation/UpdateGrid27.cpp:54-96):
-collision -> post-collision BCs (velocity, then no-slip) -> parity swap -> pre-collision
+collision -> post-collision BCs (velocity, no-slip, geometry) -> parity swap -> pre-collision
 BCs (pressure).
 """
 from __future__ import annotations
@@ -52,6 +52,7 @@ class BoundaryConditionFactory:
         self.velocity = self.VelocityBC.NotSpecified
         self.no_slip = self.NoSlipBC.NoSlipDelayBounceBack
         self.pressure = self.PressureBC.NotSpecified
+        self.geometry = self.NoSlipBC.NoSlipDelayBounceBack
 
     def set_velocity_boundary_condition(self, bc: "BoundaryConditionFactory.VelocityBC"):
         self.velocity = bc
@@ -61,6 +62,11 @@ class BoundaryConditionFactory:
 
     def set_pressure_boundary_condition(self, bc: "BoundaryConditionFactory.PressureBC"):
         self.pressure = bc
+
+    def set_geometry_boundary_condition(self, bc):
+        """A velocity or no-slip BC for the geometry's nodes (BoundaryConditionFactory.h:129-139);
+        default NoSlipDelayBounceBack."""
+        self.geometry = bc
 
     def _resolve(self, table, value, unsupported_ok=False):
         if value in table:
@@ -80,6 +86,10 @@ class BoundaryConditionFactory:
     def get_pressure_boundary_condition_pre(self):
         return self._resolve(self._PRESSURE, self.pressure)
 
+    def get_geometry_boundary_condition_post(self):
+        table = self._VELOCITY if isinstance(self.geometry, self.VelocityBC) else self._NOSLIP
+        return self._resolve(table, self.geometry)
+
 
 class Simulation:
     """Single-level time stepping of one Lattice."""
@@ -89,6 +99,7 @@ class Simulation:
         f = bc_factory or BoundaryConditionFactory()
         self.velocity_fn = f.get_velocity_boundary_condition_post()
         self.noslip_fn = f.get_no_slip_boundary_condition_post()
+        self.geometry_fn = f.get_geometry_boundary_condition_post()
         self.pressure_fn = f.get_pressure_boundary_condition_pre()
         # BoundaryConditionKernelManager: BC nodes without a function is an error
         # (src/gpu/core/BoundaryConditions/BoundaryConditionKernelManager.cpp:53-83)
@@ -106,6 +117,8 @@ class Simulation:
             L.apply_bc(self.velocity_fn, "velocity", stream)
         if self.noslip_fn:
             L.apply_bc(self.noslip_fn, "noslip", stream)
+        if self.geometry_fn:
+            L.apply_bc(self.geometry_fn, "geometry", stream)
         L.swap()
         if self.pressure_fn:
             L.apply_bc(self.pressure_fn, "pressure", stream)
