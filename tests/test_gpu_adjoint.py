@@ -87,7 +87,7 @@ def _gpu_gradient(L, f0, idx, steps, K):
     x = torch.from_numpy(f0.astype(np.float32)).cuda().requires_grad_(True)
     J = kinetic_energy(k17_steps(x, L, steps, checkpoint_every=K, even=True), idx)
     J.backward()
-    return float(J), x.grad.cpu().numpy().astype(np.float64)
+    return float(J.detach()), x.grad.cpu().numpy().astype(np.float64)
 
 
 def _oracle_J(orc, grid, buf, omega, steps, even_final):

@@ -49,14 +49,21 @@ def test_one_step_matches_oracle(gpu, orc, dims, box, even):
     assert err < 2e-7, err
 
 
-# row kernel widths: 0 = scalar kernel; 1/2/4 nodes per lane (rows of 1-4 waves)
+# row kernel widths: 0 = scalar kernel; 1/2/4 nodes per lane (rows of 1-4 waves); the row
+# kernel addresses the direction arrays through buffer descriptors up to 4 GiB per 14 slots
+# and through 64-bit global addresses beyond (forced here with VFAMD_ROW_BUFFER=0)
+@pytest.mark.parametrize("addressing", ["buffer", "global"])
 @pytest.mark.parametrize("width", [0, 1, 2, 4])
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("dims", [(64, 5, 3), (128, 4, 3), (256, 3, 2), (512, 2, 2)])
-def test_row_kernel_widths_match_oracle(gpu, orc, monkeypatch, width, storage, dims):
-    """Every row-kernel width, storage type and row length (1 to 4 waves per row, with the
-    periodic wrap) against the FP64 oracle applied to the same stored (rounded) input."""
+def test_row_kernel_widths_match_oracle(gpu, orc, monkeypatch, width, storage, dims, addressing):
+    """Every row-kernel width, storage type, addressing form and row length (1 to 4 waves per
+    row, with the periodic wrap) against the FP64 oracle applied to the same stored (rounded)
+    input."""
+    if width == 0 and addressing == "global":
+        pytest.skip("the scalar kernel has one addressing form")
     monkeypatch.setenv("VFAMD_ROW_WIDTH", str(width))
+    monkeypatch.setenv("VFAMD_ROW_BUFFER", "1" if addressing == "buffer" else "0")
     grid = periodic_box(*dims)
     rng = np.random.default_rng(200 + width + sum(dims))
     f0 = near_equilibrium(rng, grid.N, u_amp=0.04, noise=5e-3)
@@ -74,7 +81,7 @@ def test_row_kernel_widths_match_oracle(gpu, orc, monkeypatch, width, storage, d
         # FP32: a few ulp; FP16/BF16: one storage rounding of the output (|f| < 0.06)
         tol = {"fp32": 2e-7, "fp16": 0.06 * 2.0 ** -11 + 2e-7, "bf16": 0.06 * 2.0 ** -8 + 2e-7}[storage]
         err = np.abs(got - ref.f).max()
-        print(f"width={width} {storage} dims={dims} even={even} err={err:.3e}")
+        print(f"width={width} {storage} {addressing} dims={dims} even={even} err={err:.3e}")
         assert err < tol, err
 
 

@@ -142,10 +142,17 @@ __host__ __device__ constexpr int shifted_index(int d)
 
 constexpr int kRowMaxThreads = 256;
 
-template <class S, int V, bool EVEN, bool MASK, bool MACRO>
+// BUF: the direction arrays are addressed through two buffer descriptors (slots below
+// kRowSplit from f, the rest from f + kRowSplit slots), which needs kRowSplit slots to span
+// less than 4 GiB; otherwise each direction takes a 64-bit base and a global access.
+constexpr int kRowSplit = 14;
+
+template <class S, int V, bool EVEN, bool MASK, bool MACRO, bool BUF>
 __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__ f, BoxArgs a, K17Rates r)
 {
-    __shared__ float xin[kRowMaxThreads / 64][9];
+    using IO = RowIO<S, V>;
+    constexpr bool MIX = Codec<S>::storage == VF_STORAGE_FP16;
+    __shared__ unsigned xin[kRowMaxThreads / 64][9];
     __shared__ float xout[kRowMaxThreads / 64][9];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
     const uint32_t y = blockIdx.x, z = blockIdx.y;
@@ -161,43 +168,66 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__
     const uint32_t boff[4] = {rows[0] * (uint32_t)sizeof(S), rows[1] * (uint32_t)sizeof(S),
                               rows[2] * (uint32_t)sizeof(S), rows[3] * (uint32_t)sizeof(S)};
     const uint64_t slotBytes = a.N * sizeof(S);
-
-    float v[27][V];
-#pragma unroll
-    for (int d = 0; d < 27; ++d) {
-        const char* base = reinterpret_cast<const char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
-        RowIO<S, V>::load(reinterpret_cast<const S*>(base + boff[node_sel(d) >> 1]), v[d]);
+    __amdgpu_buffer_rsrc_t rsLo, rsHi;
+    if constexpr (BUF) {
+        rsLo = __builtin_amdgcn_make_buffer_rsrc(f, (short)0, (int)0xffffffff, 0x00020000);
+        rsHi = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<char*>(f) + kRowSplit * slotBytes, (short)0,
+                                                 (int)0xffffffff, 0x00020000);
     }
+    // byte offset of direction d's slot from its descriptor's base
+    auto soff = [&](int d) -> uint32_t {
+        const int sl = slot<EVEN>(d);
+        return (uint32_t)(sl >= kRowSplit ? sl - kRowSplit : sl) * (uint32_t)slotBytes;
+    };
+    auto load = [&](int d) {
+        if constexpr (BUF) return IO::load(slot<EVEN>(d) >= kRowSplit ? rsHi : rsLo, boff[node_sel(d) >> 1], soff(d));
+        else {
+            const char* base = reinterpret_cast<const char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
+            return IO::load(reinterpret_cast<const S*>(base + boff[node_sel(d) >> 1]));
+        }
+    };
+    auto store = [&](int d, const float (&vals)[V]) {
+        if constexpr (BUF) IO::store(slot<EVEN>(d) >= kRowSplit ? rsHi : rsLo, boff[node_sel(d) >> 1], soff(d), vals);
+        else {
+            char* base = reinterpret_cast<char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
+            IO::store(reinterpret_cast<S*>(base + boff[node_sel(d) >> 1]), vals);
+        }
+    };
 
-    // element after this lane's last node, for the c_x = -1 directions
-    float nxt[9];
+    typename IO::Raw raw[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) raw[d] = load(d);
+
+    // word holding the element after this lane's last node, for the c_x = -1 directions
+    unsigned nxt[9];
     if (lane == 0) {
 #pragma unroll
-        for (int j = 0; j < 9; ++j) xin[wave][j] = v[kShiftedDirs[j]][0];
+        for (int j = 0; j < 9; ++j) xin[wave][j] = IO::word0(raw[kShiftedDirs[j]]);
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 9; ++j) nxt[j] = __shfl(v[kShiftedDirs[j]][0], (lane + 1) & 63, 64);
+    for (int j = 0; j < 9; ++j) nxt[j] = __shfl(IO::word0(raw[kShiftedDirs[j]]), (lane + 1) & 63, 64);
     if (lane == 63) {
         const int src = (wave + 1 == W) ? 0 : wave + 1;
 #pragma unroll
         for (int j = 0; j < 9; ++j) nxt[j] = xin[src][j];
     }
 
+    float out[27][V];
     float carry[9];
 #pragma unroll
     for (int i = 0; i < V; ++i) {
         float fl[27];
 #pragma unroll
         for (int d = 0; d < 27; ++d) {
-            if (node_sel(d) & 1) fl[d] = (i + 1 < V) ? v[d][(i + 1) % V] : nxt[shifted_index(d)];
-            else fl[d] = v[d][i];
+            if (node_sel(d) & 1) fl[d] = (i + 1 < V) ? IO::elem(raw[d], i + 1) : IO::of_word(nxt[shifted_index(d)], 0);
+            else fl[d] = IO::elem(raw[d], i);
         }
         bool fluid = true;
         if (MASK) fluid = a.mask[rows[0] + i] != 0;
         if (fluid) {
             MacroOut mo;
-            k17_collide(fl, r, mo);
+            k17_collide<MIX>(fl, r, mo);
             if (MACRO) {
                 a.rho[rows[0] + i] = mo.drho;
                 a.vx[rows[0] + i] = mo.vx;
@@ -219,10 +249,10 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__
         for (int d = 0; d < 27; ++d) {
             const float val = fl[inv(d)];
             if (node_sel(d) & 1) {
-                if (i + 1 < V) v[d][(i + 1) % V] = val;
+                if (i + 1 < V) out[d][i + 1] = val;
                 else carry[shifted_index(d)] = val;
             } else {
-                v[d][i] = val;
+                out[d][i] = val;
             }
         }
     }
@@ -231,8 +261,7 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
         if (node_sel(d) & 1) continue;
-        char* base = reinterpret_cast<char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
-        RowIO<S, V>::store(reinterpret_cast<S*>(base + boff[node_sel(d) >> 1]), v[d]);
+        store(d, out[d]);
     }
 
     // element before this lane's first node, for the c_x = -1 directions
@@ -246,9 +275,8 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__
         float s0 = __shfl(carry[j], (lane + 63) & 63, 64);
         if (lane == 0) s0 = xout[(wave == 0) ? W - 1 : wave - 1][j];
         const int d = kShiftedDirs[j];
-        v[d][0] = s0;
-        char* base = reinterpret_cast<char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
-        RowIO<S, V>::store(reinterpret_cast<S*>(base + boff[node_sel(d) >> 1]), v[d]);
+        out[d][0] = s0;
+        store(d, out[d]);
     }
 }
 
@@ -323,15 +351,32 @@ template <class S> static int row_width()
     return 2;
 }
 
-template <class S, int V, bool EVEN>
-static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, hipStream_t s)
+template <class S, int V, bool EVEN, bool BUF>
+static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, hipStream_t s)
 {
     dim3 grid(a.ny, a.nz);
     dim3 block(a.nx / V);
-    if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true>), grid, block, 0, s, f, a, r);
-    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false>), grid, block, 0, s, f, a, r);
-    else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true>), grid, block, 0, s, f, a, r);
-    else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false>), grid, block, 0, s, f, a, r);
+    if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true, BUF>), grid, block, 0, s, f, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF>), grid, block, 0, s, f, a, r);
+    else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true, BUF>), grid, block, 0, s, f, a, r);
+    else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF>), grid, block, 0, s, f, a, r);
+}
+
+// VFAMD_ROW_BUFFER=0 forces the global-address form (used by the tests to cover both forms).
+static bool row_buffer_allowed()
+{
+    const char* e = std::getenv("VFAMD_ROW_BUFFER");
+    return !(e && *e == '0');
+}
+
+template <class S, int V, bool EVEN>
+static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, hipStream_t s)
+{
+    // the larger descriptor spans kRowSplit slots; buffer offsets are 32-bit
+    if ((uint64_t)kRowSplit * a.N * sizeof(S) <= 0xffffffffull && row_buffer_allowed())
+        launch_row_b<S, V, EVEN, true>(a, r, mask, macro, f, s);
+    else
+        launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, s);
 }
 
 template <class S, bool EVEN>
@@ -346,7 +391,9 @@ static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
     // (porous media take the box kernel)
     const int V = row_width<S>();
     if (!p.porosity && V > 0 && p.nx % (64 * V) == 0 && p.nx / V <= kRowMaxThreads) {
-        if (V == 1 && sizeof(S) == 4) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, s); return; }
+        if constexpr (sizeof(S) == 4) {
+            if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, s); return; }
+        }
         if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, s); return; }
         if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, s); return; }
     }

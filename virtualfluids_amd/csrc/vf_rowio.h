@@ -1,7 +1,12 @@
 // Aligned, non-temporal vector loads and stores of V consecutive populations of one
-// direction array, decoded to / encoded from FP32. Used by the row kernel, where every
-// lane owns V consecutive x-nodes and every global access of a wave is aligned and
-// contiguous (the x-shift of the eso-twist layout is resolved with lane shuffles).
+// direction array. Used by the row kernel, where every lane owns V consecutive x-nodes and
+// every global access of a wave is aligned and contiguous (the x-shift of the eso-twist
+// layout is resolved with lane shuffles).
+//
+// Loads keep the raw lane word; elements are widened to FP32 where they are used, so FP16
+// values can enter v_fma_mix_f32 without a conversion instruction. Stores narrow two
+// values per instruction (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32, round to nearest even,
+// the rounding of Codec<S>::store).
 #pragma once
 
 #include "vf_common.h"
@@ -13,6 +18,10 @@ template <> struct RawT<2> { using T = unsigned short; };
 template <> struct RawT<4> { using T = unsigned int; };
 template <> struct RawT<8> { using T = unsigned int __attribute__((ext_vector_type(2))); };
 template <> struct RawT<16> { using T = unsigned int __attribute__((ext_vector_type(4))); };
+
+typedef float rowio_f2 __attribute__((ext_vector_type(2)));
+typedef _Float16 rowio_h2 __attribute__((ext_vector_type(2)));
+typedef __bf16 rowio_b2 __attribute__((ext_vector_type(2)));
 
 template <int BYTES> __device__ __forceinline__ unsigned word(const typename RawT<BYTES>::T& r, int i)
 {
@@ -27,51 +36,67 @@ template <int BYTES> __device__ __forceinline__ void set_word(typename RawT<BYTE
 
 template <class S, int V> struct RowIO {
     static constexpr int BYTES = V * (int)sizeof(S);
+    static constexpr int PER_WORD = 4 / (int)sizeof(S); // elements per 32-bit word
     using Raw = typename RawT<BYTES>::T;
 
-    __device__ static void load(const S* p, float (&out)[V])
+    __device__ static Raw load(const S* p) { return __builtin_nontemporal_load(reinterpret_cast<const Raw*>(p)); }
+
+    // element j (0 or 1) of a 32-bit word, widened to FP32
+    __device__ static float of_word(unsigned w, int j)
     {
-        const Raw r = __builtin_nontemporal_load(reinterpret_cast<const Raw*>(p));
-        if constexpr (sizeof(S) == 4) {
-#pragma unroll
-            for (int i = 0; i < V; ++i) out[i] = __uint_as_float(word<BYTES>(r, i));
-        } else if constexpr (Codec<S>::storage == VF_STORAGE_BF16) {
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                const unsigned w = word<BYTES>(r, i / 2);
-                out[i] = __uint_as_float((i & 1) ? (w & 0xffff0000u) : (w << 16));
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                const unsigned w = word<BYTES>(r, i / 2);
-                out[i] = __half2float(__ushort_as_half((unsigned short)((i & 1) ? (w >> 16) : (w & 0xffffu))));
-            }
-        }
+        if constexpr (sizeof(S) == 4) return __uint_as_float(w);
+        else if constexpr (Codec<S>::storage == VF_STORAGE_BF16) return __uint_as_float(j ? (w & 0xffff0000u) : (w << 16));
+        else return (float)__builtin_bit_cast(rowio_h2, w)[j];
+    }
+
+    // element i of the lane's V values
+    __device__ static float elem(const Raw& r, int i) { return of_word(word<BYTES>(r, i / PER_WORD), i % PER_WORD); }
+
+    // the word holding element 0 (what the previous lane needs for its shifted directions)
+    __device__ static unsigned word0(const Raw& r) { return word<BYTES>(r, 0); }
+
+    // cache policy of the buffer forms: non-temporal (each population is touched once per step)
+    static constexpr int kNT = 2;
+
+    // buffer forms: address = descriptor base + soff (uniform, SGPR) + voff (per lane), so no
+    // per-lane 64-bit address arithmetic is needed
+    __device__ static Raw load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
+    {
+        if constexpr (BYTES == 4) return __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, kNT);
+        else if constexpr (BYTES == 8) return __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, kNT);
+        else return __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, kNT);
+    }
+    __device__ static void store(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff, const float (&in)[V])
+    {
+        const Raw r = pack(in);
+        if constexpr (BYTES == 4) __builtin_amdgcn_raw_buffer_store_b32(r, rs, voff, soff, kNT);
+        else if constexpr (BYTES == 8) __builtin_amdgcn_raw_buffer_store_b64(r, rs, voff, soff, kNT);
+        else __builtin_amdgcn_raw_buffer_store_b128(r, rs, voff, soff, kNT);
     }
 
     __device__ static void store(S* p, const float (&in)[V])
+    {
+        __builtin_nontemporal_store(pack(in), reinterpret_cast<Raw*>(p));
+    }
+
+    __device__ static Raw pack(const float (&in)[V])
     {
         Raw r;
         if constexpr (sizeof(S) == 4) {
 #pragma unroll
             for (int i = 0; i < V; ++i) set_word<BYTES>(r, i, __float_as_uint(in[i]));
         } else {
+            static_assert(V % 2 == 0, "16-bit rows store whole 32-bit words");
 #pragma unroll
             for (int i = 0; i < V; i += 2) {
-                unsigned lo, hi = 0;
-                if constexpr (Codec<S>::storage == VF_STORAGE_BF16) {
-                    lo = __bfloat16_as_ushort(__float2bfloat16(in[i]));
-                    if (i + 1 < V) hi = __bfloat16_as_ushort(__float2bfloat16(in[i + 1]));
-                } else {
-                    lo = __half_as_ushort(__float2half_rn(in[i]));
-                    if (i + 1 < V) hi = __half_as_ushort(__float2half_rn(in[i + 1]));
-                }
-                if constexpr (BYTES == 2) r = (Raw)lo;
-                else set_word<BYTES>(r, i / 2, lo | (hi << 16));
+                const rowio_f2 x = {in[i], in[i + 1]};
+                unsigned w;
+                if constexpr (Codec<S>::storage == VF_STORAGE_BF16) w = __builtin_bit_cast(unsigned, __builtin_convertvector(x, rowio_b2));
+                else w = __builtin_bit_cast(unsigned, __builtin_convertvector(x, rowio_h2));
+                set_word<BYTES>(r, i / 2, w);
             }
         }
-        __builtin_nontemporal_store(r, reinterpret_cast<Raw*>(p));
+        return r;
     }
 };
 
