@@ -14,6 +14,7 @@
 //             workgroups, hence none between the XCDs' L2 caches).
 //  * indexed: the reference's sparse layout (fluid-node list + neighbour arrays,
 //             src/gpu/core/Utilities/RunCollision.cuh:74-143).
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -340,15 +341,48 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
 
 // ---- host launchers ----------------------------------------------------------------------
 // Width (nodes per lane) of the row kernel per storage type; 0 selects the scalar kernel.
-// Measured on MI355X at 256^3: FP32 1 node/lane (5.8 TB/s vs 4.9 scalar); FP16 and BF16
-// 2 nodes/lane (4-byte accesses; both close to the VALU issue limit).
+// Measured on MI355X at 256^3: FP32 1 node/lane (5.9 TB/s; 2 nodes/lane 5.8, scalar 4.9);
+// FP16 and BF16 4 nodes/lane where rows allow it (1-2 % above 2 nodes/lane; both near the
+// VALU issue limit), else 2.
 // VFAMD_ROW_WIDTH overrides the choice (0, 1, 2 or 4) for benchmarking.
-template <class S> static int row_width()
+template <class S> static int row_width(int nx)
 {
     const char* e = std::getenv("VFAMD_ROW_WIDTH");
     if (e && *e) return std::atoi(e);
     if (Codec<S>::storage == VF_STORAGE_FP32) return 1;
-    return 2;
+    return (nx % 256 == 0 && nx / 4 <= kRowMaxThreads) ? 4 : 2;
+}
+
+// Resident workgroups per CU for the row kernel (0: as many as registers allow). The FP32
+// kernel moves the most bytes per instruction and runs best with few resident rows: at 256^3
+// on MI355X, 8 resident 4-wave workgroups per CU give 5.72 TB/s, 3-4 give 5.84-5.86 and 2
+// give 5.92-5.93 (fewer concurrent direction streams per memory channel); one gives 3.8. The 16-bit
+// kernels are bound by VALU issue and need every wave they can get. The cap is imposed by
+// reserving LDS at launch. VFAMD_ROW_WGS overrides it for benchmarking.
+template <class S, int V, bool BUF> static int row_wgs_per_cu()
+{
+    const char* e = std::getenv("VFAMD_ROW_WGS");
+    if (e && *e) return std::atoi(e);
+    return (Codec<S>::storage == VF_STORAGE_FP32 && V == 1 && BUF) ? 2 : 0;
+}
+
+// dynamic LDS bytes that leave room for at most k workgroups of the row kernel on one CU
+static size_t row_lds_reserve(int k)
+{
+    if (k <= 0) return 0;
+    static std::atomic<int> cached[64]; // LDS bytes per CU of each device, 0 until queried
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (dev >= 0 && dev < 64) lds = cached[dev].load(std::memory_order_relaxed);
+    if (lds <= 0) {
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || lds <= 0)
+            return 0;
+        if (dev >= 0 && dev < 64) cached[dev].store(lds, std::memory_order_relaxed);
+    }
+    // whole KiB below the share of one workgroup, less 1 KiB for the kernel's own LDS
+    // words and the allocation granularity
+    const long bytes = (long)(lds / k / 1024) * 1024 - 1024;
+    return bytes > 0 ? (size_t)bytes : 0;
 }
 
 template <class S, int V, bool EVEN, bool BUF>
@@ -356,10 +390,11 @@ static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
 {
     dim3 grid(a.ny, a.nz);
     dim3 block(a.nx / V);
-    if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true, BUF>), grid, block, 0, s, f, a, r);
-    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF>), grid, block, 0, s, f, a, r);
-    else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true, BUF>), grid, block, 0, s, f, a, r);
-    else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF>), grid, block, 0, s, f, a, r);
+    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>());
+    if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true, BUF>), grid, block, lds, s, f, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF>), grid, block, lds, s, f, a, r);
+    else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true, BUF>), grid, block, lds, s, f, a, r);
+    else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF>), grid, block, lds, s, f, a, r);
 }
 
 // VFAMD_ROW_BUFFER=0 forces the global-address form (used by the tests to cover both forms).
@@ -389,7 +424,7 @@ static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
     const bool mask = p.fluidMask != nullptr;
     // row kernel: a row of nx nodes is nx/V lanes, whole waves, at most one workgroup
     // (porous media take the box kernel)
-    const int V = row_width<S>();
+    const int V = row_width<S>(p.nx);
     if (!p.porosity && V > 0 && p.nx % (64 * V) == 0 && p.nx / V <= kRowMaxThreads) {
         if constexpr (sizeof(S) == 4) {
             if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, s); return; }
