@@ -283,9 +283,41 @@ int vf_ConservationSums(const vf_lbm_parameter* para, double* out4, void* stream
 
 /* One FP64 pass over the same nodes: out5[0..3] as vf_ConservationSums, out5[4] = sum of the
  * kinetic energy 1/2 (1+drho)|u|^2 = |j|^2 / (2 (1+drho)) (out5 device pointer, overwritten).
- * The energy term follows kineticEnergyKernel (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:99-153);
- * divide by the fluid-node count for the reference's mean. */
+ * The energy term is kineticEnergyKernel's (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:99-153)
+ * over every fluid node; vf_KineticEnergy applies the reference's node selection. */
 int vf_MomentSums(const vf_lbm_parameter* para, double* out5, void* stream);
+
+/* DragLiftPostD27 (src/gpu/core/PostProcessor/DragLift.cu:558-566, kernel DragLift.cu:50-302):
+ * per BC node i, the post-collision half of the momentum exchange over the links e with
+ * q27[e*n + i] in [0, 1]: drag{X,Y,Z}[i] = -sum_e c_e f_e^post (doubles, device arrays of
+ * numberOfBCnodes). Call after the collision and before the BC kernels, with the collision's
+ * isEvenTimestep (BoundaryConditionKernelManager.cpp:257-278). */
+int vf_DragLiftPostD27(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, double* dragX,
+                       double* dragY, double* dragZ, void* stream);
+
+/* DragLiftPreD27 (DragLift.cu:569-577, kernel DragLift.cu:304-556): the pre-collision half after
+ * the BC kernels and the parity swap, drag{X,Y,Z}[i] = -sum_e c_inv(e) f_inv(e)^pre
+ * (BoundaryConditionKernelManager.cpp:149-172). calcDragLift (DragLift.cpp:67-72) sums
+ * pre - post over the nodes. */
+int vf_DragLiftPreD27(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, double* dragX,
+                      double* dragY, double* dragZ, void* stream);
+
+/* The node sum of calcDragLift (DragLift.cpp:67-72), force3[0..2] = sum_i (pre_i - post_i), in
+ * one call on the state after the step (after the swap; the post-collision populations that
+ * reached the wall are still in the solid nodes' cells then). FP64, one workgroup, fixed
+ * summation order (identical results on every call). force3: device pointer to 3 doubles,
+ * overwritten. */
+int vf_DragLiftSum(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc, double* force3, void* stream);
+
+/* KineticEnergyAnalyzer::run (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:66-153):
+ * out2[0] = sum of 1/2 |u|^2 (1 + drho) over the nodes of type VF_GEO_FLUID whose six face
+ * neighbours and (-1,-1,-1) neighbour are fluid, out2[1] = their number (the reference's mean
+ * is out2[0] / out2[1]).
+ * Reads para->velocityX/Y/Z and para->rho (fill them with
+ * vf_CalculateMacroscopicQuantitiesCompressible first) and typeOfGridNode; neighborInverse
+ * (the (-1,-1,-1) neighbour) is needed with neighbour arrays. FP64 sums; out2 device pointer,
+ * overwritten. */
+int vf_KineticEnergy(const vf_lbm_parameter* para, const uint32_t* neighborInverse, double* out2, void* stream);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,8 @@
 //   refinement cell gather      src/gpu/core/Utilities/ScalingUtilities.h:53-241
 //   coarse -> fine kernel       src/gpu/core/GridScaling/scaleCoarseToFineCompressible.cu:46-528
 //   fine -> coarse kernel       src/gpu/core/GridScaling/scaleFineToCoarseCompressible.cu:43-124
+//   drag/lift momentum exchange src/gpu/core/PostProcessor/DragLift.cu:50-556, DragLift.cpp:67-72
+//   kinetic energy analyzer     src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:66-153
 //   adjoint (no reference counterpart): forward-mode dual numbers through k17_collide
 #include <cstdint>
 #include <cstring>
@@ -451,6 +453,64 @@ void macroscopic(const Lattice<T>& L, const u32* geo, bool even, T* rho, T* vx, 
     }
 }
 
+// ---- drag and lift by momentum exchange -------------------------------------------------------
+// DragLiftPost27 / DragLiftPre27 (src/gpu/core/PostProcessor/DragLift.cu:50-302, 304-556): for a BC
+// node and every link e with q_e in [0, 1],
+//   post half (after the collision, before the BC kernels, collision parity): the population e
+//     leaving towards the wall,                    drag_post = -sum_e c_e f_e^post
+//   pre half (after the BC kernels and the parity swap): the population inv(e) coming back,
+//                                                  drag_pre  = -sum_e c_inv(e) f_inv(e)^pre
+// calcDragLift (DragLift.cpp:67-72) sums pre - post over the nodes in double.
+template <class T>
+void drag_lift(const Lattice<T>& L, const int* k, const T* q, u64 nbc, bool even, bool post, double* dx, double* dy,
+               double* dz)
+{
+    T* P[27];
+    slot_table(L, even, P);
+    for (u64 i = 0; i < nbc; ++i) {
+        u32 n[8];
+        cell_nodes(L, (u32)k[i], n);
+        T f[27];
+        if (post) read_post(P, n, f);
+        else read_pre(P, n, f);
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int e = 1; e < 27; ++e) {
+            if (!active(q[(u64)e * nbc + i])) continue;
+            const int d = post ? e : INV[e]; // the population counted for link e
+            const double v = (double)f[d];
+            sx += CX[d] * v;
+            sy += CY[d] * v;
+            sz += CZ[d] * v;
+        }
+        dx[i] = -sx;
+        dy[i] = -sy;
+        dz[i] = -sz;
+    }
+}
+
+// KineticEnergyAnalyzer::run and kineticEnergyFunction
+// (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:66-153): nodes of type fluid whose six face
+// neighbours and (-1,-1,-1) neighbour (through which the -x/-y/-z neighbours are reached) are
+// fluid contribute 1/2 |u|^2 (1 + drho); out[0] = sum, out[1] = count.
+template <class T>
+void kinetic_energy(const u32* nbx, const u32* nby, const u32* nbz, const u32* nbinv, const u32* geo, u64 N,
+                    const T* vx, const T* vy, const T* vz, const T* rho, double* out)
+{
+    double e = 0.0, count = 0.0;
+    auto fluid = [&](u32 j) { return geo[j] == 19u; };
+    for (u64 k = 0; k < N; ++k) {
+        if (!fluid((u32)k)) continue;
+        const u32 m = nbinv[k];
+        if (!fluid(nbx[k]) || !fluid(nby[k]) || !fluid(nbz[k]) || !fluid(m) || !fluid(nbz[nby[m]]) ||
+            !fluid(nbz[nbx[m]]) || !fluid(nby[nbx[m]]))
+            continue;
+        count += 1.0;
+        e += 0.5 * ((double)vx[k] * vx[k] + (double)vy[k] * vy[k] + (double)vz[k] * vz[k]) * ((double)rho[k] + 1.0);
+    }
+    out[0] = e;
+    out[1] = count;
+}
+
 // ---- grid refinement ------------------------------------------------------------------------
 // The 8 nodes of the cell whose lowest corner is `base`: n = ix + 2 iy + 4 iz.
 template <class T> inline void cell_corner_nodes(const Lattice<T>& L, u32 base, u32 out[8])
@@ -627,6 +687,23 @@ template <class T> K17Param<T> make_param(T omega, const T* lim3, const T* force
 
 ORC_INSTANTIATE(double, f64)
 ORC_INSTANTIATE(float, f32)
+
+#define ORC_POSTPROC(T, SFX)                                                                                      \
+    extern "C" void orc_drag_lift_##SFX(T* buf, unsigned long long N, const uint32_t* nbx, const uint32_t* nby,     \
+                                        const uint32_t* nbz, const int* k, const T* q, unsigned long long nbc,    \
+                                        int even, int post, double* dx, double* dy, double* dz)                   \
+    {                                                                                                             \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                      \
+        drag_lift(L, k, q, nbc, even != 0, post != 0, dx, dy, dz);                                                \
+    }                                                                                                             \
+    extern "C" void orc_kinetic_energy_##SFX(unsigned long long N, const uint32_t* nbx, const uint32_t* nby,        \
+                                             const uint32_t* nbz, const uint32_t* nbinv, const uint32_t* geo,     \
+                                             const T* vx, const T* vy, const T* vz, const T* rho, double* out)    \
+    {                                                                                                             \
+        kinetic_energy(nbx, nby, nbz, nbinv, geo, N, vx, vy, vz, rho, out);                                       \
+    }
+ORC_POSTPROC(double, f64)
+ORC_POSTPROC(float, f32)
 
 #define ORC_SCALE(T, SFX)                                                                                         \
     extern "C" void orc_scale_##SFX(int fine_to_coarse, T* fc, unsigned long long Nc, const uint32_t* nbxc,        \

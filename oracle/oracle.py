@@ -164,6 +164,30 @@ class Lattice:
         self.even = not self.even
 
 
+def drag_lift(lattice, bc_k, q27, post):
+    """Per-node momentum-exchange halves (DragLiftPost27 with post=True after the collision,
+    DragLiftPre27 after the BC kernels and the swap), at the lattice's current parity:
+    an array (3, nbc) of doubles."""
+    k = np.ascontiguousarray(bc_k, dtype=np.int32)
+    q = np.ascontiguousarray(q27, dtype=lattice.dtype)
+    nbc = k.size
+    out = np.zeros((3, nbc))
+    lattice._fn("orc_drag_lift")(*lattice._nb(), _p(k), _p(q), ctypes.c_ulonglong(nbc), ctypes.c_int(int(lattice.even)),
+                                 ctypes.c_int(int(post)), _p(out[0]), _p(out[1]), _p(out[2]))
+    return out
+
+
+def kinetic_energy(nbx, nby, nbz, nbinv, geo, vx, vy, vz, rho):
+    """KineticEnergyAnalyzer's sum and node count over fluid nodes with six fluid face
+    neighbours, from macroscopic fields (double)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (nbx, nby, nbz, nbinv, geo)]
+    vals = [np.ascontiguousarray(a, dtype=np.float64) for a in (vx, vy, vz, rho)]
+    out = np.zeros(2)
+    lib().orc_kinetic_energy_f64(ctypes.c_ulonglong(arrs[0].size), *(_p(a) for a in arrs), *(_p(a) for a in vals),
+                                 _p(out))
+    return out
+
+
 def collide_box(f, nx, ny, nz, omega, even, limiters=(0.01, 0.01, 0.01), mask=None):
     """Dense periodic box collision (node k = x + nx*(y + ny*z)), in place on f (27*N)."""
     lim = np.asarray(limiters, dtype=f.dtype)

@@ -111,6 +111,10 @@ SIGNATURES = {
     "vf_CalculateMacroscopicQuantitiesCompressible": [_P, _vp],
     "vf_ConservationSums": [_P, _vp, _vp],
     "vf_MomentSums": [_P, _vp, _vp],
+    "vf_DragLiftPostD27": [_P, _Q, _vp, _vp, _vp, _vp],
+    "vf_DragLiftPreD27": [_P, _Q, _vp, _vp, _vp, _vp],
+    "vf_DragLiftSum": [_P, _Q, _vp, _vp],
+    "vf_KineticEnergy": [_P, _vp, _vp, _vp],
     "vf_ScaleCoarseToFineCompressible": [_P, _P, _vp, _vp, _vp],
     "vf_k17_adjoint_box": [_P, _vp, _vp, _vp, _vp],
     "vf_k17_adjoint": [_P, _vp, _vp, _vp, _vp],

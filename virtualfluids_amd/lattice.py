@@ -195,6 +195,39 @@ class Lattice:
         torch.cuda.synchronize(self.device)
         return self.sums.cpu().numpy().copy()
 
+    def drag_lift_half(self, name: str, post: bool, stream=None) -> np.ndarray:
+        """Per-node momentum-exchange half on the BC list `name` (DragLiftPost27 after the
+        collision, DragLiftPre27 after the BC kernels and the swap): host array (3, nbc)."""
+        bc = self.bcs[name]
+        out = torch.zeros((3, max(bc.n, 1)), dtype=torch.float64, device=self.device)
+        fn = "vf_DragLiftPostD27" if post else "vf_DragLiftPreD27"
+        capi.call(fn, self.pref(), ctypes.byref(bc.struct), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]),
+                  stream_handle(stream))
+        torch.cuda.synchronize(self.device)
+        return out[:, :bc.n].cpu().numpy()
+
+    def drag_force(self, name: str, stream=None) -> np.ndarray:
+        """Force on the BC list `name` by momentum exchange (sum of pre - post halves), from
+        the state after a step: FP64 (3,)."""
+        bc = self.bcs[name]
+        out = torch.zeros(3, dtype=torch.float64, device=self.device)
+        capi.call("vf_DragLiftSum", self.pref(), ctypes.byref(bc.struct), _ptr(out), stream_handle(stream))
+        torch.cuda.synchronize(self.device)
+        return out.cpu().numpy()
+
+    def kinetic_energy(self, stream=None):
+        """KineticEnergyAnalyzer: (sum of 1/2 |u|^2 (1 + drho), node count) over fluid nodes whose
+        six face neighbours and (-1,-1,-1) neighbour are fluid, from the macroscopic fields of the
+        current state."""
+        capi.call("vf_CalculateMacroscopicQuantitiesCompressible", self.pref(), stream_handle(stream))
+        if not hasattr(self, "nbinv"):
+            self.nbinv = _u32(self.grid.neighbor_inverse(), self.device)
+        out = torch.zeros(2, dtype=torch.float64, device=self.device)
+        capi.call("vf_KineticEnergy", self.pref(), _ptr(self.nbinv), _ptr(out), stream_handle(stream))
+        torch.cuda.synchronize(self.device)
+        e, c = out.cpu().numpy()
+        return float(e), int(round(c))
+
     def populations(self) -> np.ndarray:
         """The raw 27*N buffer as FP32 numpy (slot layout of the current parity)."""
         torch.cuda.synchronize(self.device)
