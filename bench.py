@@ -17,7 +17,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import re
 import sys
@@ -28,18 +27,6 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, AMD public spec
 METRIC = "MLUPS + %HBM roofline, D3Q27 K17 256^3; FP16-store and adjoint vs plain FP32"
-
-
-def tgv_fields(grid, u0):
-    import numpy as np
-
-    x, y, z = grid.coords()
-    k = 2 * math.pi / grid.nx
-    ux = u0 * np.sin(k * x) * np.cos(k * y) * np.cos(k * z)
-    uy = -u0 * np.cos(k * x) * np.sin(k * y) * np.cos(k * z)
-    uz = np.zeros_like(ux)
-    drho = 3.0 * (u0 ** 2 / 16.0) * (np.cos(2 * k * x) + np.cos(2 * k * y)) * (np.cos(2 * k * z) + 2.0)
-    return drho, ux, uy, uz
 
 
 def time_steps(L, steps, warmup):
@@ -138,14 +125,13 @@ def cpu_baseline(target_s=12.0):
     import numpy as np
 
     import oracle
-    from virtualfluids_amd.grid import periodic_box
+    from virtualfluids_amd.grid import periodic_box, taylor_green_fields, taylor_green_omega
 
     n = 128
     grid = periodic_box(n, n, n)
-    nu = 0.05 * (n / (2 * math.pi)) / 1600.0
-    omega = 1.0 / (3 * nu + 0.5)
+    omega = taylor_green_omega(n, 0.05, 1600.0)
     L = oracle.Lattice(grid.nbx, grid.nby, grid.nbz)
-    L.init_eq(*tgv_fields(grid, 0.05))
+    L.init_eq(*taylor_green_fields(grid, 0.05))
     even = True
     for _ in range(2):
         oracle.collide_box(L.f, n, n, n, omega, even, limiters=(1e6, 1e6, 1e6))
@@ -191,16 +177,15 @@ def main():
 
     import torch
 
-    from virtualfluids_amd.grid import periodic_box
+    from virtualfluids_amd.grid import periodic_box, taylor_green_fields, taylor_green_omega
     from virtualfluids_amd.lattice import Lattice
 
     n = args.n
     grid = periodic_box(n, n, n)
     N = grid.N
     u0 = 0.05
-    nu = u0 * (n / (2 * math.pi)) / 1600.0
-    omega = 1.0 / (3 * nu + 0.5)
-    fields = tgv_fields(grid, u0)
+    omega = taylor_green_omega(n, u0, 1600.0)
+    fields = taylor_green_fields(grid, u0)
     lim = (1e6, 1e6, 1e6)  # the Taylor-Green app switches the limiter off
 
     results = {}

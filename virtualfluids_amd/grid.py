@@ -223,6 +223,31 @@ def sphere_channel(n: tuple, center: tuple, radius: float, inflow: float, wall_q
     return g
 
 
+def taylor_green_fields(grid: Grid, u0: float):
+    """Taylor-Green vortex on a periodic box: drho and velocity per node, with node coordinates
+    x_i = 2 pi i / nx (k = 2 pi / nx on every axis) and the pressure perturbation of
+    apps/gpu/TaylorGreenVortex/TaylorGreenVortex.cpp:143-147:
+        u = u0 sin(kx) cos(ky) cos(kz), v = -u0 cos(kx) sin(ky) cos(kz), w = 0,
+        drho = 3 (u0^2 / 16) (cos 2kx + cos 2ky) (cos 2kz + 2)."""
+    x, y, z = grid.coords()
+    k = 2 * np.pi / grid.nx
+    sx, cx_ = np.sin(k * x), np.cos(k * x)
+    sy, cy_ = np.sin(k * y), np.cos(k * y)
+    cz_ = np.cos(k * z)
+    ux = u0 * sx * cy_ * cz_
+    uy = -u0 * cx_ * sy * cz_
+    uz = np.zeros_like(ux)
+    drho = 3.0 * (u0 ** 2 / 16.0) * (np.cos(2 * k * x) + np.cos(2 * k * y)) * (np.cos(2 * k * z) + 2.0)
+    return drho, ux, uy, uz
+
+
+def taylor_green_omega(n: int, u0: float = 0.05, re: float = 1600.0) -> float:
+    """Relaxation rate of the Taylor-Green case: nu = u0 (n / 2 pi) / Re
+    (apps/gpu/TaylorGreenVortex/TaylorGreenVortex.cpp:97-100), omega = 1 / (3 nu + 1/2)."""
+    nu = u0 * (n / (2 * np.pi)) / re
+    return 1.0 / (3.0 * nu + 0.5)
+
+
 def gaussian_random_field(shape: tuple, correlation_length: float, seed: int, low: float = 0.0,
                           high: float = 0.1) -> np.ndarray:
     """Seeded Gaussian random field on a periodic (nz, ny, nx) grid: white noise filtered in
