@@ -173,6 +173,50 @@ public:
     const float* pressureDevice() const { return press; }
 };
 
+/* ---- checkpoint / restart (RestartObject, src/gpu/core/Restart/RestartObject.h:45-80) ------ */
+/* The reference's restart file name: fname + "_Restart_" + myID + "_" + step (Simulation.cpp:93-96). */
+std::string getFileName(const std::string& fname, int step, int myID);
+
+/* Population buffers of every level, one float vector each (27 * numberOfNodes values in the
+ * device slot layout; FP16/BF16 levels widened to float and narrowed back with round to
+ * nearest even). serialize downloads and writes (NaN written as 0); deserialize reads and
+ * uploads, after checking the level count and sizes (the reference copies unchecked). Neither
+ * touches isEvenTimestep: the reference's PreProcessor leaves it even before a restart, so
+ * checkpoints written after an even number of coarse steps resume exactly. */
+class RestartObject {
+public:
+    virtual ~RestartObject() = default;
+    void deserialize(const std::string& filename, Parameter& para);
+    void serialize(const std::string& filename, const Parameter& para);
+    void delete_restart_file(const std::string& filename); /* prints a warning when absent */
+    std::vector<std::vector<float>> fs;
+    virtual void serialize_internal(const std::string& filename) = 0;
+    virtual void deserialize_internal(const std::string& filename) = 0;
+
+protected:
+    virtual std::string getFileExtension() const = 0;
+};
+
+/* ".txt": one line per level, values printed with 6 significant digits (lossy, as the reference) */
+class ASCIIRestartObject : public RestartObject {
+public:
+    void serialize_internal(const std::string& filename) override;
+    void deserialize_internal(const std::string& filename) override;
+
+protected:
+    std::string getFileExtension() const override;
+};
+
+/* ".bin": int32 level count, then per level int32 count and the floats (exact) */
+class BinaryRestartObject : public RestartObject {
+public:
+    void serialize_internal(const std::string& filename) override;
+    void deserialize_internal(const std::string& filename) override;
+
+protected:
+    std::string getFileExtension() const override;
+};
+
 /* ---- kernels ---------------------------------------------------------------------------- */
 class Kernel {
 public:

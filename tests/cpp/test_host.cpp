@@ -271,7 +271,21 @@ static std::map<std::string, std::string> read_config(const std::string& path)
     return cfg;
 }
 
-static int run_case(const std::string& dir, unsigned steps)
+// checkpoint / restart actions of a run: after `checkpointAt` steps write `file`, or start
+// from `file` instead of the initial fields
+struct RestartPlan {
+    enum Kind { None, Checkpoint, Restart } kind = None;
+    unsigned checkpointAt = 0;
+    std::string file, format = "bin";
+};
+
+static std::unique_ptr<RestartObject> make_restart(const std::string& format)
+{
+    if (format == "txt") return std::make_unique<ASCIIRestartObject>();
+    return std::make_unique<BinaryRestartObject>();
+}
+
+static int run_case(const std::string& dir, unsigned steps, const RestartPlan& plan = {})
 {
     auto cfg = read_config(dir + "/config.txt");
     const int levels = std::stoi(cfg.at("levels"));
@@ -359,9 +373,16 @@ static int run_case(const std::string& dir, unsigned steps)
         mem[L]->setCells(p->fineToCoarse, p->neighborFineToCoarse, p->hasOffsetsFineToCoarse, fc, fineN);
     }
 
+    if (plan.kind == RestartPlan::Restart) make_restart(plan.format)->deserialize(plan.file, *para);
     KernelFactory kf;
     UpdateGrid27 update(para, kf.makeKernels(para), &bcf, &gsf);
-    update.run(steps);
+    if (plan.kind == RestartPlan::Checkpoint) {
+        update.run(plan.checkpointAt);
+        make_restart(plan.format)->serialize(plan.file, *para);
+        update.run(steps - plan.checkpointAt);
+    } else {
+        update.run(steps);
+    }
     for (int L = 0; L < levels; ++L) {
         auto p = para->getParD(L);
         check(vf_CalculateMacroscopicQuantitiesCompressible(p.get(), p->stream));
@@ -374,6 +395,43 @@ static int run_case(const std::string& dir, unsigned steps)
     }
     std::printf("ran %u coarse steps on %d level(s)\n", steps, levels);
     return 0;
+}
+
+// RestartObjectTests (tests/unit-tests/gpu/core/Restart/RestartObjectTests.cpp:39-88) restated
+template <class T> static void restart_save_and_load(const std::string& dir)
+{
+    T write;
+    write.fs = {{1, 2, 3}, {4, 5, 6}};
+    const std::string name = dir + "/test_do_check_point";
+    write.serialize_internal(name);
+    T read;
+    read.deserialize_internal(name);
+    EXPECT(read.fs == write.fs);
+    write.delete_restart_file(name);
+}
+
+static int restart_tests(const std::string& dir)
+{
+    restart_save_and_load<ASCIIRestartObject>(dir);
+    restart_save_and_load<BinaryRestartObject>(dir);
+    EXPECT(!std::ifstream(dir + "/test_do_check_point.bin") && !std::ifstream(dir + "/test_do_check_point.txt"));
+    BinaryRestartObject b;
+    b.delete_restart_file(dir + "/does_not_exist"); // prints a warning
+    ASCIIRestartObject a;
+    a.delete_restart_file(dir + "/does_not_exist");
+    EXPECT(getFileName("out/cavity", 500, 0) == "out/cavity_Restart_0_500");
+    // files for the comparison with the Python writer: two levels of values over many magnitudes
+    std::vector<std::vector<float>> fs(2);
+    for (int i = 0; i < 27 * 7; ++i) fs[0].push_back(0.0123456789f * std::sin(0.37f * (float)i) * std::pow(10.0f, (float)(i % 9) - 6.0f));
+    for (int i = 0; i < 27 * 3; ++i) fs[1].push_back(-1.0f / (float)(i + 1));
+    fs[1][5] = 0.0f;
+    fs[1][6] = -0.0f;
+    b.fs = fs;
+    b.serialize_internal(dir + "/cpp");
+    a.fs = fs;
+    a.serialize_internal(dir + "/cpp");
+    std::printf("restart: %d failure(s)\n", failures);
+    return failures ? 1 : 0;
 }
 
 // the C structs' field offsets, for the ctypes binding's layout check
@@ -424,14 +482,32 @@ int main(int argc, char** argv)
         std::printf("%s: %d failure(s)\n", mode.c_str(), failures);
         return failures ? 1 : 0;
     }
-    if (mode == "run" && argc == 4) {
+    if (mode == "restart" && argc == 3) return restart_tests(argv[2]);
+    if (mode == "run" && (argc == 4 || argc == 7 || argc == 8)) {
         try {
-            return run_case(argv[2], (unsigned)std::stoul(argv[3]));
+            RestartPlan plan;
+            if (argc > 4) {
+                const std::string action = argv[4];
+                if (action == "checkpoint" && argc == 8) {
+                    plan.kind = RestartPlan::Checkpoint;
+                    plan.checkpointAt = (unsigned)std::stoul(argv[5]);
+                    plan.file = argv[6];
+                    plan.format = argv[7];
+                } else if (action == "restart" && argc == 7) {
+                    plan.kind = RestartPlan::Restart;
+                    plan.file = argv[5];
+                    plan.format = argv[6];
+                } else {
+                    throw std::invalid_argument("unknown restart action " + action);
+                }
+            }
+            return run_case(argv[2], (unsigned)std::stoul(argv[3]), plan);
         } catch (const std::exception& e) {
             std::fprintf(stderr, "run failed: %s\n", e.what());
             return 2;
         }
     }
-    std::fprintf(stderr, "usage: test_host factories | run DIR STEPS\n");
+    std::fprintf(stderr, "usage: test_host factories | layout | restart DIR | run DIR STEPS "
+                         "[checkpoint T FILE bin|txt | restart FILE bin|txt]\n");
     return 64;
 }

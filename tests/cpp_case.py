@@ -67,8 +67,10 @@ def write_case(d, levels, viscosity, limiter=0.01, storage="fp32", velocity_bc=N
             f.write(f"{k}={v}\n")
 
 
-def run_case(d, steps, timeout=120):
-    r = subprocess.run([HOST_TEST, "run", d, str(steps)], capture_output=True, text=True, timeout=timeout)
+def run_case(d, steps, timeout=120, restart=()):
+    """restart: () or ("checkpoint", T, FILE, "bin"|"txt") or ("restart", FILE, "bin"|"txt")."""
+    r = subprocess.run([HOST_TEST, "run", d, str(steps), *map(str, restart)], capture_output=True, text=True,
+                       timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"test_host run failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")
     return r.stdout

@@ -129,3 +129,49 @@ def test_two_levels_through_cpp_update_grid(gpu, tmp_path):
         assert np.array_equal(a[valid], b[valid])
     for a, b in zip(got_f, F.macroscopic()[:4]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_checkpoint_and_restart_through_cpp(gpu, tmp_path, storage):
+    """Two levels: 20 coarse steps with a binary checkpoint after 10, against a second run that
+    starts from that checkpoint and makes the last 10 steps: identical fields. The Python
+    restart reader, loading the C++ file into the Python multi-level driver, continues to the
+    same fields as well."""
+    import shutil
+
+    from helpers import shear_wave_init
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.refinement import GridScalingFactory, nested_box
+    from virtualfluids_amd.restart import BinaryRestartObject
+    from virtualfluids_amd.simulation import MultiLevelSimulation
+
+    nu = 0.01
+    nb = nested_box((12, 12, 24), (2, 2, 6), (9, 9, 18))
+    k, init_c, init_f = shear_wave_init(nb, 0.03)
+    d = str(tmp_path / "case")
+    write_case(d, [(nb.coarse, init_c), (nb.fine, init_f)], nu, storage=storage, interfaces=[(nb.cf, nb.fc)])
+    ckpt = str(tmp_path / "shear_Restart_0_10")
+    run_case(d, 20, restart=("checkpoint", 10, ckpt, "bin"))
+    full = [read_output(d, 0), read_output(d, 1)]
+    assert os.path.exists(ckpt + ".bin")
+    shutil.rmtree(d)
+    write_case(d, [(nb.coarse, init_c), (nb.fine, init_f)], nu, storage=storage, interfaces=[(nb.cf, nb.fc)])
+    run_case(d, 10, restart=("restart", ckpt, "bin"))
+    resumed = [read_output(d, 0), read_output(d, 1)]
+    valid = nb.coarse_valid()
+    for a, b in zip(full[0], resumed[0]):
+        assert np.array_equal(a[valid], b[valid])
+    for a, b in zip(full[1], resumed[1]):
+        assert np.array_equal(a, b)
+
+    C = Lattice(nb.coarse, 1.0 / (3 * nu + 0.5), storage=storage, level=0)
+    F = Lattice(nb.fine, 1.0 / (6 * nu + 0.5), storage=storage, level=1)
+    BinaryRestartObject().deserialize(ckpt, [C, F])
+    fac = GridScalingFactory()
+    fac.set_scaling_factory(GridScalingFactory.GridScaling.ScaleCompressible)
+    MultiLevelSimulation([C, F], [(nb.cf, nb.fc)], scaling_factory=fac).run(10)
+    for a, b in zip(full[0], C.macroscopic()[:4]):
+        assert np.array_equal(a[valid], b[valid])
+    for a, b in zip(full[1], F.macroscopic()[:4]):
+        assert np.array_equal(a, b)
