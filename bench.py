@@ -96,6 +96,72 @@ def time_gradient(L, steps, checkpoint_every):
     return dt
 
 
+def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, omega=1.8):
+    """Channel flow past a sphere at full size (192 x 96 x 96 fluid nodes, sphere of radius 12
+    at (48, 48, 48), velocity inflow 0.04, pressure outlet, no-slip walls, interpolated
+    bounce-back on the sphere) with a seeded Gaussian random porosity field in [0, 0.1]
+    (correlation length 8). Objective: the x drag by momentum exchange averaged over the last
+    `average_last` of `steps` steps; its gradient with respect to the porosity field through
+    k17_steps, checkpointing every `checkpoint_every` steps. Times a forward-only run of the
+    same objective and the forward + backward run."""
+    import numpy as np
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_steps, momentum_exchange, momentum_exchange_index
+    from virtualfluids_amd.grid import gaussian_random_field, sphere_channel
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory
+
+    g = sphere_channel((192, 96, 96), (48, 48, 48), 12.0, 0.04)
+    phi0 = gaussian_random_field((g.nx, g.ny, g.nz), 8.0, seed=20261202, low=0.0, high=0.1)
+    F = BoundaryConditionFactory
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_geometry_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    L = Lattice(g, omega)
+    zeros = np.zeros(g.N)
+    L.init_equilibrium(zeros, zeros, zeros, zeros)
+    idx = {e: [torch.from_numpy(a).cuda() for a in momentum_exchange_index(g, "geometry", e)] for e in (True, False)}
+    spin_up = steps - average_last
+
+    def objective(phi):
+        s = k17_steps(L.f.detach().clone(), L, spin_up, checkpoint_every=checkpoint_every, even=True,
+                      bc_factory=fac, porosity=phi)
+        even = spin_up % 2 == 0
+        J = 0.0
+        for _ in range(average_last):
+            s = k17_steps(s, L, 1, checkpoint_every=1, even=even, bc_factory=fac, porosity=phi)
+            even = not even
+            J = J + momentum_exchange(s, *idx[even])[0] / average_last
+        return J
+
+    phi = torch.from_numpy(phi0.astype(np.float32)).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        J0 = float(objective(phi))
+    torch.cuda.synchronize()
+    t_forward = time.perf_counter() - t0
+    phi.requires_grad_(True)
+    t0 = time.perf_counter()
+    J = objective(phi)
+    J.backward()
+    torch.cuda.synchronize()
+    t_gradient = time.perf_counter() - t0
+    grad = phi.grad
+    out = {"case": "sphere channel 192x96x96 fluid, r 12, inflow 0.04, Gaussian porosity [0, 0.1] (corr. 8), "
+                   f"omega {omega}, {steps} steps, checkpoint every {checkpoint_every}, "
+                   f"drag averaged over the last {average_last}",
+           "nodes": int(g.N), "drag": round(J0, 6), "grad_l2": float(grad.double().norm()),
+           "forward_s": round(t_forward, 3), "gradient_s": round(t_gradient, 3),
+           "gradient_vs_forward": round(t_gradient / t_forward, 3)}
+    del L, phi, J, grad
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth():
     """Device-to-device copy of 2 GiB: (read + write bytes) / time."""
     import torch
@@ -167,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="timed adjoint kernel steps")
+    ap.add_argument("--no-sphere", action="store_true", help="skip the full-size sphere-channel gradient")
     ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
                     help="time one storage type only (for profiling runs)")
     args = ap.parse_args()
@@ -241,6 +308,7 @@ def main():
                                              if "fp32" in results else None)}
                              for k, v in results.items()},
         "adjoint": adjoint,
+        "sphere_channel_gradient": None if (args.no_sphere or args.no_adjoint) else sphere_channel_gradient(),
         "hbm_copy_GBps": round(copy_gbps, 1),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }

@@ -176,3 +176,39 @@ def test_mass_and_momentum_sums_fp64(gpu, orc):
         L.collide()
         L.swap()
     assert abs(L.moment_sums()[0] - m0) < 1e-6 * grid.N * 1e-2
+
+
+@pytest.mark.parametrize("addressing", ["buffer", "global"])
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("dims,solid", [((128, 4, 3), False), ((128, 4, 3), True), ((40, 6, 5), True)])
+def test_out_of_place_step_matches_in_place(gpu, monkeypatch, storage, dims, solid, addressing):
+    """vf_k17_run_box_into (row kernel for rows of whole waves, box kernel otherwise; with and
+    without solid nodes) writes exactly the state the in-place step leaves, and leaves its
+    input untouched."""
+    import ctypes
+
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import stream_handle
+
+    monkeypatch.setenv("VFAMD_ROW_BUFFER", "1" if addressing == "buffer" else "0")
+    grid = periodic_box(*dims)
+    rng = np.random.default_rng(300 + sum(dims) + solid)
+    if solid:
+        grid.geo[rng.random(grid.N) < 0.15] = 2
+        grid.fluid = np.nonzero(grid.geo == 19)[0].astype(np.uint32)
+    f0 = near_equilibrium(rng, grid.N, u_amp=0.04, noise=5e-3)
+    for even in (True, False):
+        L = _lattice(grid, 1.8, storage=storage)
+        assert (L.mask is not None) == solid
+        L.even = even
+        L.set_populations(pack_pre(f0, grid, even))
+        before = L.f.clone()
+        out = torch.full_like(L.f, float("nan"))
+        capi.call("vf_k17_run_box_into", L.pref(), ctypes.c_void_p(out.data_ptr()), stream_handle())
+        torch.cuda.synchronize()
+        assert torch.equal(L.f, before)
+        L.collide()
+        torch.cuda.synchronize()
+        assert torch.equal(out, L.f), f"{storage} {dims} solid={solid} even={even}"

@@ -148,8 +148,10 @@ constexpr int kRowMaxThreads = 256;
 // less than 4 GiB; otherwise each direction takes a 64-bit base and a global access.
 constexpr int kRowSplit = 14;
 
-template <class S, int V, bool EVEN, bool MASK, bool MACRO, bool BUF>
-__global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__ f, BoxArgs a, K17Rates r)
+// INTO: the stepped state goes to `fout` (same addresses, another buffer) and `f` is only read;
+// solid nodes then copy their cells, so `fout` is the complete state the in-place step leaves.
+template <class S, int V, bool EVEN, bool MASK, bool MACRO, bool BUF, bool INTO = false>
+__global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, BoxArgs a, K17Rates r)
 {
     using IO = RowIO<S, V>;
     constexpr bool MIX = Codec<S>::storage == VF_STORAGE_FP16;
@@ -169,11 +171,20 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__
     const uint32_t boff[4] = {rows[0] * (uint32_t)sizeof(S), rows[1] * (uint32_t)sizeof(S),
                               rows[2] * (uint32_t)sizeof(S), rows[3] * (uint32_t)sizeof(S)};
     const uint64_t slotBytes = a.N * sizeof(S);
-    __amdgpu_buffer_rsrc_t rsLo, rsHi;
+    S* const dst = INTO ? fout : f;
+    __amdgpu_buffer_rsrc_t rsLo, rsHi, wsLo, wsHi;
     if constexpr (BUF) {
         rsLo = __builtin_amdgcn_make_buffer_rsrc(f, (short)0, (int)0xffffffff, 0x00020000);
         rsHi = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<char*>(f) + kRowSplit * slotBytes, (short)0,
                                                  (int)0xffffffff, 0x00020000);
+        if constexpr (INTO) {
+            wsLo = __builtin_amdgcn_make_buffer_rsrc(dst, (short)0, (int)0xffffffff, 0x00020000);
+            wsHi = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<char*>(dst) + kRowSplit * slotBytes, (short)0,
+                                                     (int)0xffffffff, 0x00020000);
+        } else {
+            wsLo = rsLo;
+            wsHi = rsHi;
+        }
     }
     // byte offset of direction d's slot from its descriptor's base
     auto soff = [&](int d) -> uint32_t {
@@ -188,9 +199,9 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* __restrict__
         }
     };
     auto store = [&](int d, const float (&vals)[V]) {
-        if constexpr (BUF) IO::store(slot<EVEN>(d) >= kRowSplit ? rsHi : rsLo, boff[node_sel(d) >> 1], soff(d), vals);
+        if constexpr (BUF) IO::store(slot<EVEN>(d) >= kRowSplit ? wsHi : wsLo, boff[node_sel(d) >> 1], soff(d), vals);
         else {
-            char* base = reinterpret_cast<char*>(f) + (uint64_t)slot<EVEN>(d) * slotBytes;
+            char* base = reinterpret_cast<char*>(dst) + (uint64_t)slot<EVEN>(d) * slotBytes;
             IO::store(reinterpret_cast<S*>(base + boff[node_sel(d) >> 1]), vals);
         }
     };
@@ -385,16 +396,22 @@ static size_t row_lds_reserve(int k)
     return bytes > 0 ? (size_t)bytes : 0;
 }
 
+// out: null for the in-place step, else the buffer the out-of-place step writes
 template <class S, int V, bool EVEN, bool BUF>
-static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, hipStream_t s)
+static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, hipStream_t s)
 {
     dim3 grid(a.ny, a.nz);
     dim3 block(a.nx / V);
     const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>());
-    if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true, BUF>), grid, block, lds, s, f, a, r);
-    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF>), grid, block, lds, s, f, a, r);
-    else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true, BUF>), grid, block, lds, s, f, a, r);
-    else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF>), grid, block, lds, s, f, a, r);
+    if (out) {
+        if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, true>), grid, block, lds, s, f, out, a, r);
+        else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF, true>), grid, block, lds, s, f, out, a, r);
+        return;
+    }
+    if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true, BUF>), grid, block, lds, s, f, f, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF>), grid, block, lds, s, f, f, a, r);
+    else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true, BUF>), grid, block, lds, s, f, f, a, r);
+    else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF>), grid, block, lds, s, f, f, a, r);
 }
 
 // VFAMD_ROW_BUFFER=0 forces the global-address form (used by the tests to cover both forms).
@@ -405,13 +422,31 @@ static bool row_buffer_allowed()
 }
 
 template <class S, int V, bool EVEN>
-static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, hipStream_t s)
+static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, hipStream_t s)
 {
     // the larger descriptor spans kRowSplit slots; buffer offsets are 32-bit
     if ((uint64_t)kRowSplit * a.N * sizeof(S) <= 0xffffffffull && row_buffer_allowed())
-        launch_row_b<S, V, EVEN, true>(a, r, mask, macro, f, s);
+        launch_row_b<S, V, EVEN, true>(a, r, mask, macro, f, out, s);
     else
-        launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, s);
+        launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, out, s);
+}
+
+// The row kernel when the box allows it: a row of nx nodes is nx/V lanes, whole waves, at most
+// one workgroup (porous media take the box kernels). Returns false when it does not apply.
+template <class S, bool EVEN>
+static bool try_launch_row(const vf_lbm_parameter& p, const BoxArgs& a, const K17Rates& r, bool macro, S* out,
+                           hipStream_t s)
+{
+    S* f = static_cast<S*>(p.distributions);
+    const bool mask = p.fluidMask != nullptr;
+    const int V = row_width<S>(p.nx);
+    if (p.porosity || V <= 0 || p.nx % (64 * V) != 0 || p.nx / V > kRowMaxThreads) return false;
+    if constexpr (sizeof(S) == 4) {
+        if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, s); return true; }
+    }
+    if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, out, s); return true; }
+    if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, out, s); return true; }
+    return false;
 }
 
 template <class S, bool EVEN>
@@ -422,16 +457,7 @@ static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
     const K17Rates r = make_k17_rates(p);
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
-    // row kernel: a row of nx nodes is nx/V lanes, whole waves, at most one workgroup
-    // (porous media take the box kernel)
-    const int V = row_width<S>(p.nx);
-    if (!p.porosity && V > 0 && p.nx % (64 * V) == 0 && p.nx / V <= kRowMaxThreads) {
-        if constexpr (sizeof(S) == 4) {
-            if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, s); return; }
-        }
-        if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, s); return; }
-        if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, s); return; }
-    }
+    if (try_launch_row<S, EVEN>(p, a, r, macro, nullptr, s)) return;
     const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
     dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
     if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r);
@@ -484,6 +510,7 @@ template <class S, bool EVEN> struct BoxIntoL {
     {
         BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, nullptr, nullptr, nullptr, nullptr, p.porosity};
         const K17Rates r = make_k17_rates(p);
+        if (try_launch_row<S, EVEN>(p, a, r, false, static_cast<S*>(out), s)) return;
         const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
         const dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
         const S* in = static_cast<const S*>(p.distributions);
