@@ -720,6 +720,40 @@ ORC_POSTPROC(float, f32)
 ORC_SCALE(double, f64)
 ORC_SCALE(float, f32)
 
+// Reference adjoint of one interpolation cell by forward-mode dual numbers through the
+// templated restatement above (source moments -> coefficients -> rebuilt nodes): one pass per
+// source population (8 x 27) gives a column of the cell's Jacobian, contracted with the adjoint
+// g of the written populations. mode 0: CF, g[8][27] of the eight fine children at
+// (+-1/4, +-1/4, +-1/4) with eps 1/2 and omegaDst = omega_fine; mode 1: FC, g[27] of the coarse
+// node at the centre with eps 2 and omegaDst = omega_coarse. out[8][27] = J^T g.
+extern "C" void orc_interp_cell_vjp_f64(const double* f8, double omegaSrc, double xo, double yo, double zo, int mode,
+                                        double omegaDst, const double* g, double* out)
+{
+    for (int col = 0; col < 8 * 27; ++col) {
+        SourceMoments<Dual> m[8];
+        for (int n = 0; n < 8; ++n) {
+            Dual f[27];
+            for (int d = 0; d < 27; ++d) f[d] = Dual(f8[n * 27 + d], n * 27 + d == col ? 1.0 : 0.0);
+            m[n] = source_moments(f, Dual(omegaSrc));
+        }
+        const Coefficients<Dual> c = coefficients(m, Dual(xo), Dual(yo), Dual(zo));
+        double s = 0.0;
+        if (mode == 0) {
+            for (int q = 0; q < 8; ++q) {
+                Dual fo[27];
+                interpolate_cf(fo, Dual(omegaDst), Dual(0.5), c, Dual((q & 1) ? 0.25 : -0.25),
+                               Dual((q & 2) ? 0.25 : -0.25), Dual((q & 4) ? 0.25 : -0.25));
+                for (int d = 0; d < 27; ++d) s += fo[d].d * g[q * 27 + d];
+            }
+        } else {
+            Dual fo[27];
+            interpolate_fc(fo, Dual(2.0), Dual(omegaDst), c);
+            for (int d = 0; d < 27; ++d) s += fo[d].d * g[d];
+        }
+        out[col] = s;
+    }
+}
+
 // One interpolation cell from 8 given source distributions (node-major [8][27], n = ix + 2 iy + 4 iz):
 // coefficients (42 values, order of Coefficients<T>) and the rebuilt node (mode 0: coarse -> fine at
 // (x, y, z) with eps and omegaDst = omega_fine; mode 1: fine -> coarse with eps and omegaDst = omega_coarse).

@@ -261,6 +261,18 @@ def interp_cell(f8, omega_src, offset=(0.0, 0.0, 0.0), mode=0, omega_dst=1.0, ep
     return c, out
 
 
+def interp_cell_vjp(f8, omega_src, g, offset=(0.0, 0.0, 0.0), mode=0, omega_dst=1.0):
+    """J^T g of one refinement cell by dual numbers: g (8, 27) of the CF children (mode 0) or
+    (27,) of the FC node (mode 1); returns (8, 27)."""
+    f8 = np.ascontiguousarray(f8, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    out = np.empty((8, 27))
+    D = ctypes.c_double
+    lib().orc_interp_cell_vjp_f64(_p(f8), D(omega_src), D(offset[0]), D(offset[1]), D(offset[2]), ctypes.c_int(mode),
+                                  D(omega_dst), _p(g), _p(out))
+    return out
+
+
 def ref_interp_cell(f8, omega_src, offset=(0.0, 0.0, 0.0), mode=0, omega_dst=1.0, eps=0.5, pos=(0.0, 0.0, 0.0)):
     return interp_cell(f8, omega_src, offset, mode, omega_dst, eps, pos, lib_handle=ref("f64"), fn="ref_interp_cell")
 

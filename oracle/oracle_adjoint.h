@@ -29,6 +29,7 @@ inline Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
 inline Dual operator*(Dual a, const Dual& b) { return a *= b; }
 inline Dual operator/(const Dual& a, const Dual& b) { return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
 inline bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+inline bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
 inline bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
 inline bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
 // |x| with the derivative sign(x); at x = 0 the limiter term x |x| has derivative 0 either way

@@ -5,6 +5,7 @@
 
 #include "vf_bc_adjoint.h"
 #include "vf_k17_adjoint.h"
+#include "vf_scaling.h"
 
 using namespace vfa;
 
@@ -88,4 +89,48 @@ extern "C" void host_k17_vjp_scaled(const float* fin, const float* g, long long 
         k17_vjp(f, gg, r, out, nullptr, vscale[i], sb + i);
         std::memcpy(fb + 27 * i, out, sizeof(out));
     }
+}
+
+// ---- refinement interpolation of one cell (vf_scaling.h) and its reverse -----------------
+// mode 0: CF (out: 8 children x 27), mode 1: FC (out: 27); src: 8 x 27 source populations
+extern "C" void host_interp_cell(const float* src, float omegaSrc, float xo, float yo, float zo, int mode,
+                                 float omegaDst, float* out)
+{
+    SrcMom m[8];
+    for (int n = 0; n < 8; ++n) {
+        float f[27];
+        std::memcpy(f, src + 27 * n, sizeof(f));
+        m[n] = source_moments_of(f, omegaSrc);
+    }
+    const Coef c = coefficients(m, xo, yo, zo);
+    if (mode == 0) {
+        for (int q = 0; q < 8; ++q) {
+            float f[27];
+            rebuild_populations(f, cf_node_moments(c, cf_child_offset(q, 0), cf_child_offset(q, 1),
+                                                   cf_child_offset(q, 2), 0.5f, omegaDst));
+            std::memcpy(out + 27 * q, f, sizeof(f));
+        }
+    } else {
+        float f[27];
+        rebuild_populations(f, fc_node_moments(c, omegaDst));
+        std::memcpy(out, f, sizeof(f));
+    }
+}
+
+// J^T g for one cell: g is 8 x 27 (CF) or 27 (FC); out 8 x 27
+extern "C" void host_interp_cell_vjp(const float* src, float omegaSrc, float xo, float yo, float zo, int mode,
+                                     float omegaDst, const float* g, float* out)
+{
+    float s[8][27], sb[8][27];
+    std::memcpy(s, src, sizeof(s));
+    if (mode == 0) {
+        float gg[8][27];
+        std::memcpy(gg, g, sizeof(gg));
+        cf_cell_vjp(s, omegaSrc, omegaDst, xo, yo, zo, gg, sb);
+    } else {
+        float gg[27];
+        std::memcpy(gg, g, sizeof(gg));
+        fc_cell_vjp(s, omegaSrc, omegaDst, xo, yo, zo, gg, sb);
+    }
+    std::memcpy(out, sb, sizeof(sb));
 }

@@ -1,0 +1,100 @@
+"""Reverse mode of the refinement interpolation (CPU).
+
+The interpolation of one cell (vf_scaling.h: source moments -> coefficients -> rebuilt nodes)
+and its hand-written vector-Jacobian product, compiled for the host, against the oracle: the
+forward against the oracle's restatement (itself pinned to the reference build), the reverse
+against forward-mode dual numbers through that restatement, and the dual numbers against FP64
+central differences of the oracle's forward. Near-equilibrium random cells, with and without
+wall offsets (the offsets switch the LaplaceRho term off), several relaxation rates.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from helpers import near_equilibrium
+
+HOST_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "libk17host.so")
+F = ctypes.c_float
+
+
+def _host():
+    h = ctypes.CDLL(HOST_LIB)
+    return h
+
+
+def _cell(rng, scale=1.0):
+    return near_equilibrium(rng, 8, u_amp=0.04 * scale, noise=3e-3)
+
+
+def _host_forward(h, f8, ws, off, mode, wd):
+    out = np.empty((8, 27) if mode == 0 else (27,), np.float32)
+    src = np.ascontiguousarray(f8, np.float32)
+    h.host_interp_cell(src.ctypes.data_as(ctypes.c_void_p), F(ws), F(off[0]), F(off[1]), F(off[2]), ctypes.c_int(mode),
+                       F(wd), out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def _host_vjp(h, f8, ws, off, mode, wd, g):
+    out = np.empty((8, 27), np.float32)
+    src = np.ascontiguousarray(f8, np.float32)
+    gg = np.ascontiguousarray(g, np.float32)
+    h.host_interp_cell_vjp(src.ctypes.data_as(ctypes.c_void_p), F(ws), F(off[0]), F(off[1]), F(off[2]),
+                           ctypes.c_int(mode), F(wd), gg.ctypes.data_as(ctypes.c_void_p),
+                           out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def _oracle_forward(orc, f8, ws, off, mode, wd):
+    if mode == 1:
+        return orc.interp_cell(f8, ws, off, 1, wd, 2.0)[1]
+    return np.stack([orc.interp_cell(f8, ws, off, 0, wd, 0.5, pos=tuple(0.25 if (q >> a) & 1 else -0.25
+                                                                        for a in range(3)))[1] for q in range(8)])
+
+
+CASES = [(0, 1.6, 1.8, (0.0, 0.0, 0.0)), (0, 1.3, 1.7, (0.1, -0.2, 0.05)), (1, 1.8, 1.6, (0.0, 0.0, 0.0)),
+         (1, 1.7, 1.4, (-0.15, 0.0, 0.2))]
+
+
+@pytest.mark.parametrize("mode,ws,wd,off", CASES)
+def test_host_forward_matches_oracle(orc, mode, ws, wd, off):
+    h = _host()
+    rng = np.random.default_rng(1000 + mode)
+    for _ in range(5):
+        f8 = _cell(rng)
+        f8 = f8.astype(np.float32).astype(np.float64)
+        got = _host_forward(h, f8, ws, off, mode, wd)
+        want = _oracle_forward(orc, f8, ws, off, mode, wd)
+        assert np.abs(got - want).max() < 2e-7
+
+
+@pytest.mark.parametrize("mode,ws,wd,off", CASES)
+def test_dual_reference_matches_central_differences(orc, mode, ws, wd, off):
+    rng = np.random.default_rng(1100 + mode)
+    f8 = _cell(rng)
+    g = rng.normal(size=(8, 27) if mode == 0 else (27,))
+    vjp = orc.interp_cell_vjp(f8, ws, g, off, mode, wd)
+    h = 1e-6
+    for _ in range(3):
+        v = rng.normal(size=(8, 27))
+        fd = ((_oracle_forward(orc, f8 + h * v, ws, off, mode, wd) - _oracle_forward(orc, f8 - h * v, ws, off, mode, wd))
+              * g).sum() / (2 * h)
+        ad = (vjp * v).sum()
+        assert abs(ad - fd) < 1e-7 * max(1.0, abs(fd)), (ad, fd)
+
+
+@pytest.mark.parametrize("mode,ws,wd,off", CASES)
+def test_host_reverse_matches_dual_numbers(orc, mode, ws, wd, off):
+    h = _host()
+    rng = np.random.default_rng(1200 + mode)
+    worst = 0.0
+    for _ in range(6):
+        f8 = _cell(rng).astype(np.float32).astype(np.float64)
+        g = rng.normal(size=(8, 27) if mode == 0 else (27,)).astype(np.float32).astype(np.float64)
+        want = orc.interp_cell_vjp(f8, ws, g, off, mode, wd)
+        got = _host_vjp(h, f8, ws, off, mode, wd, g)
+        err = np.abs(got - want).max() / np.abs(want).max()
+        worst = max(worst, err)
+    print(f"mode {mode} omega {ws}->{wd} offset {off}: max rel err {worst:.2e}")
+    assert worst < 1e-5

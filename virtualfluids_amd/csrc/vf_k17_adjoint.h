@@ -484,6 +484,70 @@ VF_HD void mid_reverse(const Mid& s, const Macro& mc, const K17Rates& r, const f
     }
 }
 
+// backward chimera x, y, z on moments M (idx order), giving populations in idx order
+VF_HD void backward_chimera(float (&M)[27], float vx, float vy, float vz)
+{
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) line_bwd(M[idx(0, j, k)], M[idx(1, j, k)], M[idx(2, j, k)], vx, kv(j) * kv(k));
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) line_bwd(M[idx(i, 0, k)], M[idx(i, 1, k)], M[idx(i, 2, k)], vy, kw(i) * kv(k));
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) line_bwd(M[idx(i, j, 0)], M[idx(i, j, 1)], M[idx(i, j, 2)], vz, kw(i) * kw(j));
+}
+
+// its reverse: N the moments entering the passes; A holds the adjoints of the outputs on entry
+// and the adjoints of N on return; the velocity terms are added to vxb, vyb, vzb
+VF_HD void backward_chimera_rev(const float (&N)[27], float vx, float vy, float vz, float (&A)[27], float& vxb,
+                                float& vyb, float& vzb)
+{
+    {
+        float Y[27];  // input of the z pass = N after the x and y passes
+#pragma unroll
+        for (int i = 0; i < 27; ++i) Y[i] = N[i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) line_bwd(Y[idx(0, j, k)], Y[idx(1, j, k)], Y[idx(2, j, k)], vx, kv(j) * kv(k));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) line_bwd(Y[idx(i, 0, k)], Y[idx(i, 1, k)], Y[idx(i, 2, k)], vy, kw(i) * kv(k));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                line_bwd_rev(Y[idx(i, j, 0)], Y[idx(i, j, 1)], vz, kw(i) * kw(j), A[idx(i, j, 0)], A[idx(i, j, 1)],
+                             A[idx(i, j, 2)], vzb);
+    }
+    {
+        float X[27];  // input of the y pass = N after the x pass
+#pragma unroll
+        for (int i = 0; i < 27; ++i) X[i] = N[i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) line_bwd(X[idx(0, j, k)], X[idx(1, j, k)], X[idx(2, j, k)], vx, kv(j) * kv(k));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                line_bwd_rev(X[idx(i, 0, k)], X[idx(i, 1, k)], vy, kw(i) * kv(k), A[idx(i, 0, k)], A[idx(i, 1, k)],
+                             A[idx(i, 2, k)], vyb);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            line_bwd_rev(N[idx(0, j, k)], N[idx(1, j, k)], vx, kv(j) * kv(k), A[idx(0, j, k)], A[idx(1, j, k)],
+                         A[idx(2, j, k)], vxb);
+}
+
 } // namespace adj
 
 // Reverse-mode of k17_collide for one node. f: shifted pre-collision populations (direction
@@ -516,46 +580,7 @@ VF_HD void k17_vjp(const float (&f)[27], const float (&g)[27], const K17Rates& r
     float A[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) A[i] = g[dir_of_idx(i)];
-    {
-        float Y[27];  // input of the z pass = N after the x and y passes
-#pragma unroll
-        for (int i = 0; i < 27; ++i) Y[i] = N[i];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) line_bwd(Y[idx(0, j, k)], Y[idx(1, j, k)], Y[idx(2, j, k)], mc.vx, kv(j) * kv(k));
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) line_bwd(Y[idx(i, 0, k)], Y[idx(i, 1, k)], Y[idx(i, 2, k)], mc.vy, kw(i) * kv(k));
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                line_bwd_rev(Y[idx(i, j, 0)], Y[idx(i, j, 1)], mc.vz, kw(i) * kw(j), A[idx(i, j, 0)], A[idx(i, j, 1)],
-                             A[idx(i, j, 2)], vzb);
-    }
-    {
-        float X[27];  // input of the y pass = N after the x pass
-#pragma unroll
-        for (int i = 0; i < 27; ++i) X[i] = N[i];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) line_bwd(X[idx(0, j, k)], X[idx(1, j, k)], X[idx(2, j, k)], mc.vx, kv(j) * kv(k));
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                line_bwd_rev(X[idx(i, 0, k)], X[idx(i, 1, k)], mc.vy, kw(i) * kv(k), A[idx(i, 0, k)], A[idx(i, 1, k)],
-                             A[idx(i, 2, k)], vyb);
-    }
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            line_bwd_rev(N[idx(0, j, k)], N[idx(1, j, k)], mc.vx, kv(j) * kv(k), A[idx(0, j, k)], A[idx(1, j, k)],
-                         A[idx(2, j, k)], vxb);
+    backward_chimera_rev(N, mc.vx, mc.vy, mc.vz, A, vxb, vyb, vzb);
 
     // ---- reverse of the cumulant stage ---------------------------------------------------
     float mb[27];
