@@ -287,6 +287,24 @@ int vf_ConservationSums(const vf_lbm_parameter* para, double* out4, void* stream
  * over every fluid node; vf_KineticEnergy applies the reference's node selection. */
 int vf_MomentSums(const vf_lbm_parameter* para, double* out5, void* stream);
 
+/* Adjoint of vf_ScaleCoarseToFineCompressible (no counterpart in the reference: the reverse mode
+ * of the refinement interface for gradients through two levels). coarseAdjoint / fineAdjoint:
+ * FP32 adjoint buffers with the population layout of each level. The adjoint of the fine cells
+ * the forward call wrote is read and cleared; its transpose through the interpolation is added
+ * to the coarse source nodes' cells (coarse parity). Linearisation point: the coarse state the
+ * forward call read (same levels, parities and cells). */
+int vf_ScaleCoarseToFineCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                            const vf_interpolation_cells* coarseToFine,
+                                            const vf_interpolation_cell_neighbor* offsets, float* coarseAdjoint,
+                                            float* fineAdjoint, void* stream);
+
+/* Adjoint of vf_ScaleFineToCoarseCompressible: the adjoint of the coarse cells it wrote is read
+ * and cleared, its transpose added to the fine source nodes' cells (fine parity even). */
+int vf_ScaleFineToCoarseCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                            const vf_interpolation_cells* fineToCoarse,
+                                            const vf_interpolation_cell_neighbor* offsets, float* coarseAdjoint,
+                                            float* fineAdjoint, void* stream);
+
 /* DragLiftPostD27 (src/gpu/core/PostProcessor/DragLift.cu:558-566, kernel DragLift.cu:50-302):
  * per BC node i, the post-collision half of the momentum exchange over the links e with
  * q27[e*n + i] in [0, 1]: drag{X,Y,Z}[i] = -sum_e c_e f_e^post (doubles, device arrays of

@@ -726,8 +726,8 @@ ORC_SCALE(float, f32)
 // g of the written populations. mode 0: CF, g[8][27] of the eight fine children at
 // (+-1/4, +-1/4, +-1/4) with eps 1/2 and omegaDst = omega_fine; mode 1: FC, g[27] of the coarse
 // node at the centre with eps 2 and omegaDst = omega_coarse. out[8][27] = J^T g.
-extern "C" void orc_interp_cell_vjp_f64(const double* f8, double omegaSrc, double xo, double yo, double zo, int mode,
-                                        double omegaDst, const double* g, double* out)
+static void interp_cell_vjp(const double* f8, double omegaSrc, double xo, double yo, double zo, int mode,
+                            double omegaDst, const double* g, double* out)
 {
     for (int col = 0; col < 8 * 27; ++col) {
         SourceMoments<Dual> m[8];
@@ -751,6 +751,66 @@ extern "C" void orc_interp_cell_vjp_f64(const double* f8, double omegaSrc, doubl
             for (int d = 0; d < 27; ++d) s += fo[d].d * g[d];
         }
         out[col] = s;
+    }
+}
+
+extern "C" void orc_interp_cell_vjp_f64(const double* f8, double omegaSrc, double xo, double yo, double zo, int mode,
+                                        double omegaDst, const double* g, double* out)
+{
+    interp_cell_vjp(f8, omegaSrc, xo, yo, zo, mode, omegaDst, g, out);
+}
+
+// Reference adjoint of scale_cf (fine_to_coarse 0) or scale_fc (1) over a lattice pair: per cell,
+// the adjoint of the written cells is read and cleared and J^T of it (dual numbers) added to the
+// source nodes' cells. Cells in order, one at a time.
+extern "C" void orc_scale_adjoint_f64(int fine_to_coarse, double* fc, unsigned long long Nc, const uint32_t* nbxc,
+                                      const uint32_t* nbyc, const uint32_t* nbzc, double* ff, unsigned long long Nf,
+                                      const uint32_t* nbxf, const uint32_t* nbyf, const uint32_t* nbzf, int evenC,
+                                      const uint32_t* cellsC, const uint32_t* cellsF, unsigned long long n,
+                                      const double* ox, const double* oy, const double* oz, double omegaC,
+                                      double omegaF, double* lamC, double* lamF)
+{
+    Lattice<double> C{fc, Nc, nbxc, nbyc, nbzc}, F{ff, Nf, nbxf, nbyf, nbzf};
+    Lattice<double> LC{lamC, Nc, nbxc, nbyc, nbzc}, LF{lamF, Nf, nbxf, nbyf, nbzf};
+    const bool cf = fine_to_coarse == 0;
+    const Lattice<double>& src = cf ? C : F;
+    const Lattice<double>& lsrc = cf ? LC : LF;
+    const bool evenSrc = cf ? evenC != 0 : true;
+    double *P[27], *PL[27], *PW[27];
+    slot_table(src, evenSrc, P);
+    slot_table(lsrc, evenSrc, PL);
+    slot_table(cf ? LF : LC, cf ? true : evenC != 0, PW);
+    for (unsigned long long i = 0; i < n; ++i) {
+        u32 corners[8];
+        cell_corner_nodes(src, cf ? cellsC[i] : cellsF[i], corners);
+        std::vector<double> f8(8 * 27), g(cf ? 8 * 27 : 27), out(8 * 27);
+        for (int q = 0; q < 8; ++q) {
+            u32 nn[8];
+            cell_nodes(src, corners[q], nn);
+            read_pre(P, nn, f8.data() + 27 * q);
+        }
+        if (cf) {
+            u32 fine[8];
+            cell_corner_nodes(LF, cellsF[i], fine);
+            for (int q = 0; q < 8; ++q) {
+                u32 nn[8];
+                cell_nodes(LF, fine[q], nn);
+                read_pre(PW, nn, g.data() + 27 * q);
+                for (int d = 0; d < 27; ++d) write_pre(PW, nn, d, 0.0);
+            }
+        } else {
+            u32 nn[8];
+            cell_nodes(LC, cellsC[i], nn);
+            read_pre(PW, nn, g.data());
+            for (int d = 0; d < 27; ++d) write_pre(PW, nn, d, 0.0);
+        }
+        interp_cell_vjp(f8.data(), cf ? omegaC : omegaF, ox ? ox[i] : 0.0, oy ? oy[i] : 0.0, oz ? oz[i] : 0.0, cf ? 0 : 1,
+                        cf ? omegaF : omegaC, g.data(), out.data());
+        for (int q = 0; q < 8; ++q) {
+            u32 nn[8];
+            cell_nodes(lsrc, corners[q], nn);
+            for (int d = 0; d < 27; ++d) PL[d][nn[node_sel(d)]] += out[27 * q + d];
+        }
     }
 }
 

@@ -273,6 +273,24 @@ def interp_cell_vjp(f8, omega_src, g, offset=(0.0, 0.0, 0.0), mode=0, omega_dst=
     return out
 
 
+def scale_adjoint(fine_to_coarse, coarse, fine, even_coarse, cells_coarse, cells_fine, offsets, omega_c, omega_f,
+                  lam_coarse, lam_fine):
+    """Reference adjoint of `scale` (dual numbers per cell), in place on the FP64 adjoint arrays
+    lam_coarse / lam_fine (population layout of each level)."""
+    cc = np.ascontiguousarray(cells_coarse, dtype=np.uint32)
+    cf = np.ascontiguousarray(cells_fine, dtype=np.uint32)
+    n = cc.size
+    ox = oy = oz = None
+    if offsets is not None:
+        ox, oy, oz = (np.ascontiguousarray(o, dtype=np.float64) for o in offsets)
+    for a in (coarse.f, fine.f, lam_coarse, lam_fine):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    lib().orc_scale_adjoint_f64(ctypes.c_int(int(fine_to_coarse)), *coarse._nb(), *fine._nb(),
+                                ctypes.c_int(int(even_coarse)), _p(cc), _p(cf), ctypes.c_ulonglong(n), _p(ox), _p(oy),
+                                _p(oz), ctypes.c_double(omega_c), ctypes.c_double(omega_f), _p(lam_coarse),
+                                _p(lam_fine))
+
+
 def ref_interp_cell(f8, omega_src, offset=(0.0, 0.0, 0.0), mode=0, omega_dst=1.0, eps=0.5, pos=(0.0, 0.0, 0.0)):
     return interp_cell(f8, omega_src, offset, mode, omega_dst, eps, pos, lib_handle=ref("f64"), fn="ref_interp_cell")
 

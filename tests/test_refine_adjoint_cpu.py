@@ -98,3 +98,51 @@ def test_host_reverse_matches_dual_numbers(orc, mode, ws, wd, off):
         worst = max(worst, err)
     print(f"mode {mode} omega {ws}->{wd} offset {off}: max rel err {worst:.2e}")
     assert worst < 1e-5
+
+
+@pytest.mark.parametrize("fine_to_coarse", [False, True])
+@pytest.mark.parametrize("even_coarse", [True, False])
+@pytest.mark.parametrize("offsets", [False, True])
+def test_lattice_scale_adjoint_dot_product(orc, fine_to_coarse, even_coarse, offsets):
+    """The reference adjoint of a whole interface call (written cells cleared, transposed
+    interpolation added to the source cells) against FP64 central differences of the oracle's
+    forward call: <lam, T(x + h v) - T(x - h v)> / 2h = <T^T lam, v> over both levels."""
+    from helpers import pack_pre
+    from virtualfluids_amd.refinement import nested_box
+
+    nb = nested_box((12, 12, 12), (2, 3, 2), (8, 9, 9))
+    cells = nb.fc if fine_to_coarse else nb.cf
+    rng = np.random.default_rng(1300 + 2 * fine_to_coarse + even_coarse + 4 * offsets)
+    C0 = pack_pre(near_equilibrium(rng, nb.coarse.N, u_amp=0.04, noise=3e-3), nb.coarse, even_coarse)
+    F0 = pack_pre(near_equilibrium(rng, nb.fine.N, u_amp=0.04, noise=3e-3), nb.fine, True)
+    off = [rng.uniform(-0.3, 0.3, cells.n) for _ in range(3)] if offsets else [np.zeros(cells.n)] * 3
+    wc, wf = 1.7, 1.5
+
+    def forward(c, f):
+        C = orc.Lattice(nb.coarse.nbx, nb.coarse.nby, nb.coarse.nbz)
+        F = orc.Lattice(nb.fine.nbx, nb.fine.nby, nb.fine.nbz)
+        C.f[:] = c
+        F.f[:] = f
+        C.even = even_coarse
+        orc.scale(fine_to_coarse, C, F, even_coarse, cells.coarse, cells.fine, off, wc, wf)
+        return C.f.copy(), F.f.copy()
+
+    lc = rng.normal(size=C0.size)
+    lf = rng.normal(size=F0.size)
+    C = orc.Lattice(nb.coarse.nbx, nb.coarse.nby, nb.coarse.nbz)
+    F = orc.Lattice(nb.fine.nbx, nb.fine.nby, nb.fine.nbz)
+    C.f[:] = C0
+    F.f[:] = F0
+    ac, af = lc.copy(), lf.copy()
+    orc.scale_adjoint(fine_to_coarse, C, F, even_coarse, cells.coarse, cells.fine, off if offsets else None, wc, wf,
+                      ac, af)
+    h = 1e-7  # central-difference truncation ~1e-9 absolute here (it scales as h^2)
+    for _ in range(2):
+        vc = rng.normal(size=C0.size)
+        vf = rng.normal(size=F0.size)
+        cp, fp = forward(C0 + h * vc, F0 + h * vf)
+        cm, fm = forward(C0 - h * vc, F0 - h * vf)
+        fd = ((cp - cm) @ lc + (fp - fm) @ lf) / (2 * h)
+        ad = ac @ vc + af @ vf
+        print(f"f2c={fine_to_coarse} even={even_coarse} off={offsets}: adjoint {ad:.10e} FD {fd:.10e}")
+        assert abs(ad - fd) < 1e-7 * abs(fd) + 1e-8

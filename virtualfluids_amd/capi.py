@@ -111,6 +111,8 @@ SIGNATURES = {
     "vf_CalculateMacroscopicQuantitiesCompressible": [_P, _vp],
     "vf_ConservationSums": [_P, _vp, _vp],
     "vf_MomentSums": [_P, _vp, _vp],
+    "vf_ScaleCoarseToFineCompressibleAdjoint": [_P, _P, _vp, _vp, _vp, _vp, _vp],
+    "vf_ScaleFineToCoarseCompressibleAdjoint": [_P, _P, _vp, _vp, _vp, _vp, _vp],
     "vf_DragLiftPostD27": [_P, _Q, _vp, _vp, _vp, _vp],
     "vf_DragLiftPreD27": [_P, _Q, _vp, _vp, _vp, _vp],
     "vf_DragLiftSum": [_P, _Q, _vp, _vp],

@@ -109,6 +109,99 @@ __global__ __launch_bounds__(64) void scale_fc_kernel(S* __restrict__ fc, const 
     for (int d = 0; d < 27; ++d) write_pre(fc, ac, nn, d, evenC != 0, f[d]);
 }
 
+// ---- reverse mode ------------------------------------------------------------------------
+// The adjoint buffers have the population layout of their level (FP32). The cells a forward
+// kernel wrote are read and cleared (their old content did not reach the output); the
+// transposed interpolation of their adjoint is added to the source nodes' cells, which
+// neighbouring interface cells share (atomic adds). Linearisation point: the source state the
+// forward kernel read. Per cell the chain is recomputed from that state (nothing is stored).
+
+__device__ __forceinline__ float* adj_cell(float* lam, const Addr& a, const uint32_t (&n)[8], int d, bool even)
+{
+    return slot_ptr(lam, a.N, d, even) + n[node_sel(d)];
+}
+
+template <class S>
+__global__ __launch_bounds__(64) void scale_cf_adjoint_kernel(const S* __restrict__ fc, float* __restrict__ lamC,
+                                                              float* __restrict__ lamF, Addr ac, Addr af, ScaleArgs s,
+                                                              int evenC)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    SrcMom m[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) m[n] = source_moments(fc, ac, corner(ac, s.cellsC[i], n), evenC != 0, s.omegaC);
+    const float xo = s.ox ? s.ox[i] : 0.f, yo = s.oy ? s.oy[i] : 0.f, zo = s.oz ? s.oz[i] : 0.f;
+    const Coef c = coefficients(m, xo, yo, zo);
+    Coef cb = coef_zero();
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        uint32_t nn[8];
+        cell_nodes(af, corner(af, s.cellsF[i], q), nn);
+        float g[27];
+#pragma unroll
+        for (int d = 0; d < 27; ++d) {
+            float* w = adj_cell(lamF, af, nn, d, true);
+            g[d] = *w;
+            *w = 0.f;
+        }
+        const float x = cf_child_offset(q, 0), y = cf_child_offset(q, 1), z = cf_child_offset(q, 2);
+        const NodeMoments nb = rebuild_vjp(cf_node_moments(c, x, y, z, 0.5f, s.omegaF), g);
+        cf_node_moments_vjp(c, x, y, z, 0.5f, s.omegaF, nb, cb);
+    }
+    SrcMom mb[8];
+    coefficients_vjp(m, xo, yo, zo, cb, mb);
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const uint32_t k = corner(ac, s.cellsC[i], n);
+        float f[27], fb[27];
+        node_populations(fc, ac, k, evenC != 0, f);
+        source_moments_vjp(f, s.omegaC, mb[n], fb);
+        uint32_t nn[8];
+        cell_nodes(ac, k, nn);
+#pragma unroll
+        for (int d = 0; d < 27; ++d) atomicAdd(adj_cell(lamC, ac, nn, d, evenC != 0), fb[d]);
+    }
+}
+
+template <class S>
+__global__ __launch_bounds__(64) void scale_fc_adjoint_kernel(const S* __restrict__ ff, float* __restrict__ lamC,
+                                                              float* __restrict__ lamF, Addr ac, Addr af, ScaleArgs s,
+                                                              int evenC)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.n) return;
+    SrcMom m[8];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) m[n] = source_moments(ff, af, corner(af, s.cellsF[i], n), true, s.omegaF);
+    const float xo = s.ox ? s.ox[i] : 0.f, yo = s.oy ? s.oy[i] : 0.f, zo = s.oz ? s.oz[i] : 0.f;
+    const Coef c = coefficients(m, xo, yo, zo);
+    uint32_t nc[8];
+    cell_nodes(ac, s.cellsC[i], nc);
+    float g[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        float* w = adj_cell(lamC, ac, nc, d, evenC != 0);
+        g[d] = *w;
+        *w = 0.f;
+    }
+    Coef cb = coef_zero();
+    fc_node_moments_vjp(c, s.omegaC, rebuild_vjp(fc_node_moments(c, s.omegaC), g), cb);
+    SrcMom mb[8];
+    coefficients_vjp(m, xo, yo, zo, cb, mb);
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const uint32_t k = corner(af, s.cellsF[i], n);
+        float f[27], fb[27];
+        node_populations(ff, af, k, true, f);
+        source_moments_vjp(f, s.omegaF, mb[n], fb);
+        uint32_t nn[8];
+        cell_nodes(af, k, nn);
+#pragma unroll
+        for (int d = 0; d < 27; ++d) atomicAdd(adj_cell(lamF, af, nn, d, true), fb[d]);
+    }
+}
+
 static int check_level(const vf_lbm_parameter* p)
 {
     if (!p || !p->distributions || p->numberOfNodes == 0 || p->numberOfNodes >= (1ull << 30)) return 0;
@@ -116,9 +209,16 @@ static int check_level(const vf_lbm_parameter* p)
     return 1;
 }
 
+// lamC/lamF null: the forward kernels; otherwise their adjoints
 static int launch_scale(bool fine_to_coarse, const vf_lbm_parameter* c, const vf_lbm_parameter* f,
-                        const vf_interpolation_cells* cells, const vf_interpolation_cell_neighbor* off, void* stream)
+                        const vf_interpolation_cells* cells, const vf_interpolation_cell_neighbor* off, void* stream,
+                        float* lamC = nullptr, float* lamF = nullptr)
 {
+    const bool adjoint = lamC || lamF;
+    if (adjoint && (!lamC || !lamF)) {
+        set_last_error("grid scaling adjoint needs the adjoint buffers of both levels");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
     if (!check_level(c) || !check_level(f) || !cells) {
         set_last_error("grid scaling needs two valid levels and the interface cells");
         return VF_ERR_INVALID_ARGUMENT;
@@ -142,7 +242,13 @@ static int launch_scale(bool fine_to_coarse, const vf_lbm_parameter* c, const vf
     const dim3 grid((cells->numberOfCells + 63) / 64), block(64);
     hipStream_t st = (hipStream_t)stream;
 #define VF_LAUNCH(S)                                                                                                 \
-    if (fine_to_coarse)                                                                                              \
+    if (adjoint && fine_to_coarse)                                                                                   \
+        hipLaunchKernelGGL(scale_fc_adjoint_kernel<S>, grid, block, 0, st, (const S*)f->distributions, lamC, lamF,  \
+                           ac, af, s, c->isEvenTimestep);                                                            \
+    else if (adjoint)                                                                                                \
+        hipLaunchKernelGGL(scale_cf_adjoint_kernel<S>, grid, block, 0, st, (const S*)c->distributions, lamC, lamF,  \
+                           ac, af, s, c->isEvenTimestep);                                                            \
+    else if (fine_to_coarse)                                                                                         \
         hipLaunchKernelGGL(scale_fc_kernel<S>, grid, block, 0, st, (S*)c->distributions, (const S*)f->distributions, \
                            ac, af, s, c->isEvenTimestep);                                                            \
     else                                                                                                             \
@@ -155,6 +261,7 @@ static int launch_scale(bool fine_to_coarse, const vf_lbm_parameter* c, const vf
     default: set_last_error("unknown storage type"); return VF_ERR_INVALID_ARGUMENT;
     }
 #undef VF_LAUNCH
+    if (adjoint) return check_launch(fine_to_coarse ? "scaleFineToCoarseCompressible adjoint" : "scaleCoarseToFineCompressible adjoint");
     return check_launch(fine_to_coarse ? "scaleFineToCoarseCompressible_Device execution failed"
                                        : "scaleCoarseToFineCompressible_Device execution failed");
 }
@@ -175,4 +282,20 @@ extern "C" int vf_ScaleFineToCoarseCompressible(const vf_lbm_parameter* coarse, 
                                                 const vf_interpolation_cell_neighbor* offsets, void* stream)
 {
     return launch_scale(true, coarse, fine, fineToCoarse, offsets, stream);
+}
+
+extern "C" int vf_ScaleCoarseToFineCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                                       const vf_interpolation_cells* coarseToFine,
+                                                       const vf_interpolation_cell_neighbor* offsets,
+                                                       float* coarseAdjoint, float* fineAdjoint, void* stream)
+{
+    return launch_scale(false, coarse, fine, coarseToFine, offsets, stream, coarseAdjoint, fineAdjoint);
+}
+
+extern "C" int vf_ScaleFineToCoarseCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                                       const vf_interpolation_cells* fineToCoarse,
+                                                       const vf_interpolation_cell_neighbor* offsets,
+                                                       float* coarseAdjoint, float* fineAdjoint, void* stream)
+{
+    return launch_scale(true, coarse, fine, fineToCoarse, offsets, stream, coarseAdjoint, fineAdjoint);
 }
