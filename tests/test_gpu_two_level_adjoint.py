@@ -1,0 +1,133 @@
+"""Gradients through two refinement levels on the GPU (k17_two_level_steps).
+
+A channel (velocity inflow, pressure outlet, interpolated no-slip walls) with a nested fine box
+and a random porosity field on the fine level, started from a uniform flow. Objective: the
+kinetic energy of a coarse plane downstream of the fine box, averaged over the last coarse
+steps. Its gradients with respect
+to the inflow velocities and the fine porosity match FP64 central differences of the oracle's
+two-level run (the north star's 1e-4 bound); they do not depend on the checkpoint interval.
+"""
+import numpy as np
+import pytest
+
+from helpers import ORACLE_POST_BC
+from virtualfluids_amd.grid import channel, gaussian_random_field
+from virtualfluids_amd.refinement import nested_box
+
+pytestmark = pytest.mark.gpu
+
+NU = 0.01
+OMEGA_C, OMEGA_F = 1.0 / (3 * NU + 0.5), 1.0 / (6 * NU + 0.5)
+SPIN, AVG = 14, 4
+
+
+def _case():
+    coarse = channel((20, 12, 12), inflow=0.03)
+    nb = nested_box(None, (6, 3, 3), (11, 9, 9), coarse=coarse)
+    x, y, z = nb.coarse.coords()
+    plane = np.nonzero((x == 15) & (nb.coarse.geo == 19))[0]  # downstream of the fine box
+    phi = gaussian_random_field((nb.fine.nx, nb.fine.ny, nb.fine.nz), 2.0, seed=1500, low=0.0, high=0.05)
+    return nb, plane, phi.astype(np.float32).astype(np.float64)
+
+
+def _oracle_objective(orc, nb, plane, f0c, f0f, vx, phi):
+    from virtualfluids_amd.adjoint import pre_collision_index
+
+    C = orc.Lattice(nb.coarse.nbx, nb.coarse.nby, nb.coarse.nbz)
+    F = orc.Lattice(nb.fine.nbx, nb.fine.nby, nb.fine.nbz)
+    C.f[:] = f0c
+    F.f[:] = f0f
+    g = nb.coarse
+    vel, ns, pr = g.bcs["velocity"], g.bcs["noslip"], g.bcs["pressure"]
+    velocity = (vx, vel.velocity[1], vel.velocity[2])
+    zc, zf = [np.zeros(nb.cf.n)] * 3, [np.zeros(nb.fc.n)] * 3
+    idx = {e: pre_collision_index(g, e) for e in (True, False)}
+    J = 0.0
+    for step in range(SPIN + AVG):
+        for _ in range(2):
+            F.collide(nb.fine.fluid, OMEGA_F, porosity=phi)
+            F.swap()
+        C.collide(g.fluid, OMEGA_C)
+        C.bc_post(ORACLE_POST_BC["VelocityInterpolatedCompressible"], vel.k, vel.q27, OMEGA_C, velocity)
+        C.bc_post(ORACLE_POST_BC["NoSlipInterpolatedCompressible"], ns.k, ns.q27, OMEGA_C)
+        C.swap()
+        C.bc_pressure(pr.k, pr.kN, pr.rho_bc, pr.direction)
+        orc.scale(True, C, F, C.even, nb.fc.coarse, nb.fc.fine, zf, OMEGA_C, OMEGA_F)
+        orc.scale(False, C, F, C.even, nb.cf.coarse, nb.cf.fine, zc, OMEGA_C, OMEGA_F)
+        if step >= SPIN:
+            f = C.f[idx[C.even][:, plane]]
+            drho = f.sum(0)
+            j = np.stack([(f * c[:, None]).sum(0) for c in _C])
+            J += (0.5 * (j * j).sum(0) / (1.0 + drho)).sum() / AVG
+    return J
+
+
+from virtualfluids_amd.grid import CX, CY, CZ  # noqa: E402
+
+_C = [np.asarray(CX, float), np.asarray(CY, float), np.asarray(CZ, float)]
+
+
+@pytest.mark.parametrize("K,observe", [(3, False), (100, False), (5, True)])
+def test_two_level_gradients_match_fp64_differences(gpu, orc, K, observe):
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_two_level_steps, kinetic_energy, pre_collision_index
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory
+
+    nb, plane, phi0 = _case()
+    Fac = BoundaryConditionFactory
+    fac = Fac()
+    fac.set_velocity_boundary_condition(Fac.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(Fac.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(Fac.PressureBC.PressureNonEquilibriumCompressible)
+    C = Lattice(nb.coarse, OMEGA_C, level=0)
+    F = Lattice(nb.fine, OMEGA_F, level=1)
+    for L in (C, F):
+        z = np.zeros(L.N)
+        L.init_equilibrium(z, np.full(L.N, 0.03), z, z)
+    f0c, f0f = C.populations().astype(np.float64), F.populations().astype(np.float64)
+    vel = nb.coarse.bcs["velocity"]
+    u0 = np.stack([np.asarray(v, dtype=np.float64) for v in vel.velocity])
+    U = torch.from_numpy(u0.astype(np.float32)).cuda().requires_grad_(True)
+    phi = torch.from_numpy(phi0.astype(np.float32)).cuda().requires_grad_(True)
+    idx = {e: torch.from_numpy(pre_collision_index(nb.coarse, e)).cuda() for e in (True, False)}
+    nodes = torch.from_numpy(plane).cuda()
+
+    if observe:
+        # the time average through the observation hook, in one call
+        _, _, J = k17_two_level_steps(C.f.detach().clone(), F.f.detach().clone(), C, F, (nb.cf, nb.fc), SPIN + AVG,
+                                      checkpoint_every=K, even=True, bc_factory=fac, velocity=U, porosity_fine=phi,
+                                      observe=lambda b, e: kinetic_energy(b, idx[e], nodes) / AVG,
+                                      observe_from=SPIN)
+    else:
+        c, f, _ = k17_two_level_steps(C.f.detach().clone(), F.f.detach().clone(), C, F, (nb.cf, nb.fc), SPIN,
+                                      checkpoint_every=K, even=True, bc_factory=fac, velocity=U, porosity_fine=phi)
+        even = SPIN % 2 == 0
+        J = 0.0
+        for _ in range(AVG):
+            c, f, _ = k17_two_level_steps(c, f, C, F, (nb.cf, nb.fc), 1, checkpoint_every=K, even=even,
+                                          bc_factory=fac, velocity=U, porosity_fine=phi)
+            even = not even
+            J = J + kinetic_energy(c, idx[even], nodes) / AVG
+    J.backward()
+    gU = U.grad.cpu().numpy().astype(np.float64)
+    gphi = phi.grad.cpu().numpy().astype(np.float64)
+    J_or = _oracle_objective(orc, nb, plane, f0c, f0f, u0[0], phi0)
+    print(f"K={K} observe={observe}: J gpu {float(J.detach()):.10e} oracle {J_or:.10e}")
+    assert abs(float(J.detach()) - J_or) < 1e-5 * abs(J_or)
+
+    rng = np.random.default_rng(1501)
+    h = 1e-7
+    v = rng.normal(size=u0.shape[1])
+    fd = (_oracle_objective(orc, nb, plane, f0c, f0f, u0[0] + h * v, phi0) -
+          _oracle_objective(orc, nb, plane, f0c, f0f, u0[0] - h * v, phi0)) / (2 * h)
+    ad = float(gU[0] @ v)
+    print(f"  dJ/dU_x . v: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) < 1e-4 * abs(fd)
+    w = rng.normal(size=phi0.size)
+    fd = (_oracle_objective(orc, nb, plane, f0c, f0f, u0[0], phi0 + h * w) -
+          _oracle_objective(orc, nb, plane, f0c, f0f, u0[0], phi0 - h * w)) / (2 * h)
+    ad = float(gphi @ w)
+    print(f"  dJ/dphi_fine . w: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) < 1e-4 * abs(fd)

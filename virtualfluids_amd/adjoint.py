@@ -190,6 +190,155 @@ def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: 
     return _K17Steps.apply(f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0)
 
 
+class _TwoLevelSteps(torch.autograd.Function):
+    """Coarse steps of a coarse/fine pair (UpdateGrid27::updateGrid for two levels: the fine level
+    steps twice, the coarse level steps, then fine -> coarse and coarse -> fine), differentiable
+    in both initial buffers and in the coarse velocity BC values, the coarse outlet densities and
+    the porosity of either level. Backward per coarse step, in reverse: the coarse-to-fine and
+    fine-to-coarse interface adjoints, the coarse step's adjoint, the two fine steps' adjoints."""
+
+    @staticmethod
+    def forward(ctx, fc0, ff0, velocity, rho_bc, por_c, por_f, sc, sf, cells, steps, K, even0, observe, observe_from):
+        Lc, Lf = sc.L, sf.L
+        for t, L in ((fc0, Lc), (ff0, Lf)):
+            if t.dtype != L.f.dtype or t.numel() != L.f.numel() or not t.is_cuda:
+                raise ValueError("initial buffers must be device buffers of each level's storage type and size")
+        stream = torch.cuda.current_stream()
+        if velocity is not None:
+            for i in range(3):
+                Lc.bcs["velocity"].vel[i].copy_(velocity[i].detach().to(torch.float32))
+        if rho_bc is not None:
+            Lc.bcs["pressure"].rho_bc.copy_(rho_bc.detach().to(torch.float32))
+        if por_c is not None:
+            Lc.set_porosity(por_c.detach())
+        if por_f is not None:
+            Lf.set_porosity(por_f.detach())
+        C, F = fc0.detach().clone(), ff0.detach().clone()
+        checkpoints = []
+        even = even0
+        J = torch.zeros((), dtype=torch.float64, device=fc0.device)
+        for i in range(steps):
+            if i % K == 0:
+                checkpoints.append((i, even, C.clone(), F.clone()))
+            C, F = _two_level_step(sc, sf, cells, C, F, even, stream)[-2:]
+            even = not even
+            if observe is not None and i >= observe_from:
+                J += observe(C, even).detach().to(torch.float64)
+        ctx.sc, ctx.sf, ctx.cells, ctx.steps, ctx.K = sc, sf, cells, steps, K
+        ctx.observe, ctx.observe_from = observe, observe_from
+        ctx.checkpoints = checkpoints
+        ctx.flags = (velocity is not None, rho_bc is not None, por_c is not None, por_f is not None)
+        return C, F, J
+
+    @staticmethod
+    def backward(ctx, gC, gF, gJ):
+        sc, sf, cells, steps, K = ctx.sc, ctx.sf, ctx.cells, ctx.steps, ctx.K
+        Lc, Lf = sc.L, sf.L
+        stream = torch.cuda.current_stream()
+        dev = gC.device
+        lc = gC.detach().to(torch.float32).contiguous().clone()
+        lf = gF.detach().to(torch.float32).contiguous().clone()
+        has_u, has_r, has_pc, has_pf = ctx.flags
+        ubar = torch.zeros(3 * Lc.bcs["velocity"].n, device=dev) if has_u else None
+        rbar = torch.zeros(Lc.bcs["pressure"].n, device=dev) if has_r else None
+        pcb = torch.zeros(Lc.N, device=dev) if has_pc else None
+        pfb = torch.zeros(Lf.N, device=dev) if has_pf else None
+        s = stream_handle(stream)
+        cf, fc = cells
+        for start, even_start, C0, F0 in reversed(ctx.checkpoints):
+            n = min(K, steps - start)
+            states = []
+            C, F, even = C0, F0, even_start
+            for _ in range(n):
+                st = _two_level_step(sc, sf, cells, C, F, even, stream)
+                states.append((even, C, F) + st)
+                C, F = st[-2], st[-1]
+                even = not even
+            for j, (even, C0s, F0s, F1, F2, C1, C2, _, _) in reversed(list(enumerate(states))):
+                if ctx.observe is not None and start + j >= ctx.observe_from:
+                    # the observed objective of the state after this step
+                    with torch.enable_grad():
+                        x = C2.detach().to(torch.float32).requires_grad_(True)
+                        (g,) = torch.autograd.grad(ctx.observe(x, not even), x)
+                    lc.add_(g.to(torch.float32), alpha=float(gJ))
+                pc = _param(Lc, C2, not even)
+                pf = _param(Lf, F2, True)
+                if cf.n:
+                    capi.call("vf_ScaleCoarseToFineCompressibleAdjoint", ctypes.byref(pc), ctypes.byref(pf),
+                              ctypes.byref(cf.struct), _neigh(cf), _vp(lc), _vp(lf), s)
+                if fc.n:
+                    capi.call("vf_ScaleFineToCoarseCompressibleAdjoint", ctypes.byref(pc), ctypes.byref(pf),
+                              ctypes.byref(fc.struct), _neigh(fc), _vp(lc), _vp(lf), s)
+                sc.adjoint(C0s, C1, lc, even, ubar, rbar, pcb, stream)
+                sf.adjoint(F1, F2, lf, False, None, None, pfb, stream)
+                sf.adjoint(F0s, F1, lf, True, None, None, pfb, stream)
+            del states
+        ctx.checkpoints = None
+        gv = ubar.view(3, -1) if ubar is not None else None
+        return (lc.to(Lc.f.dtype), lf.to(Lf.f.dtype), gv, rbar, pcb, pfb, None, None, None, None, None, None, None,
+                None)
+
+
+def _neigh(cells):
+    return ctypes.byref(cells.neigh) if cells.offsets is not None else None
+
+
+def _two_level_step(sc, sf, cells, C, F, even, stream):
+    """One coarse step out of place: returns (F1, F2, C1, C2, C_next, F_next) where F1, F2 follow
+    the two fine steps, C1 the coarse step, C2 = C_next the fine -> coarse write and F_next the
+    coarse -> fine write."""
+    Lc, Lf = sc.L, sf.L
+    cf, fc = cells
+    s = stream_handle(stream)
+    F1 = sf.forward(F, True, torch.empty_like(F), stream)
+    F2 = sf.forward(F1, False, torch.empty_like(F), stream)
+    C1 = sc.forward(C, even, torch.empty_like(C), stream)
+    C2 = C1.clone()
+    F3 = F2.clone()
+    pc = _param(Lc, C2, not even)
+    pf = _param(Lf, F2, True)
+    if fc.n:
+        capi.call("vf_ScaleFineToCoarseCompressible", ctypes.byref(pc), ctypes.byref(pf), ctypes.byref(fc.struct),
+                  _neigh(fc), s)
+    pf3 = _param(Lf, F3, True)
+    if cf.n:
+        capi.call("vf_ScaleCoarseToFineCompressible", ctypes.byref(pc), ctypes.byref(pf3), ctypes.byref(cf.struct),
+                  _neigh(cf), s)
+    return F1, F2, C1, C2, C2, F3
+
+
+def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, fine: Lattice, interface,
+                        steps: int, checkpoint_every: int = 10, even: bool | None = None,
+                        bc_factory: BoundaryConditionFactory | None = None,
+                        fine_bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
+                        rho_bc: torch.Tensor | None = None, porosity_coarse: torch.Tensor | None = None,
+                        porosity_fine: torch.Tensor | None = None, observe=None, observe_from: int = 0):
+    """`steps` coarse steps of a two-level pair from (fc0, ff0), differentiable in both buffers and,
+    when given, in the coarse velocity BC values (3, n), the coarse outlet densities and the
+    porosity of either level. interface: (cf, fc) InterfaceCells of the pair (NestedBox.cf,
+    NestedBox.fc). The fine level takes its steps with the even parity first, as after every
+    coarse step; `even` is the coarse parity of the first step (default: the lattice's).
+
+    observe(buf, even) -> scalar tensor, a differentiable function of a coarse buffer and its
+    parity, is summed over the coarse states after steps observe_from .. steps-1 and returned as
+    the third output (FP64); its gradient enters the reverse sweep step by step, so a time
+    average over thousands of steps needs no stored states beyond the checkpoints.
+    Returns (coarse buffer, fine buffer, observed sum)."""
+    from .refinement import DeviceInterfaceCells
+
+    if steps < 0 or checkpoint_every < 1:
+        raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
+    if fine.storage != coarse.storage:
+        raise ValueError("both levels must use the same storage type")
+    sc = Stepper(coarse, bc_factory)
+    sf = Stepper(fine, fine_bc_factory)
+    cf, fc = interface
+    cells = (DeviceInterfaceCells(cf, coarse.device), DeviceInterfaceCells(fc, coarse.device))
+    even0 = coarse.even if even is None else bool(even)
+    return _TwoLevelSteps.apply(fc0, ff0, velocity, rho_bc, porosity_coarse, porosity_fine, sc, sf, cells, steps,
+                                checkpoint_every, even0, observe, observe_from)
+
+
 def pre_collision_index(grid, even: bool) -> np.ndarray:
     """(27, N) flat indices into a 27*N buffer of every node's pre-collision populations."""
     N = grid.N

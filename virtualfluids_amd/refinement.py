@@ -78,28 +78,43 @@ def _surface(lo, hi, coords):
     return inside & on
 
 
-def nested_box(nc: tuple, lo: tuple, hi: tuple) -> NestedBox:
-    """Periodic coarse box of nc nodes with a fine box covering the coarse cells [lo, hi]."""
-    nc = tuple(int(v) for v in nc)
+def nested_box(nc: tuple, lo: tuple, hi: tuple, coarse: Grid | None = None) -> NestedBox:
+    """A fine box covering the coarse cells [lo, hi] inside a coarse grid: a periodic box of nc
+    nodes, or the given dense coarse grid (a channel with its boundary lists, for instance),
+    whose nodes and boundary nodes around the fine box must all be fluid."""
     lo = tuple(int(v) for v in lo)
     hi = tuple(int(v) for v in hi)
+    if coarse is not None:
+        nc = (coarse.nx, coarse.ny, coarse.nz)
+    nc = tuple(int(v) for v in nc)
     for a in range(3):
         if hi[a] - lo[a] < 4:
             raise ValueError("the fine box must span at least 4 coarse cells per axis")
         if lo[a] < 1 or hi[a] > nc[a] - 2:
             raise ValueError("the fine box must keep one coarse layer clear of the periodic wrap")
-    coarse = periodic_box(*nc)
+    if coarse is None:
+        coarse = periodic_box(*nc)
     mf = tuple(2 * (hi[a] - lo[a]) for a in range(3))
     fine = periodic_box(*mf)
     cx, cy, cz = coarse.coords()
     cc = (cx, cy, cz)
 
+    # the interface reads one coarse layer around [lo, hi]: it must be plain fluid
+    near = np.ones(coarse.N, dtype=bool)
+    for a in range(3):
+        near &= (cc[a] >= lo[a] - 1) & (cc[a] <= hi[a] + 1)
+    if np.any(coarse.geo[near] != GEO_FLUID):
+        raise ValueError("the fine box and its surrounding coarse layer must be fluid")
+    for name, bc in coarse.bcs.items():
+        if bc.n and np.any(near[bc.k]):
+            raise ValueError(f"the {name} boundary reaches the fine box")
+
     # coarse nodes under the fine level that no coarse step reads: not computed
     under = np.ones(coarse.N, dtype=bool)
     for a in range(3):
         under &= (cc[a] >= lo[a] + 3) & (cc[a] <= hi[a] - 3)
-    coarse.geo = np.where(under, GEO_SOLID, GEO_FLUID).astype(np.uint32)
-    coarse.fluid = np.nonzero(~under)[0].astype(np.uint32)
+    coarse.geo = np.where(under, GEO_SOLID, coarse.geo).astype(np.uint32)
+    coarse.fluid = np.nonzero(coarse.geo == GEO_FLUID)[0].astype(np.uint32)
 
     cf_sel = _surface(lo, tuple(h - 1 for h in hi), cc)
     kc = np.nonzero(cf_sel)[0]
