@@ -162,6 +162,97 @@ def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, 
     return out
 
 
+def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, storage="fp16", modes=(8, 4)):
+    """Two-level 2:1 nested refinement: coarse channel of 256 x 128 x 128 fluid nodes (velocity
+    inflow, pressure outlet, no-slip walls), Re 500 on the coarse level (inflow 0.03, channel
+    height 128), a fine box of 128^3 nodes over the coarse cells [64, 128) x [32, 96)^2 around a
+    porous obstacle (a seeded Gaussian random porosity in [0, 0.1], correlation length 8 fine
+    nodes, inside a ball of radius 24 fine nodes), FP16 shifted storage on both levels. The
+    inflow profile is u_x(y, z) = sum_mn a_mn sin(m pi y^) sin(n pi z^) over 8 x 4 modes, started
+    from the sine series of the uniform 0.03 inflow. Objective: the kinetic energy of the last
+    coarse fluid plane before the outlet, averaged over the last `average_last` coarse steps; its
+    gradient with respect to the 32 coefficients through k17_two_level_steps. Times a
+    forward-only run and the forward + backward run."""
+    import numpy as np
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_two_level_steps, kinetic_energy, pre_collision_index
+    from virtualfluids_amd.grid import channel, gaussian_random_field
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.refinement import nested_box
+    from virtualfluids_amd.simulation import BoundaryConditionFactory
+
+    u0, re = 0.03, 500.0
+    nu = u0 * 128 / re
+    omega_c, omega_f = 1.0 / (3 * nu + 0.5), 1.0 / (6 * nu + 0.5)
+    nb = nested_box(None, (65, 33, 33), (129, 97, 97), coarse=channel((256, 128, 128), inflow=u0))
+    xf, yf, zf = nb.fine.coords()
+    r2 = (xf - 63.5) ** 2 + (yf - 63.5) ** 2 + (zf - 63.5) ** 2
+    grf = gaussian_random_field((nb.fine.nx, nb.fine.ny, nb.fine.nz), 8.0, seed=20261202, low=0.0, high=0.1)
+    phi = np.where(r2 <= 24.0 ** 2, grf, 0.0).astype(np.float32)
+    F = BoundaryConditionFactory
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    C = Lattice(nb.coarse, omega_c, storage=storage, level=0)
+    Fl = Lattice(nb.fine, omega_f, storage=storage, level=1)
+    for L in (C, Fl):
+        z = np.zeros(L.N)
+        L.init_equilibrium(z, np.where(L.grid.geo == 19, u0, 0.0), z, z)
+    # inflow profile basis on the inflow nodes
+    vel = nb.coarse.bcs["velocity"]
+    _, yk, zk = nb.coarse.coords()
+    yh = (yk[vel.k] - 0.5) / 128.0
+    zh = (zk[vel.k] - 0.5) / 128.0
+    basis = np.stack([np.sin(m * np.pi * yh) * np.sin(n * np.pi * zh)
+                      for m in range(1, modes[0] + 1) for n in range(1, modes[1] + 1)])  # (32, nbc)
+    a0 = np.array([u0 * 16 / (np.pi ** 2 * m * n) if (m % 2 and n % 2) else 0.0
+                   for m in range(1, modes[0] + 1) for n in range(1, modes[1] + 1)])
+    B = torch.from_numpy(basis.astype(np.float32)).cuda()
+    x, _, _ = nb.coarse.coords()
+    plane = torch.from_numpy(np.nonzero((x == nb.coarse.nx - 3) & (nb.coarse.geo == 19))[0]).cuda()
+    idx = {e: torch.from_numpy(pre_collision_index(nb.coarse, e)).cuda() for e in (True, False)}
+    phi_t = torch.from_numpy(phi).cuda()
+    zero = torch.zeros(vel.n, device="cuda")
+
+    def objective(a):
+        ux = a.to(torch.float32) @ B
+        U = torch.stack([ux, zero, zero])
+        return k17_two_level_steps(C.f.detach().clone(), Fl.f.detach().clone(), C, Fl, (nb.cf, nb.fc), steps,
+                                   checkpoint_every=checkpoint_every, even=True, bc_factory=fac, velocity=U,
+                                   porosity_fine=phi_t,
+                                   observe=lambda b, e: kinetic_energy(b, idx[e], plane) / average_last,
+                                   observe_from=steps - average_last)[2]
+
+    a = torch.from_numpy(a0).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        J0 = float(objective(a))
+    torch.cuda.synchronize()
+    t_forward = time.perf_counter() - t0
+    a.requires_grad_(True)
+    t0 = time.perf_counter()
+    J = objective(a)
+    J.backward()
+    torch.cuda.synchronize()
+    t_gradient = time.perf_counter() - t0
+    g = a.grad.double().cpu().numpy()
+    nodes = int(nb.coarse.fluid.size + nb.fine.N)
+    out = {"case": f"two-level channel 256x128x128 coarse + 128^3 fine around a porous obstacle, Re 500, {storage} "
+                   f"storage, {steps} coarse steps, checkpoint every {checkpoint_every}, outlet kinetic energy "
+                   f"averaged over the last {average_last}; gradient w.r.t. {len(a0)} inflow Fourier coefficients",
+           "fluid_nodes": nodes, "objective": J0, "grad_l2": float(np.linalg.norm(g)),
+           "grad_first4": [float(v) for v in g[:4]],
+           "forward_s": round(t_forward, 3), "gradient_s": round(t_gradient, 3),
+           "gradient_vs_forward": round(t_gradient / t_forward, 3),
+           "forward_MLUPS": round((nb.coarse.fluid.size + 2 * nb.fine.N) * steps / t_forward / 1e6, 1)}
+    del C, Fl, J
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth():
     """Device-to-device copy of 2 GiB: (read + write bytes) / time."""
     import torch
@@ -234,6 +325,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="timed adjoint kernel steps")
     ap.add_argument("--no-sphere", action="store_true", help="skip the full-size sphere-channel gradient")
+    ap.add_argument("--no-two-level", action="store_true", help="skip the full-size two-level gradient")
     ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
                     help="time one storage type only (for profiling runs)")
     args = ap.parse_args()
@@ -309,6 +401,7 @@ def main():
                              for k, v in results.items()},
         "adjoint": adjoint,
         "sphere_channel_gradient": None if (args.no_sphere or args.no_adjoint) else sphere_channel_gradient(),
+        "two_level_gradient": None if (args.no_two_level or args.no_adjoint) else two_level_gradient(),
         "hbm_copy_GBps": round(copy_gbps, 1),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
