@@ -176,7 +176,7 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
     import numpy as np
     import torch
 
-    from virtualfluids_amd.adjoint import k17_two_level_steps, kinetic_energy, pre_collision_index
+    from virtualfluids_amd.adjoint import KineticEnergyObservation, k17_two_level_steps
     from virtualfluids_amd.grid import channel, gaussian_random_field
     from virtualfluids_amd.lattice import Lattice
     from virtualfluids_amd.refinement import nested_box
@@ -211,8 +211,8 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
                    for m in range(1, modes[0] + 1) for n in range(1, modes[1] + 1)])
     B = torch.from_numpy(basis.astype(np.float32)).cuda()
     x, _, _ = nb.coarse.coords()
-    plane = torch.from_numpy(np.nonzero((x == nb.coarse.nx - 3) & (nb.coarse.geo == 19))[0]).cuda()
-    idx = {e: torch.from_numpy(pre_collision_index(nb.coarse, e)).cuda() for e in (True, False)}
+    plane = np.nonzero((x == nb.coarse.nx - 3) & (nb.coarse.geo == 19))[0]
+    outlet_ke = KineticEnergyObservation(nb.coarse, plane, scale=1.0 / average_last)
     phi_t = torch.from_numpy(phi).cuda()
     zero = torch.zeros(vel.n, device="cuda")
 
@@ -222,8 +222,7 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
         return k17_two_level_steps(C.f.detach().clone(), Fl.f.detach().clone(), C, Fl, (nb.cf, nb.fc), steps,
                                    checkpoint_every=checkpoint_every, even=True, bc_factory=fac, velocity=U,
                                    porosity_fine=phi_t,
-                                   observe=lambda b, e: kinetic_energy(b, idx[e], plane) / average_last,
-                                   observe_from=steps - average_last)[2]
+                                   observe=outlet_ke, observe_from=steps - average_last)[2]
 
     a = torch.from_numpy(a0).cuda()
     torch.cuda.synchronize()

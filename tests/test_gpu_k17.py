@@ -49,13 +49,15 @@ def test_one_step_matches_oracle(gpu, orc, dims, box, even):
     assert err < 2e-7, err
 
 
-# row kernel widths: 0 = scalar kernel; 1/2/4 nodes per lane (rows of 1-4 waves); the row
+# row kernel widths: 0 = scalar kernel; 1/2/4 nodes per lane (rows of 1-4 waves, the last one
+# possibly partial: 40, 130 and 260 nodes); the row
 # kernel addresses the direction arrays through buffer descriptors up to 4 GiB per 14 slots
 # and through 64-bit global addresses beyond (forced here with VFAMD_ROW_BUFFER=0)
 @pytest.mark.parametrize("addressing", ["buffer", "global"])
 @pytest.mark.parametrize("width", [0, 1, 2, 4])
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
-@pytest.mark.parametrize("dims", [(64, 5, 3), (128, 4, 3), (256, 3, 2), (512, 2, 2)])
+@pytest.mark.parametrize("dims", [(64, 5, 3), (128, 4, 3), (256, 3, 2), (512, 2, 2), (40, 5, 3), (130, 4, 3),
+                                  (260, 3, 2)])
 def test_row_kernel_widths_match_oracle(gpu, orc, monkeypatch, width, storage, dims, addressing):
     """Every row-kernel width, storage type, addressing form and row length (1 to 4 waves per
     row, with the periodic wrap) against the FP64 oracle applied to the same stored (rounded)

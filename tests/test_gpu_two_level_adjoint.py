@@ -67,11 +67,12 @@ from virtualfluids_amd.grid import CX, CY, CZ  # noqa: E402
 _C = [np.asarray(CX, float), np.asarray(CY, float), np.asarray(CZ, float)]
 
 
-@pytest.mark.parametrize("K,observe", [(3, False), (100, False), (5, True)])
+@pytest.mark.parametrize("K,observe", [(3, False), (100, False), (5, "callable"), (4, "cells")])
 def test_two_level_gradients_match_fp64_differences(gpu, orc, K, observe):
     import torch
 
-    from virtualfluids_amd.adjoint import k17_two_level_steps, kinetic_energy, pre_collision_index
+    from virtualfluids_amd.adjoint import (KineticEnergyObservation, k17_two_level_steps, kinetic_energy,
+                                           pre_collision_index)
     from virtualfluids_amd.lattice import Lattice
     from virtualfluids_amd.simulation import BoundaryConditionFactory
 
@@ -95,11 +96,13 @@ def test_two_level_gradients_match_fp64_differences(gpu, orc, K, observe):
     nodes = torch.from_numpy(plane).cuda()
 
     if observe:
-        # the time average through the observation hook, in one call
+        # the time average through the observation hook, in one call: a plain callable (dense
+        # gradient) or an observation that names its cells (sparse gradient)
+        fn = (KineticEnergyObservation(nb.coarse, plane, scale=1.0 / AVG) if observe == "cells"
+              else (lambda b, e: kinetic_energy(b, idx[e], nodes) / AVG))
         _, _, J = k17_two_level_steps(C.f.detach().clone(), F.f.detach().clone(), C, F, (nb.cf, nb.fc), SPIN + AVG,
                                       checkpoint_every=K, even=True, bc_factory=fac, velocity=U, porosity_fine=phi,
-                                      observe=lambda b, e: kinetic_energy(b, idx[e], nodes) / AVG,
-                                      observe_from=SPIN)
+                                      observe=fn, observe_from=SPIN)
     else:
         c, f, _ = k17_two_level_steps(C.f.detach().clone(), F.f.detach().clone(), C, F, (nb.cf, nb.fc), SPIN,
                                       checkpoint_every=K, even=True, bc_factory=fac, velocity=U, porosity_fine=phi)

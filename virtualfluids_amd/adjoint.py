@@ -257,10 +257,7 @@ class _TwoLevelSteps(torch.autograd.Function):
             for j, (even, C0s, F0s, F1, F2, C1, C2, _, _) in reversed(list(enumerate(states))):
                 if ctx.observe is not None and start + j >= ctx.observe_from:
                     # the observed objective of the state after this step
-                    with torch.enable_grad():
-                        x = C2.detach().to(torch.float32).requires_grad_(True)
-                        (g,) = torch.autograd.grad(ctx.observe(x, not even), x)
-                    lc.add_(g.to(torch.float32), alpha=float(gJ))
+                    _observe_grad(ctx.observe, C2, not even, lc, gJ)
                 pc = _param(Lc, C2, not even)
                 pf = _param(Lf, F2, True)
                 if cf.n:
@@ -277,6 +274,21 @@ class _TwoLevelSteps(torch.autograd.Function):
         gv = ubar.view(3, -1) if ubar is not None else None
         return (lc.to(Lc.f.dtype), lf.to(Lf.f.dtype), gv, rbar, pcb, pfb, None, None, None, None, None, None, None,
                 None)
+
+
+def _observe_grad(observe, buf, even, lam, gJ):
+    """lam += gJ * d observe(buf, even) / d buf: on the observation's own cells when it names them
+    (cells / value), else through a dense FP32 copy of the buffer."""
+    with torch.enable_grad():
+        if hasattr(observe, "cells"):
+            cells = observe.cells(even)
+            vals = buf.reshape(-1)[cells].detach().to(torch.float32).requires_grad_(True)
+            (g,) = torch.autograd.grad(observe.value(vals, even), vals)
+            lam.index_add_(0, cells.reshape(-1), g.reshape(-1).to(torch.float32), alpha=float(gJ))
+        else:
+            x = buf.detach().to(torch.float32).requires_grad_(True)
+            (g,) = torch.autograd.grad(observe(x, even), x)
+            lam.add_(g.to(torch.float32), alpha=float(gJ))
 
 
 def _neigh(cells):
@@ -371,11 +383,37 @@ def node_moments(buf: torch.Tensor, index: torch.Tensor):
 
 def kinetic_energy(buf: torch.Tensor, index: torch.Tensor, nodes: torch.Tensor | None = None) -> torch.Tensor:
     """FP64 sum over all nodes (or the given node subset) of 1/2 |j|^2 / (1 + drho)."""
-    drho, j = node_moments(buf, index)
-    ke = 0.5 * (j * j).sum(0) / (1.0 + drho)
     if nodes is not None:
-        ke = ke[nodes]
-    return ke.sum()
+        index = index[:, nodes]
+    drho, j = node_moments(buf, index)
+    return (0.5 * (j * j).sum(0) / (1.0 + drho)).sum()
+
+
+class KineticEnergyObservation:
+    """scale * kinetic energy (FP64 sum of 1/2 |j|^2 / (1 + drho)) of a node subset of a state: an
+    observation for k17_two_level_steps that reads only the 27 cells of each of those nodes, so
+    its gradient is scattered into the adjoint without touching the rest of the buffer."""
+
+    def __init__(self, grid, nodes, scale: float = 1.0, device="cuda"):
+        nodes = np.asarray(nodes, dtype=np.int64)
+        self.index = {e: torch.from_numpy(pre_collision_index(grid, e)[:, nodes]).to(device) for e in (True, False)}
+        self.scale = float(scale)
+
+    def cells(self, even: bool) -> torch.Tensor:
+        """(27, M) flat indices of the cells the observation reads in a state of parity `even`."""
+        return self.index[bool(even)]
+
+    def value(self, f: torch.Tensor, even: bool) -> torch.Tensor:
+        """The observation from the values of its cells, (27, M)."""
+        f = f.to(torch.float64)
+        c = _CXYZ.get(f.device)
+        if c is None:
+            c = _CXYZ[f.device] = torch.tensor(np.stack([CX, CY, CZ]), dtype=torch.float64, device=f.device)
+        drho, j = f.sum(0), c @ f
+        return self.scale * (0.5 * (j * j).sum(0) / (1.0 + drho)).sum()
+
+    def __call__(self, buf: torch.Tensor, even: bool) -> torch.Tensor:
+        return self.value(buf.reshape(-1)[self.index[bool(even)]], even)
 
 
 def _cell_index(grid, k: np.ndarray, d: int, even: bool) -> np.ndarray:

@@ -134,6 +134,8 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_into_kernel(const S* __rest
 // loads its own aligned elements and takes the element after its last node from lane+1
 // (__shfl), and stores the element before its first node from lane-1. Across the waves of
 // the row, and around the periodic wrap, the boundary lanes trade those values through LDS.
+// A row need not fill its last wave (the lanes beyond it idle), and a Brinkman porosity field
+// scales each node's collision velocity.
 __device__ constexpr int kShiftedDirs[9] = {dM00, dMM0, dMP0, dM0M, dM0P, dMPP, dMMP, dMPM, dMMM};
 __host__ __device__ constexpr int shifted_index(int d)
 {
@@ -157,7 +159,13 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, 
     constexpr bool MIX = Codec<S>::storage == VF_STORAGE_FP16;
     __shared__ unsigned xin[kRowMaxThreads / 64][9];
     __shared__ float xout[kRowMaxThreads / 64][9];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, W = blockDim.x >> 6;
+    __shared__ float xlast[9];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // a row is nx / V lanes; the lanes of the last wave beyond it idle (loads, stores and the
+    // collision are skipped; they still reach every barrier)
+    const uint32_t lanes = (uint32_t)a.nx / V;
+    const bool active = threadIdx.x < lanes;
+    const bool lastLane = threadIdx.x + 1 == lanes;
     const uint32_t y = blockIdx.x, z = blockIdx.y;
     const uint32_t yp = (y + 1 == (uint32_t)a.ny) ? 0 : y + 1;
     const uint32_t zp = (z + 1 == (uint32_t)a.nz) ? 0 : z + 1;
@@ -207,8 +215,13 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, 
     };
 
     typename IO::Raw raw[27];
+    if (active) {
 #pragma unroll
-    for (int d = 0; d < 27; ++d) raw[d] = load(d);
+        for (int d = 0; d < 27; ++d) raw[d] = load(d);
+    } else {
+#pragma unroll
+        for (int d = 0; d < 27; ++d) raw[d] = typename IO::Raw{};
+    }
 
     // word holding the element after this lane's last node, for the c_x = -1 directions
     unsigned nxt[9];
@@ -219,8 +232,9 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, 
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 9; ++j) nxt[j] = __shfl(IO::word0(raw[kShiftedDirs[j]]), (lane + 1) & 63, 64);
-    if (lane == 63) {
-        const int src = (wave + 1 == W) ? 0 : wave + 1;
+    if (active && (lane == 63 || lastLane)) {
+        // the next wave's first element, or the row's first element around the periodic wrap
+        const int src = lastLane ? 0 : wave + 1;
 #pragma unroll
         for (int j = 0; j < 9; ++j) nxt[j] = xin[src][j];
     }
@@ -235,11 +249,13 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, 
             if (node_sel(d) & 1) fl[d] = (i + 1 < V) ? IO::elem(raw[d], i + 1) : IO::of_word(nxt[shifted_index(d)], 0);
             else fl[d] = IO::elem(raw[d], i);
         }
-        bool fluid = true;
-        if (MASK) fluid = a.mask[rows[0] + i] != 0;
+        bool fluid = active;
+        if (MASK && active) fluid = a.mask[rows[0] + i] != 0;
         if (fluid) {
             MacroOut mo;
-            k17_collide<MIX>(fl, r, mo);
+            // Brinkman porosity: the velocity scale 1 - porosity / 2 (1 without a field)
+            const float vs = a.porosity ? 1.0f - 0.5f * a.porosity[rows[0] + i] : 1.0f;
+            k17_collide<MIX>(fl, r, mo, vs);
             if (MACRO) {
                 a.rho[rows[0] + i] = mo.drho;
                 a.vx[rows[0] + i] = mo.vx;
@@ -270,25 +286,32 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, 
     }
 
     // directions without an x-shift are complete: store them before the exchange
+    if (active) {
 #pragma unroll
-    for (int d = 0; d < 27; ++d) {
-        if (node_sel(d) & 1) continue;
-        store(d, out[d]);
+        for (int d = 0; d < 27; ++d) {
+            if (node_sel(d) & 1) continue;
+            store(d, out[d]);
+        }
     }
 
-    // element before this lane's first node, for the c_x = -1 directions
-    if (lane == 63) {
+    // element before this lane's first node, for the c_x = -1 directions: from the previous
+    // lane, the previous wave's last lane, or (first lane) the row's last lane
+    if (active && lane == 63) {
 #pragma unroll
         for (int j = 0; j < 9; ++j) xout[wave][j] = carry[j];
+    }
+    if (lastLane) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) xlast[j] = carry[j];
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
         float s0 = __shfl(carry[j], (lane + 63) & 63, 64);
-        if (lane == 0) s0 = xout[(wave == 0) ? W - 1 : wave - 1][j];
+        if (lane == 0) s0 = (wave == 0) ? xlast[j] : xout[wave - 1][j];
         const int d = kShiftedDirs[j];
         out[d][0] = s0;
-        store(d, out[d]);
+        if (active) store(d, out[d]);
     }
 }
 
@@ -353,15 +376,18 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
 // ---- host launchers ----------------------------------------------------------------------
 // Width (nodes per lane) of the row kernel per storage type; 0 selects the scalar kernel.
 // Measured on MI355X at 256^3: FP32 1 node/lane (5.9 TB/s; 2 nodes/lane 5.8, scalar 4.9);
-// FP16 and BF16 4 nodes/lane where rows allow it (1-2 % above 2 nodes/lane; both near the
-// VALU issue limit), else 2.
+// FP16 and BF16 4 nodes/lane where rows are whole waves of it (1-2 % above 2 nodes/lane; both
+// near the VALU issue limit), else 2. Rows longer than one workgroup's lanes take the next width
+// that fits; rows no width fits take the scalar kernel.
 // VFAMD_ROW_WIDTH overrides the choice (0, 1, 2 or 4) for benchmarking.
 template <class S> static int row_width(int nx)
 {
     const char* e = std::getenv("VFAMD_ROW_WIDTH");
     if (e && *e) return std::atoi(e);
-    if (Codec<S>::storage == VF_STORAGE_FP32) return 1;
-    return (nx % 256 == 0 && nx / 4 <= kRowMaxThreads) ? 4 : 2;
+    auto fits = [nx](int v) { return nx % v == 0 && nx / v <= kRowMaxThreads; };
+    if (Codec<S>::storage == VF_STORAGE_FP32) return fits(1) ? 1 : fits(2) ? 2 : fits(4) ? 4 : 0;
+    if (nx % 256 == 0 && fits(4)) return 4;
+    return fits(2) ? 2 : fits(4) ? 4 : 0;
 }
 
 // Resident workgroups per CU for the row kernel (0: as many as registers allow). The FP32
@@ -401,7 +427,7 @@ template <class S, int V, bool EVEN, bool BUF>
 static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, hipStream_t s)
 {
     dim3 grid(a.ny, a.nz);
-    dim3 block(a.nx / V);
+    dim3 block(((a.nx / V + 63) / 64) * 64); // whole waves; lanes beyond the row idle
     const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>());
     if (out) {
         if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, true>), grid, block, lds, s, f, out, a, r);
@@ -440,7 +466,7 @@ static bool try_launch_row(const vf_lbm_parameter& p, const BoxArgs& a, const K1
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
     const int V = row_width<S>(p.nx);
-    if (p.porosity || V <= 0 || p.nx % (64 * V) != 0 || p.nx / V > kRowMaxThreads) return false;
+    if (V <= 0 || p.nx % V != 0 || p.nx / V > kRowMaxThreads) return false;
     if constexpr (sizeof(S) == 4) {
         if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, s); return true; }
     }
