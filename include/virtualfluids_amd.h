@@ -134,6 +134,11 @@ typedef struct vf_interpolation_cell_neighbor {
 int vf_abi_version(void);
 const char* vf_get_last_error(void);
 
+/* Device-to-device copy of `bytes` (a multiple of 16, both pointers 16-byte aligned) with
+ * aligned non-temporal 16-byte accesses. No counterpart in the reference: the copy kernel whose
+ * bandwidth the collision kernels are measured against. */
+int vf_stream_copy(const void* src, void* dst, uint64_t bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------
  * K17 collide + eso-twist stream
  * --------------------------------------------------------------------------------- */

@@ -95,3 +95,11 @@ def test_velocity_bc_needs_wall_velocity():
     assert lib.vf_VelocityBounceBack(ctypes.byref(p), ctypes.byref(bc), None) == capi.VF_ERR_INVALID_ARGUMENT
     bc.numberOfBCnodes = 0  # empty list: nothing to do
     assert lib.vf_VelocityBounceBack(ctypes.byref(p), ctypes.byref(bc), None) == capi.VF_OK
+
+
+def test_stream_copy_validates_alignment():
+    lib = capi.load()
+    assert lib.vf_stream_copy(ctypes.c_void_p(0x10), ctypes.c_void_p(0x20), 24, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_copy(ctypes.c_void_p(0x18), ctypes.c_void_p(0x20), 32, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_copy(None, ctypes.c_void_p(0x20), 32, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_copy(None, None, 0, None) == capi.VF_OK  # nothing to copy

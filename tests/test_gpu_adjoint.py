@@ -21,19 +21,34 @@ pytestmark = pytest.mark.gpu
 
 
 def _grid(kind):
-    return periodic_box(24, 10, 8) if kind == "periodic" else walled_box((22, 8, 6), {})
+    if kind == "periodic":
+        return periodic_box(24, 10, 8)
+    if kind == "walled":
+        return walled_box((22, 8, 6), {})
+    # rows of several waves, the last one partial (the row kernel's LDS hand-over)
+    return walled_box((150, 4, 3), {}) if kind == "long_walled" else periodic_box(130, 4, 3)
+
+
+# box: the row kernel (buffer addressing); box_global: its global-address form; box_scalar:
+# the one-node-per-thread kernel; indexed: fluid-node list and neighbour arrays
+ADJ_ENV = {"box": {}, "box_global": {"VFAMD_ROW_BUFFER": "0"}, "box_scalar": {"VFAMD_ADJ_ROW": "0"}, "indexed": {}}
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
-@pytest.mark.parametrize("path", ["box", "indexed"])
+@pytest.mark.parametrize("path", ["box", "box_global", "box_scalar", "indexed"])
 @pytest.mark.parametrize("even", [True, False])
-@pytest.mark.parametrize("kind", ["periodic", "walled"])
-def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, storage, path, even, kind):
+@pytest.mark.parametrize("kind", ["periodic", "walled", "long_periodic", "long_walled"])
+def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, monkeypatch, storage, path, even, kind):
     import torch
 
     from virtualfluids_amd import capi
     from virtualfluids_amd.lattice import Lattice
 
+    if kind.startswith("long") and path in ("box_scalar", "indexed") and storage != "fp32":
+        pytest.skip("long rows exercise the row kernel; the other paths are covered on the short boxes")
+    for k, v in ADJ_ENV[path].items():
+        monkeypatch.setenv(k, v)
+    path = "indexed" if path == "indexed" else "box"
     grid = _grid(kind)
     rng = np.random.default_rng(500 + 2 * even + (path == "box") + 4 * len(storage))
     omega = 1.8
@@ -59,7 +74,7 @@ def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, storage, path, even,
     err_s = np.abs(got_shift - ref_shift).max() / np.abs(ref_shift).max()
     print(f"  rel err adjoint {err:.2e} shift {err_s:.2e}")
     assert err < 3e-5 and err_s < 3e-4
-    if kind == "walled":  # cells only solid nodes own keep their adjoint
+    if "walled" in kind:  # cells only solid nodes own keep their adjoint
         assert np.any(got == lam0.astype(np.float32))
 
 

@@ -94,3 +94,19 @@ def test_kinetic_energy_matches_oracle(gpu, orc, kind):
     assert abs(e - ref[0]) <= 1e-13 * abs(ref[0])
     if kind == "walled":
         assert count == (10 - 2) * (7 - 2) * (6 - 2)
+
+
+@pytest.mark.parametrize("nbytes", [16, 4096 + 48, 64 << 20])
+def test_stream_copy_copies_exactly(gpu, nbytes):
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import stream_handle
+
+    a = torch.randint(0, 255, (nbytes + 32,), dtype=torch.uint8, device="cuda")
+    b = torch.zeros_like(a)
+    capi.call("vf_stream_copy", ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+              ctypes.c_uint64(nbytes), stream_handle())
+    torch.cuda.synchronize()
+    assert torch.equal(a[:nbytes], b[:nbytes])
+    assert int(b[nbytes:].abs().sum()) == 0  # nothing beyond the requested bytes

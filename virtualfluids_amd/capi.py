@@ -97,6 +97,7 @@ _Q = ctypes.POINTER(QBoundaryConditions)
 # name -> argtypes; every entry point returns int (vf_status)
 SIGNATURES = {
     "vf_abi_version": [],
+    "vf_stream_copy": [_vp, _vp, ctypes.c_uint64, _vp],
     "vf_k17_run": [_P, _vp],
     "vf_k17_run_on_indices": [_P, _vp, ctypes.c_uint32, ctypes.c_int32, _vp],
     "vf_k17_run_box": [_P, ctypes.c_int32, _vp],

@@ -11,9 +11,11 @@
 // are the caller's: the Python layer keeps a checkpoint every K steps and replays a segment).
 //
 // No counterpart in the reference.
+#include <cstdlib>
 #include <type_traits>
 
 #include "vf_k17_adjoint.h"
+#include "vf_rowio.h"
 
 namespace vfa {
 
@@ -86,6 +88,159 @@ __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __r
     brinkman_adjoint(a.porosity, a.porosityAdj, n[0], sb);
 }
 
+// ---- row kernel: one workgroup per x-row, one node per lane -------------------------------
+// The access pattern of the forward row kernel: every global access of a wave is aligned and
+// contiguous. The state and the adjoint of the nine c_x = -1 directions are read one node
+// further in +x, so a lane takes them from lane+1 (__shfl) and gives its result for those
+// cells to lane+1; across waves and around the periodic wrap the boundary lanes trade through
+// LDS. Lanes beyond the row idle. Solid nodes write their adjoint cells back unchanged.
+constexpr int kAdjRowMaxThreads = 512;
+
+template <class S, bool EVEN, bool MASK, bool SHIFT, bool BUF>
+__global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(const S* f, float* lam, AdjBoxArgs a,
+                                                                           K17Rates r)
+{
+    using IO = RowIO<S, 1>;
+    using LO = RowIO<float, 1>;
+    __shared__ unsigned xinS[kAdjRowMaxThreads / 64][9];
+    __shared__ float xinL[kAdjRowMaxThreads / 64][9];
+    __shared__ float xout[kAdjRowMaxThreads / 64][9];
+    __shared__ float xlast[9];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
+    const bool active = threadIdx.x < nx;
+    const bool lastLane = threadIdx.x + 1 == nx;
+    const uint32_t y = blockIdx.x, z = blockIdx.y;
+    const uint32_t yp = (y + 1 == ny) ? 0 : y + 1;
+    const uint32_t zp = (z + 1 == (uint32_t)a.nz) ? 0 : z + 1;
+    const uint32_t x = threadIdx.x;
+    // this lane's node in the 4 rows its cells touch, indexed by the y/z bits of node_sel
+    const uint32_t rows[4] = {nx * (y + ny * z) + x, nx * (yp + ny * z) + x, nx * (y + ny * zp) + x,
+                              nx * (yp + ny * zp) + x};
+    const uint64_t slotS = a.N * sizeof(S), slotL = a.N * sizeof(float);
+    __amdgpu_buffer_rsrc_t sLo, sHi, lLo, lHi;
+    if constexpr (BUF) {
+        void* fb0 = const_cast<S*>(f);
+        sLo = __builtin_amdgcn_make_buffer_rsrc(fb0, (short)0, (int)0xffffffff, 0x00020000);
+        sHi = __builtin_amdgcn_make_buffer_rsrc(static_cast<char*>(fb0) + kRowSplit * slotS, (short)0, (int)0xffffffff,
+                                                0x00020000);
+        lLo = __builtin_amdgcn_make_buffer_rsrc(lam, (short)0, (int)0xffffffff, 0x00020000);
+        lHi = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<char*>(lam) + kRowSplit * slotL, (short)0,
+                                                (int)0xffffffff, 0x00020000);
+    }
+    auto soff = [&](int d, uint64_t slotBytes) -> uint32_t {
+        const int sl = slot<EVEN>(d);
+        return (uint32_t)(sl >= kRowSplit ? sl - kRowSplit : sl) * (uint32_t)slotBytes;
+    };
+    auto loadS = [&](int d) {
+        const uint32_t off = rows[node_sel(d) >> 1] * (uint32_t)sizeof(S);
+        if constexpr (BUF) return IO::load(slot<EVEN>(d) >= kRowSplit ? sHi : sLo, off, soff(d, slotS));
+        else return IO::load(reinterpret_cast<const S*>(reinterpret_cast<const char*>(f) + slot<EVEN>(d) * slotS + off));
+    };
+    auto loadL = [&](int d) {
+        const uint32_t off = rows[node_sel(d) >> 1] * (uint32_t)sizeof(float);
+        if constexpr (BUF) return LO::load(slot<EVEN>(d) >= kRowSplit ? lHi : lLo, off, soff(d, slotL));
+        else return LO::load(reinterpret_cast<const float*>(reinterpret_cast<const char*>(lam) + slot<EVEN>(d) * slotL + off));
+    };
+    auto storeL = [&](int d, float val) {
+        const uint32_t off = rows[node_sel(d) >> 1] * (uint32_t)sizeof(float);
+        const float vals[1] = {val};
+        if constexpr (BUF) LO::store(slot<EVEN>(d) >= kRowSplit ? lHi : lLo, off, soff(d, slotL), vals);
+        else LO::store(reinterpret_cast<float*>(reinterpret_cast<char*>(lam) + slot<EVEN>(d) * slotL + off), vals);
+    };
+
+    typename IO::Raw rawS[27];
+    typename LO::Raw rawL[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        rawS[d] = active ? loadS(d) : typename IO::Raw{};
+        rawL[d] = active ? loadL(d) : typename LO::Raw{};
+    }
+
+    // state and adjoint of the cells one node further in +x, for the c_x = -1 directions
+    unsigned nxtS[9];
+    float nxtL[9];
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            xinS[wave][j] = IO::word0(rawS[kShiftedDirs[j]]);
+            xinL[wave][j] = LO::elem(rawL[kShiftedDirs[j]], 0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        nxtS[j] = __shfl(IO::word0(rawS[kShiftedDirs[j]]), (lane + 1) & 63, 64);
+        nxtL[j] = __shfl(LO::elem(rawL[kShiftedDirs[j]], 0), (lane + 1) & 63, 64);
+    }
+    if (active && (lane == 63 || lastLane)) {
+        // the next wave's first node, or the row's first node around the periodic wrap
+        const int src = lastLane ? 0 : wave + 1;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            nxtS[j] = xinS[src][j];
+            nxtL[j] = xinL[src][j];
+        }
+    }
+
+    // state v[d] and adjoint of the post-collision values g[inv(d)], both from the cell of pre[d]
+    float v[27], g[27], fb[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        if (node_sel(d) & 1) {
+            v[d] = IO::of_word(nxtS[shifted_index(d)], 0);
+            g[inv(d)] = nxtL[shifted_index(d)];
+        } else {
+            v[d] = IO::elem(rawS[d], 0);
+            g[inv(d)] = LO::elem(rawL[d], 0);
+        }
+    }
+    const uint32_t k = rows[0];
+    bool fluid = active;
+    if (MASK && active) fluid = a.mask[k] != 0;
+    if (fluid) {
+        float ub[3], sb = 0.0f;
+        const float vs = a.porosity ? 1.0f - 0.5f * a.porosity[k] : 1.0f;
+        k17_vjp(v, g, r, fb, SHIFT ? ub : nullptr, vs, a.porosity ? &sb : nullptr);
+        if (SHIFT) {
+            a.shiftAdj[k] += ub[0];
+            a.shiftAdj[a.N + k] += ub[1];
+            a.shiftAdj[2 * a.N + k] += ub[2];
+        }
+        brinkman_adjoint(a.porosity, a.porosityAdj, k, sb);
+    } else {
+        // the forward step leaves a solid node's cells alone: their adjoint passes through
+#pragma unroll
+        for (int d = 0; d < 27; ++d) fb[d] = g[inv(d)];
+    }
+
+    // cells without an x-shift are this lane's own
+    if (active) {
+#pragma unroll
+        for (int d = 0; d < 27; ++d) {
+            if (node_sel(d) & 1) continue;
+            storeL(d, fb[d]);
+        }
+    }
+    // the shifted cells belong to lane+1: take lane-1's, the previous wave's last lane's, or
+    // (first lane of the row) the row's last lane's
+    if (active && lane == 63) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) xout[wave][j] = fb[kShiftedDirs[j]];
+    }
+    if (lastLane) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) xlast[j] = fb[kShiftedDirs[j]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        float s0 = __shfl(fb[kShiftedDirs[j]], (lane + 63) & 63, 64);
+        if (lane == 0) s0 = (wave == 0) ? xlast[j] : xout[wave - 1][j];
+        if (active) storeL(kShiftedDirs[j], s0);
+    }
+}
+
 struct AdjIndexedArgs {
     uint64_t N;
     const uint32_t* nbx;
@@ -149,11 +304,39 @@ int check_adjoint_args(const vf_lbm_parameter* p, const float* adjoint)
     return VF_OK;
 }
 
+// VFAMD_ADJ_ROW=0 selects the scalar box kernel and VFAMD_ROW_BUFFER=0 the global-address form
+// of the row kernel (used by the tests to cover every form)
+bool env_off(const char* name)
+{
+    const char* e = std::getenv(name);
+    return e && *e == '0';
+}
+
+template <class S, bool EVEN, bool BUF>
+void launch_row_b(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift, hipStream_t s)
+{
+    const dim3 grid(a.ny, a.nz), block(((a.nx + 63) / 64) * 64);
+    if (mask && shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, true, BUF>), grid, block, 0, s, f, lam, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, false, BUF>), grid, block, 0, s, f, lam, a, r);
+    else if (shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, false, true, BUF>), grid, block, 0, s, f, lam, a, r);
+    else hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, false, false, BUF>), grid, block, 0, s, f, lam, a, r);
+}
+
 template <class S, bool EVEN>
 void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj, hipStream_t s)
 {
     AdjBoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, shift, p.porosity, padj};
     const K17Rates r = make_k17_rates(p);
+    if (p.nx <= kAdjRowMaxThreads && !env_off("VFAMD_ADJ_ROW")) {
+        // a row is one workgroup; the buffer form needs kRowSplit slots of the adjoint (the
+        // larger element) within 32-bit offsets
+        const S* f = static_cast<const S*>(p.distributions);
+        if ((uint64_t)kRowSplit * a.N * sizeof(float) <= 0xffffffffull && !env_off("VFAMD_ROW_BUFFER"))
+            launch_row_b<S, EVEN, true>(a, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
+        else
+            launch_row_b<S, EVEN, false>(a, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
+        return;
+    }
     const int bx = p.nx >= kAdjBlock ? kAdjBlock : ((p.nx + 63) / 64) * 64;
     const dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
     const S* f = static_cast<const S*>(p.distributions);
@@ -207,7 +390,7 @@ extern "C" int vf_k17_adjoint_box(const vf_lbm_parameter* p, float* adjoint, flo
         using S = std::remove_pointer_t<decltype(tag)>;
         launch_box<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, porosityAdjoint, s);
     });
-    return check_launch("k17_adjoint_box_kernel execution failed");
+    return check_launch("k17 adjoint box kernel execution failed");
 }
 
 extern "C" int vf_k17_adjoint(const vf_lbm_parameter* p, float* adjoint, float* velocityShiftAdjoint,

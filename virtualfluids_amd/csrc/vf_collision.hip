@@ -136,19 +136,11 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_into_kernel(const S* __rest
 // the row, and around the periodic wrap, the boundary lanes trade those values through LDS.
 // A row need not fill its last wave (the lanes beyond it idle), and a Brinkman porosity field
 // scales each node's collision velocity.
-__device__ constexpr int kShiftedDirs[9] = {dM00, dMM0, dMP0, dM0M, dM0P, dMPP, dMMP, dMPM, dMMM};
-__host__ __device__ constexpr int shifted_index(int d)
-{
-    return d == dM00 ? 0 : d == dMM0 ? 1 : d == dMP0 ? 2 : d == dM0M ? 3 : d == dM0P ? 4 : d == dMPP ? 5
-         : d == dMMP ? 6 : d == dMPM ? 7 : d == dMMM ? 8 : -1;
-}
-
 constexpr int kRowMaxThreads = 256;
 
-// BUF: the direction arrays are addressed through two buffer descriptors (slots below
-// kRowSplit from f, the rest from f + kRowSplit slots), which needs kRowSplit slots to span
-// less than 4 GiB; otherwise each direction takes a 64-bit base and a global access.
-constexpr int kRowSplit = 14;
+// BUF: the direction arrays are addressed through two buffer descriptors (kRowSplit), which
+// needs kRowSplit slots to span less than 4 GiB; otherwise each direction takes a 64-bit base
+// and a global access.
 
 // INTO: the stepped state goes to `fout` (same addresses, another buffer) and `f` is only read;
 // solid nodes then copy their cells, so `fout` is the complete state the in-place step leaves.
@@ -457,8 +449,8 @@ static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
         launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, out, s);
 }
 
-// The row kernel when the box allows it: a row of nx nodes is nx/V lanes, whole waves, at most
-// one workgroup (porous media take the box kernels). Returns false when it does not apply.
+// The row kernel when the box allows it: a row of nx nodes is nx/V lanes, at most one
+// workgroup. Returns false when it does not apply.
 template <class S, bool EVEN>
 static bool try_launch_row(const vf_lbm_parameter& p, const BoxArgs& a, const K17Rates& r, bool macro, S* out,
                            hipStream_t s)

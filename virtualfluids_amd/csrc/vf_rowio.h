@@ -13,6 +13,19 @@
 
 namespace vfa {
 
+// The nine directions with c_x = -1: their cells lie one node further in +x (node_sel bit 0).
+__device__ constexpr int kShiftedDirs[9] = {dM00, dMM0, dMP0, dM0M, dM0P, dMPP, dMMP, dMPM, dMMM};
+__host__ __device__ constexpr int shifted_index(int d)
+{
+    return d == dM00 ? 0 : d == dMM0 ? 1 : d == dMP0 ? 2 : d == dM0M ? 3 : d == dM0P ? 4 : d == dMPP ? 5
+         : d == dMMP ? 6 : d == dMPM ? 7 : d == dMMM ? 8 : -1;
+}
+
+// Buffer-descriptor addressing of a row kernel: slots below kRowSplit from the buffer's base,
+// the rest from base + kRowSplit slots, so 32-bit offsets reach every slot while kRowSplit
+// slots span less than 4 GiB.
+constexpr int kRowSplit = 14;
+
 template <int BYTES> struct RawT;
 template <> struct RawT<2> { using T = unsigned short; };
 template <> struct RawT<4> { using T = unsigned int; };
@@ -62,7 +75,8 @@ template <class S, int V> struct RowIO {
     // per-lane 64-bit address arithmetic is needed
     __device__ static Raw load(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
     {
-        if constexpr (BYTES == 4) return __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, kNT);
+        if constexpr (BYTES == 2) return __builtin_amdgcn_raw_buffer_load_b16(rs, voff, soff, kNT);
+        else if constexpr (BYTES == 4) return __builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, kNT);
         else if constexpr (BYTES == 8) return __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, kNT);
         else return __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, kNT);
     }
