@@ -25,8 +25,8 @@ def _grid(kind):
         return periodic_box(24, 10, 8)
     if kind == "walled":
         return walled_box((22, 8, 6), {})
-    # rows of several waves, the last one partial (the row kernel's LDS hand-over)
-    return walled_box((150, 4, 3), {}) if kind == "long_walled" else periodic_box(130, 4, 3)
+    # rows of four waves, the last one partial (the row kernel's LDS hand-over)
+    return walled_box((228, 4, 3), {}) if kind == "long_walled" else periodic_box(200, 4, 3)
 
 
 # box: the row kernel (buffer addressing); box_global: its global-address form; box_scalar:

@@ -1,0 +1,141 @@
+"""HIP-graph replay of the forward sweep against eager launches, bit for bit.
+
+* One level (a channel with velocity inflow, pressure outlet, interpolated no-slip walls and a
+  porosity field; dense-box and indexed addressing) and a two-level pair: the step replayed from
+  its graph and the same step launched eagerly on a second copy leave identical states after
+  every step, with eager torch work and a synchronisation between the replays.
+* k17_steps / k17_two_level_steps with graphs=True and graphs=False: identical outputs,
+  observed sums and gradients.
+"""
+import numpy as np
+import pytest
+
+from virtualfluids_amd.grid import channel, gaussian_random_field
+from virtualfluids_amd.refinement import nested_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _factory():
+    from virtualfluids_amd.simulation import BoundaryConditionFactory as F
+
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    return fac
+
+
+def _channel_lattice(box, storage="fp32"):
+    import torch
+
+    from virtualfluids_amd.lattice import Lattice
+
+    g = channel((24, 10, 8), inflow=0.03)
+    L = Lattice(g, 1.7, storage=storage, box=box)
+    z = np.zeros(g.N)
+    L.init_equilibrium(z, np.where(g.geo == 19, 0.03, 0.0), z, z)
+    phi = gaussian_random_field((g.nx, g.ny, g.nz), 2.0, seed=1700, low=0.0, high=0.05)
+    L.set_porosity(torch.from_numpy(phi.astype(np.float32)).cuda())
+    return L
+
+
+@pytest.mark.parametrize("box", [True, False])
+def test_single_level_replay_matches_eager_steps(gpu, box):
+    import torch
+
+    from virtualfluids_amd.adjoint import Stepper, _StepGraphs
+
+    L = _channel_lattice(box)
+    st = Stepper(L, _factory())
+    A, B = L.f.clone(), L.f.clone()
+    run = _StepGraphs(lambda ev, s: st.forward(A, ev, None, s))
+    even = True
+    for i in range(9):
+        run(even)
+        st.forward(B, even, None, torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        assert torch.equal(A, B), f"step {i} (graphs: {sorted(run.graphs)})"
+        _ = (B.float() * 2).sum().item()  # eager work between the replays
+        even = not even
+    assert len(run.graphs) == 2
+
+
+def _pair(storage="fp16"):
+    import torch
+
+    from virtualfluids_amd.lattice import Lattice
+
+    nb = nested_box(None, (6, 3, 3), (11, 9, 9), coarse=channel((20, 12, 12), inflow=0.03))
+    C = Lattice(nb.coarse, 1.6, storage=storage, level=0)
+    F = Lattice(nb.fine, 1.45, storage=storage, level=1)
+    for L in (C, F):
+        z = np.zeros(L.N)
+        L.init_equilibrium(z, np.where(L.grid.geo == 19, 0.03, 0.0), z, z)
+    phi = gaussian_random_field((nb.fine.nx, nb.fine.ny, nb.fine.nz), 2.0, seed=1710, low=0.0, high=0.05)
+    F.set_porosity(torch.from_numpy(phi.astype(np.float32)).cuda())
+    return nb, C, F
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_two_level_replay_matches_eager_steps(gpu, storage):
+    import torch
+
+    from virtualfluids_amd.adjoint import Stepper, _StepGraphs, _two_level_step_in_place
+    from virtualfluids_amd.refinement import DeviceInterfaceCells
+
+    nb, Lc, Lf = _pair(storage)
+    sc, sf = Stepper(Lc, _factory()), Stepper(Lf)
+    cells = (DeviceInterfaceCells(nb.cf, Lc.device), DeviceInterfaceCells(nb.fc, Lc.device))
+    Ca, Fa, Cb, Fb = Lc.f.clone(), Lf.f.clone(), Lc.f.clone(), Lf.f.clone()
+    run = _StepGraphs(lambda ev, s: _two_level_step_in_place(sc, sf, cells, Ca, Fa, ev, s))
+    even = True
+    for i in range(7):
+        run(even)
+        _two_level_step_in_place(sc, sf, cells, Cb, Fb, even, torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        assert torch.equal(Ca, Cb) and torch.equal(Fa, Fb), f"coarse step {i}"
+        _ = Fb.float().abs().max().item()
+        even = not even
+
+
+def test_k17_steps_same_with_and_without_graphs(gpu):
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_steps
+
+    L = _channel_lattice(True)
+    U0 = torch.stack(L.bcs["velocity"].vel).clone()
+    out = []
+    for graphs in (True, False):
+        x = L.f.detach().clone().requires_grad_(True)
+        U = U0.clone().requires_grad_(True)
+        y = k17_steps(x, L, 13, checkpoint_every=4, even=True, bc_factory=_factory(), velocity=U, graphs=graphs)
+        w = torch.linspace(-1, 1, y.numel(), device="cuda").reshape(y.shape)
+        (y.float() * w).sum().backward()
+        out.append((y.detach(), x.grad, U.grad))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+
+
+def test_two_level_steps_same_with_and_without_graphs(gpu):
+    import torch
+
+    from virtualfluids_amd.adjoint import KineticEnergyObservation, k17_two_level_steps
+
+    nb, Lc, Lf = _pair("fp32")
+    x, _, _ = nb.coarse.coords()
+    # a plane the inflow's perturbation reaches within the run
+    obs = KineticEnergyObservation(nb.coarse, np.nonzero((x == 3) & (nb.coarse.geo == 19))[0], scale=0.25)
+    U0 = torch.stack(Lc.bcs["velocity"].vel).clone()
+    out = []
+    for graphs in (True, False):
+        U = U0.clone().requires_grad_(True)
+        C, F, J = k17_two_level_steps(Lc.f.clone(), Lf.f.clone(), Lc, Lf, (nb.cf, nb.fc), 9, checkpoint_every=4,
+                                      even=True, bc_factory=_factory(), velocity=U, observe=obs, observe_from=5,
+                                      graphs=graphs)
+        J.backward()
+        out.append((C.detach(), F.detach(), J.detach(), U.grad))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+    assert float(out[0][2]) > 0 and float(out[0][3].abs().sum()) > 0

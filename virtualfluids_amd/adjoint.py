@@ -6,9 +6,10 @@ UpdateGrid27 (collision, velocity BC, no-slip BC, geometry BC, parity swap, pres
 src/gpu/core/Calculation/UpdateGrid27.cpp:54-96) and returns the final buffer. Its backward
 sweeps the steps in reverse with the adjoint kernels (vf_k17_adjoint[_box] and the BC
 adjoints): pressure BC, then geometry, no-slip and velocity, then the collision. The forward keeps a
-copy of the buffer every `checkpoint_every` steps; the backward replays one segment at a time
-from its checkpoint (out of place on the dense box, so the replay writes each intermediate
-state once and copies nothing) and walks the segment backwards. Memory: steps/K + K + 1
+copy of the buffer every `checkpoint_every` steps and steps in place, replaying one HIP graph
+per parity; the backward replays one segment at a time from its checkpoint (out of place on
+the dense box, so the replay writes each intermediate state once and copies nothing) and walks
+the segment backwards. Memory: steps/K + K + 1
 buffers of the storage type. Gradients flow to f0 and, when given, to the wall velocities of
 the velocity BC (3, n), the outlet densities of the pressure BC (n,) and the Brinkman porosity
 of every node (N,).
@@ -40,6 +41,49 @@ def _param(L: Lattice, state: torch.Tensor, even: bool) -> capi.LBMParameter:
 
 def _vp(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class _StepGraphs:
+    """Runs a time step that works in place on fixed buffers, replaying one HIP graph per parity.
+
+    step(even, stream) launches one step on `stream`. The first two calls (one of each parity)
+    run eagerly, so that every lazy set-up of the library happens outside a capture; from then
+    on each parity's launches are captured once (torch.cuda.CUDAGraph on a side stream of
+    default priority) and replayed on the current stream. The captured launches read their
+    kernel arguments at capture time: the buffers, BC arrays and porosity fields they name must
+    stay where they are while the graphs live (the owner drops them with the run). The step must
+    not allocate or clear memory with memset nodes. (torch.cuda.graph would also collect
+    garbage and empty the allocator's cache around every capture, which costs more than a
+    thousand replayed steps save.)"""
+
+    def __init__(self, step, enabled: bool = True):
+        self.step = step
+        self.enabled = enabled
+        self.graphs = {}
+        self.eager_calls = 0
+        self.side = None
+
+    def __call__(self, even: bool):
+        if not self.enabled or self.eager_calls < 2:
+            self.eager_calls += 1
+            self.step(even, torch.cuda.current_stream())
+            return
+        g = self.graphs.get(even)
+        if g is None:
+            cur = torch.cuda.current_stream()
+            if self.side is None:
+                self.side = torch.cuda.Stream()
+            self.side.wait_stream(cur)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(self.side):
+                g.capture_begin()
+                try:
+                    self.step(even, self.side)
+                finally:
+                    g.capture_end()
+            cur.wait_stream(self.side)
+            self.graphs[even] = g
+        g.replay()
 
 
 class Stepper:
@@ -115,7 +159,7 @@ class Stepper:
 
 class _K17Steps(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0):
+    def forward(ctx, f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs):
         L = stepper.L
         if f0.dtype != L.f.dtype or f0.numel() != L.f.numel() or not f0.is_cuda:
             raise ValueError("f0 must be a device buffer of the lattice's storage type and size")
@@ -128,21 +172,22 @@ class _K17Steps(torch.autograd.Function):
             L.bcs["pressure"].rho_bc.copy_(rho_bc.detach().to(torch.float32))
         if porosity is not None:
             L.set_porosity(porosity.detach())
-        bufs = [f0.detach().clone(), torch.empty_like(f0)]
-        cur = 0
+        # in place on one buffer: the same arithmetic as the out-of-place replay of the backward
+        buf = f0.detach().clone()
+        run = _StepGraphs(lambda ev, st: stepper.forward(buf, ev, None, st), graphs)
         checkpoints = []
         even = even0
         for i in range(steps):
             if i % checkpoint_every == 0:
-                checkpoints.append((i, even, bufs[cur].clone()))
-            res = stepper.forward(bufs[cur], even, bufs[1 - cur], stream)
-            cur = 1 - cur if res is bufs[1 - cur] else cur
+                checkpoints.append((i, even, buf.clone()))
+            run(even)
             even = not even
+        del run
         ctx.stepper, ctx.steps, ctx.K = stepper, steps, checkpoint_every
         ctx.checkpoints = checkpoints
         ctx.has_velocity, ctx.has_rho = velocity is not None, rho_bc is not None
         ctx.has_porosity = porosity is not None
-        return bufs[cur]
+        return buf
 
     @staticmethod
     def backward(ctx, grad_out):
@@ -166,18 +211,21 @@ class _K17Steps(torch.autograd.Function):
             del states
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
-        return lam.to(grad_out.dtype), gv, rbar, padj, None, None, None, None
+        return lam.to(grad_out.dtype), gv, rbar, padj, None, None, None, None, None
 
 
 def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: int = 10, even: bool | None = None,
               bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
-              rho_bc: torch.Tensor | None = None, porosity: torch.Tensor | None = None) -> torch.Tensor:
+              rho_bc: torch.Tensor | None = None, porosity: torch.Tensor | None = None,
+              graphs: bool = True) -> torch.Tensor:
     """`steps` time steps of `lattice` from buffer f0, differentiable in f0 and, when given, in
     the velocity BC's wall velocities `velocity` (3, n), the pressure BC's densities `rho_bc`
     and the Brinkman porosity per node `porosity` (N).
 
     The lattice's own buffer and parity are not touched (its BC parameter arrays take the given
     values); `even` is the parity of the first step (default: the lattice's current parity).
+    graphs: the forward sweep replays one HIP graph per parity (False: every launch from the
+    host; the results are the same bit for bit).
     """
     if steps < 0 or checkpoint_every < 1:
         raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
@@ -187,7 +235,7 @@ def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: 
     if rho_bc is not None and lattice.bcs.get("pressure") is None:
         raise ValueError("the lattice has no pressure BC nodes")
     even0 = lattice.even if even is None else bool(even)
-    return _K17Steps.apply(f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0)
+    return _K17Steps.apply(f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs)
 
 
 class _TwoLevelSteps(torch.autograd.Function):
@@ -198,7 +246,8 @@ class _TwoLevelSteps(torch.autograd.Function):
     fine-to-coarse interface adjoints, the coarse step's adjoint, the two fine steps' adjoints."""
 
     @staticmethod
-    def forward(ctx, fc0, ff0, velocity, rho_bc, por_c, por_f, sc, sf, cells, steps, K, even0, observe, observe_from):
+    def forward(ctx, fc0, ff0, velocity, rho_bc, por_c, por_f, sc, sf, cells, steps, K, even0, observe, observe_from,
+                graphs):
         Lc, Lf = sc.L, sf.L
         for t, L in ((fc0, Lc), (ff0, Lf)):
             if t.dtype != L.f.dtype or t.numel() != L.f.numel() or not t.is_cuda:
@@ -213,17 +262,20 @@ class _TwoLevelSteps(torch.autograd.Function):
             Lc.set_porosity(por_c.detach())
         if por_f is not None:
             Lf.set_porosity(por_f.detach())
+        # in place on one buffer per level: the same arithmetic as the out-of-place replay
         C, F = fc0.detach().clone(), ff0.detach().clone()
+        run = _StepGraphs(lambda ev, st: _two_level_step_in_place(sc, sf, cells, C, F, ev, st), graphs)
         checkpoints = []
         even = even0
         J = torch.zeros((), dtype=torch.float64, device=fc0.device)
         for i in range(steps):
             if i % K == 0:
                 checkpoints.append((i, even, C.clone(), F.clone()))
-            C, F = _two_level_step(sc, sf, cells, C, F, even, stream)[-2:]
+            run(even)
             even = not even
             if observe is not None and i >= observe_from:
                 J += observe(C, even).detach().to(torch.float64)
+        del run
         ctx.sc, ctx.sf, ctx.cells, ctx.steps, ctx.K = sc, sf, cells, steps, K
         ctx.observe, ctx.observe_from = observe, observe_from
         ctx.checkpoints = checkpoints
@@ -273,7 +325,7 @@ class _TwoLevelSteps(torch.autograd.Function):
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
         return (lc.to(Lc.f.dtype), lf.to(Lf.f.dtype), gv, rbar, pcb, pfb, None, None, None, None, None, None, None,
-                None)
+                None, None)
 
 
 def _observe_grad(observe, buf, even, lam, gJ):
@@ -319,12 +371,31 @@ def _two_level_step(sc, sf, cells, C, F, even, stream):
     return F1, F2, C1, C2, C2, F3
 
 
+def _two_level_step_in_place(sc, sf, cells, C, F, even, stream):
+    """One coarse step in place on C and F, in the order of _two_level_step."""
+    Lc, Lf = sc.L, sf.L
+    cf, fc = cells
+    sf.forward(F, True, None, stream)
+    sf.forward(F, False, None, stream)
+    sc.forward(C, even, None, stream)
+    s = stream_handle(stream)
+    pc = _param(Lc, C, not even)
+    pf = _param(Lf, F, True)
+    if fc.n:
+        capi.call("vf_ScaleFineToCoarseCompressible", ctypes.byref(pc), ctypes.byref(pf), ctypes.byref(fc.struct),
+                  _neigh(fc), s)
+    if cf.n:
+        capi.call("vf_ScaleCoarseToFineCompressible", ctypes.byref(pc), ctypes.byref(pf), ctypes.byref(cf.struct),
+                  _neigh(cf), s)
+
+
 def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, fine: Lattice, interface,
                         steps: int, checkpoint_every: int = 10, even: bool | None = None,
                         bc_factory: BoundaryConditionFactory | None = None,
                         fine_bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
                         rho_bc: torch.Tensor | None = None, porosity_coarse: torch.Tensor | None = None,
-                        porosity_fine: torch.Tensor | None = None, observe=None, observe_from: int = 0):
+                        porosity_fine: torch.Tensor | None = None, observe=None, observe_from: int = 0,
+                        graphs: bool = True):
     """`steps` coarse steps of a two-level pair from (fc0, ff0), differentiable in both buffers and,
     when given, in the coarse velocity BC values (3, n), the coarse outlet densities and the
     porosity of either level. interface: (cf, fc) InterfaceCells of the pair (NestedBox.cf,
@@ -335,6 +406,8 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
     parity, is summed over the coarse states after steps observe_from .. steps-1 and returned as
     the third output (FP64); its gradient enters the reverse sweep step by step, so a time
     average over thousands of steps needs no stored states beyond the checkpoints.
+    graphs: the forward sweep replays one HIP graph per coarse parity (False: every launch from
+    the host; the results are the same bit for bit).
     Returns (coarse buffer, fine buffer, observed sum)."""
     from .refinement import DeviceInterfaceCells
 
@@ -348,7 +421,7 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
     cells = (DeviceInterfaceCells(cf, coarse.device), DeviceInterfaceCells(fc, coarse.device))
     even0 = coarse.even if even is None else bool(even)
     return _TwoLevelSteps.apply(fc0, ff0, velocity, rho_bc, porosity_coarse, porosity_fine, sc, sf, cells, steps,
-                                checkpoint_every, even0, observe, observe_from)
+                                checkpoint_every, even0, observe, observe_from, graphs)
 
 
 def pre_collision_index(grid, even: bool) -> np.ndarray:

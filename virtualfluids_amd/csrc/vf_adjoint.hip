@@ -113,7 +113,9 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
     const uint32_t y = blockIdx.x, z = blockIdx.y;
     const uint32_t yp = (y + 1 == ny) ? 0 : y + 1;
     const uint32_t zp = (z + 1 == (uint32_t)a.nz) ? 0 : z + 1;
-    const uint32_t x = threadIdx.x;
+    // idle lanes read the row's last node (so that every load is unconditional and in bounds)
+    // and drop their results
+    const uint32_t x = active ? threadIdx.x : nx - 1;
     // this lane's node in the 4 rows its cells touch, indexed by the y/z bits of node_sel
     const uint32_t rows[4] = {nx * (y + ny * z) + x, nx * (yp + ny * z) + x, nx * (y + ny * zp) + x,
                               nx * (yp + ny * zp) + x};
@@ -153,8 +155,8 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
     typename LO::Raw rawL[27];
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
-        rawS[d] = active ? loadS(d) : typename IO::Raw{};
-        rawL[d] = active ? loadL(d) : typename LO::Raw{};
+        rawS[d] = loadS(d);
+        rawL[d] = loadL(d);
     }
 
     // state and adjoint of the cells one node further in +x, for the c_x = -1 directions
@@ -233,11 +235,15 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
         for (int j = 0; j < 9; ++j) xlast[j] = fb[kShiftedDirs[j]];
     }
     __syncthreads();
+    float s0[9];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
-        float s0 = __shfl(fb[kShiftedDirs[j]], (lane + 63) & 63, 64);
-        if (lane == 0) s0 = (wave == 0) ? xlast[j] : xout[wave - 1][j];
-        if (active) storeL(kShiftedDirs[j], s0);
+        s0[j] = __shfl(fb[kShiftedDirs[j]], (lane + 63) & 63, 64);
+        if (lane == 0) s0[j] = (wave == 0) ? xlast[j] : xout[wave - 1][j];
+    }
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) storeL(kShiftedDirs[j], s0[j]);
     }
 }
 
@@ -327,9 +333,14 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj
 {
     AdjBoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, shift, p.porosity, padj};
     const K17Rates r = make_k17_rates(p);
-    if (p.nx <= kAdjRowMaxThreads && !env_off("VFAMD_ADJ_ROW")) {
-        // a row is one workgroup; the buffer form needs kRowSplit slots of the adjoint (the
-        // larger element) within 32-bit offsets
+    // A row is one workgroup of ceil(nx / 64) waves. At ~245 VGPRs a CU holds 8 waves of this
+    // kernel, so only rows of 1, 2, 4 or 8 waves fill it (a 5-wave row of 258 nodes leaves 3 of
+    // 8 wave slots empty and runs 1.5x slower than the scalar kernel); other widths take the
+    // scalar kernel.
+    const int waves = (p.nx + 63) / 64;
+    if (p.nx <= kAdjRowMaxThreads && 8 % waves == 0 && !env_off("VFAMD_ADJ_ROW")) {
+        // the buffer form needs kRowSplit slots of the adjoint (the larger element) within
+        // 32-bit offsets
         const S* f = static_cast<const S*>(p.distributions);
         if ((uint64_t)kRowSplit * a.N * sizeof(float) <= 0xffffffffull && !env_off("VFAMD_ROW_BUFFER"))
             launch_row_b<S, EVEN, true>(a, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
