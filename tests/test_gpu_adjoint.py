@@ -190,7 +190,8 @@ def _duct(q_random, seed=47):
 
 @pytest.mark.parametrize("fn", sorted(POST_KINDS))
 @pytest.mark.parametrize("even", [True, False])
-def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even):
+@pytest.mark.parametrize("box", [True, False])  # (x, y, z) addressing or neighbour arrays
+def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box):
     import torch
 
     from virtualfluids_amd import capi
@@ -201,7 +202,7 @@ def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even):
     bc_name = "velocity" if kind in (1, 3, 4) else "noslip"
     rng = np.random.default_rng(700 + kind + 10 * even)
     omega = 1.8
-    L = Lattice(g, omega)
+    L = Lattice(g, omega, box=box)
     L.even = even
     L.set_populations(pack_pre(near_equilibrium(rng, g.N, u_amp=0.04, noise=5e-3), g, even))
     lam0 = rng.normal(0, 1, 27 * g.N).astype(np.float32)
@@ -231,7 +232,8 @@ def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even):
 
 
 @pytest.mark.parametrize("even", [True, False])
-def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even):
+@pytest.mark.parametrize("box", [True, False])
+def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box):
     import torch
 
     from virtualfluids_amd import capi
@@ -239,7 +241,7 @@ def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even):
 
     g = _duct(False)
     rng = np.random.default_rng(720 + even)
-    L = Lattice(g, 1.8)
+    L = Lattice(g, 1.8, box=box)
     L.even = even
     L.set_populations(pack_pre(near_equilibrium(rng, g.N, u_amp=0.04, noise=5e-3), g, even))
     lam0 = rng.normal(0, 1, 27 * g.N).astype(np.float32)

@@ -32,12 +32,12 @@ def test_init_neq_matches_oracle(gpu, orc, storage, addressing, kind):
     g = periodic_box(20, 12, 10) if kind == "periodic" else walled_box((18, 10, 8), {})
     omega = 1.75
     fields = _field(g)
-    L = Lattice(g, omega, storage=storage)
+    # box lattices address by (x, y, z); indexed lattices pass the neighbour arrays
+    L = Lattice(g, omega, storage=storage, box=(addressing == "box"))
     for dst, src in zip((L.rho, L.vx, L.vy, L.vz), fields):
         dst.copy_(__import__("torch").from_numpy(src.astype(np.float32)))
     nbinv = _u32(g.neighbor_inverse(), L.device)
-    if addressing == "box":
-        L.param.neighborX = L.param.neighborY = L.param.neighborZ = None
+    assert (L.param.neighborX is None) == (addressing == "box")
     print(f"init neq {storage} {addressing} {kind}", flush=True)
     capi.call("vf_InitNavierStokesCompressibleNonEquilibrium", L.pref(), _ptr(nbinv), stream_handle())
     got = L.populations()

@@ -79,11 +79,11 @@ def test_kinetic_energy_matches_oracle(gpu, orc, kind):
 
     g = walled_box((10, 7, 6), {}) if kind == "walled" else periodic_box(16, 6, 5)
     rng = np.random.default_rng(960)
-    L = Lattice(g, 1.7)
+    L = Lattice(g, 1.7, box=(kind != "walled"))  # walled: the neighbour-array addressing
     L.set_populations(pack_pre(near_equilibrium(rng, g.N, u_amp=0.04, noise=1e-3), g, True))
+    assert (L.param.neighborX is None) == (kind != "walled")
     if kind != "walled":
         # dense-box addressing: no neighbour arrays reach the kernel
-        L.param.neighborX = L.param.neighborY = L.param.neighborZ = None
         g.geo[rng.random(g.N) < 0.1] = 2  # a few solid nodes exclude themselves and their neighbours
         L.geo.copy_(torch.from_numpy(g.geo.astype(np.uint32).view(np.int32)).cuda())
     e, count = L.kinetic_energy()

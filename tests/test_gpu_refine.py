@@ -20,11 +20,12 @@ OMEGA_C, OMEGA_F = 1.0 / (3 * NU + 0.5), 1.0 / (6 * NU + 0.5)
 TOL = {"fp32": 5e-7, "fp16": 0.06 * 2.0 ** -11 + 5e-7, "bf16": 0.06 * 2.0 ** -8 + 5e-7}
 
 
-def _levels(nb, storage="fp32"):
+def _levels(nb, storage="fp32", box=True):
     from virtualfluids_amd.lattice import Lattice
 
-    return (Lattice(nb.coarse, OMEGA_C, storage=storage, level=0),
-            Lattice(nb.fine, OMEGA_F, storage=storage, level=1))
+    # box lattices address by (x, y, z); indexed lattices pass the neighbour arrays
+    return (Lattice(nb.coarse, OMEGA_C, storage=storage, level=0, box=box),
+            Lattice(nb.fine, OMEGA_F, storage=storage, level=1, box=box))
 
 
 def _oracle_levels(orc, nb):
@@ -52,7 +53,7 @@ def test_scale_kernel_matches_oracle(gpu, orc, storage, fine_to_coarse, even_coa
         offs = [rng.uniform(-0.5, 0.5, cells.n) for _ in range(3)]
         cells = InterfaceCells(cells.coarse, cells.fine, tuple(offs))
 
-    C, F = _levels(nb, storage)
+    C, F = _levels(nb, storage, box=(variant == "box_addressing"))
     C.even = even_coarse
     C.set_populations(pack_pre(fc0, nb.coarse, even_coarse))
     F.set_populations(pack_pre(ff0, nb.fine, True))
@@ -63,9 +64,8 @@ def test_scale_kernel_matches_oracle(gpu, orc, storage, fine_to_coarse, even_coa
 
     dc = DeviceInterfaceCells(cells, C.device)
     neigh = ctypes.byref(dc.neigh) if dc.offsets is not None else None
-    if variant == "box_addressing":  # the dense-box addressing path (no neighbour arrays)
-        for L in (C, F):
-            L.param.neighborX = L.param.neighborY = L.param.neighborZ = None
+    for L in (C, F):  # box_addressing: no neighbour arrays reach the kernel
+        assert (L.param.neighborX is None) == (variant == "box_addressing")
     fn = "vf_ScaleFineToCoarseCompressible" if fine_to_coarse else "vf_ScaleCoarseToFineCompressible"
     print(f"{fn} {storage} even_coarse={even_coarse} {variant} cells={cells.n}", flush=True)
     capi.call(fn, C.pref(), F.pref(), ctypes.byref(dc.struct), neigh, None)

@@ -37,8 +37,10 @@ def test_scale_adjoint_kernel_matches_reference(gpu, orc, storage, fine_to_coars
     if variant == "offsets":
         offs = [rng.uniform(-0.4, 0.4, cells.n) for _ in range(3)]
         cells = InterfaceCells(cells.coarse, cells.fine, tuple(offs))
-    C = Lattice(nb.coarse, OMEGA_C, storage=storage, level=0)
-    F = Lattice(nb.fine, OMEGA_F, storage=storage, level=1)
+    # box lattices address by (x, y, z); indexed lattices pass the neighbour arrays
+    box = variant == "box_addressing"
+    C = Lattice(nb.coarse, OMEGA_C, storage=storage, level=0, box=box)
+    F = Lattice(nb.fine, OMEGA_F, storage=storage, level=1, box=box)
     C.even = even_coarse
     C.set_populations(pack_pre(near_equilibrium(rng, nb.coarse.N, u_amp=0.04, noise=5e-3), nb.coarse, even_coarse))
     F.set_populations(pack_pre(near_equilibrium(rng, nb.fine.N, u_amp=0.04, noise=5e-3), nb.fine, True))
@@ -49,9 +51,8 @@ def test_scale_adjoint_kernel_matches_reference(gpu, orc, storage, fine_to_coars
 
     dc = DeviceInterfaceCells(cells, C.device)
     neigh = ctypes.byref(dc.neigh) if dc.offsets is not None else None
-    if variant == "box_addressing":
-        for L in (C, F):
-            L.param.neighborX = L.param.neighborY = L.param.neighborZ = None
+    for L in (C, F):
+        assert (L.param.neighborX is None) == box
     fn = "vf_ScaleFineToCoarseCompressibleAdjoint" if fine_to_coarse else "vf_ScaleCoarseToFineCompressibleAdjoint"
     print(f"{fn} {storage} even_coarse={even_coarse} {variant} cells={cells.n}", flush=True)
     capi.call(fn, C.pref(), F.pref(), ctypes.byref(dc.struct), neigh, ctypes.c_void_p(lc.data_ptr()),

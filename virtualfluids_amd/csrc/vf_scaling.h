@@ -2,7 +2,7 @@
 //
 //   arithmetic  src/lbm/interpolation/InterpolationCoefficients.h:63-432,
 //               src/lbm/refinement/InterpolationCF.h:57-306, InterpolationFC.h:56-263
-//   kernels     vf_scaling.hip (forward), vf_scaling_adjoint.hip (reverse)
+//   kernels     vf_scaling.hip (forward and reverse)
 //
 // The 8 source nodes of a cell sit at (sx, sy, sz)/2, n = ix + 2 iy + 4 iz (s = -1 for i = 0).
 // Each gives drho, u and the non-equilibrium second moments; every coefficient of the
@@ -455,9 +455,9 @@ VF_HD void fc_node_moments_vjp(const Coef& c, float omegaC, const NodeMoments& q
     cb.b100 += Eb;
 }
 
-// (d coefficients / d source moments)^T cb for the 8 nodes (mb overwritten). c0: the
-// coefficients before the offset shift are recomputed where the LaplaceRho term needs them.
-VF_HD void coefficients_vjp(const SrcMom (&m)[8], float xo, float yo, float zo, Coef cb, SrcMom (&mb)[8])
+// (d coefficients / d signed sums)^T cb: the adjoints of the cell sums. The coefficients before
+// the offset shift are recomputed where the LaplaceRho term needs them.
+VF_HD CellSums coefficients_vjp_sums(const SrcMom (&m)[8], float xo, float yo, float zo, Coef cb)
 {
     // ---- reverse of the offset shift (statements in reverse order) ----
     const float xx = xo * xo, yy = yo * yo, zz = zo * zo;
@@ -539,23 +539,36 @@ VF_HD void coefficients_vjp(const SrcMom (&m)[8], float xo, float yo, float zo, 
     Sb.RH.s[P1] += e8 * cb.d000; Sb.RH.s[PX] += 0.25f * cb.d100; Sb.RH.s[PY] += 0.25f * cb.d010;
     Sb.RH.s[PZ] += 0.25f * cb.d001; Sb.RH.s[PXY] += 0.5f * cb.d110; Sb.RH.s[PXZ] += 0.5f * cb.d101;
     Sb.RH.s[PYZ] += 0.5f * cb.d011; Sb.RH.s[PXYZ] += cb.d111;
-    // ---- signed sums -> nodes ----
-    for (int n = 0; n < 8; ++n) {
-        auto back = [&](const Sums& s) {
-            float v = 0.f;
-            for (int p = 0; p < 8; ++p) v += corner_sign(n, p) * s.s[p];
-            return v;
-        };
-        mb[n].vx = back(Sb.VX);
-        mb[n].vy = back(Sb.VY);
-        mb[n].vz = back(Sb.VZ);
-        mb[n].drho = back(Sb.RH);
-        mb[n].kxy = back(Sb.KXY);
-        mb[n].kyz = back(Sb.KYZ);
-        mb[n].kxz = back(Sb.KXZ);
-        mb[n].kxxMyy = back(Sb.KXXYY);
-        mb[n].kxxMzz = back(Sb.KXXZZ);
-    }
+    return Sb;
+}
+
+// signed sums -> the adjoint of node n's moments (n may be a run-time value)
+VF_HD SrcMom node_of_sums(const CellSums& Sb, int n)
+{
+    auto back = [&](const Sums& s) {
+        float v = 0.f;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) v += corner_sign(n, p) * s.s[p];
+        return v;
+    };
+    SrcMom mb;
+    mb.vx = back(Sb.VX);
+    mb.vy = back(Sb.VY);
+    mb.vz = back(Sb.VZ);
+    mb.drho = back(Sb.RH);
+    mb.kxy = back(Sb.KXY);
+    mb.kyz = back(Sb.KYZ);
+    mb.kxz = back(Sb.KXZ);
+    mb.kxxMyy = back(Sb.KXXYY);
+    mb.kxxMzz = back(Sb.KXXZZ);
+    return mb;
+}
+
+// (d coefficients / d source moments)^T cb for the 8 nodes (mb overwritten)
+VF_HD void coefficients_vjp(const SrcMom (&m)[8], float xo, float yo, float zo, Coef cb, SrcMom (&mb)[8])
+{
+    const CellSums Sb = coefficients_vjp_sums(m, xo, yo, zo, cb);
+    for (int n = 0; n < 8; ++n) mb[n] = node_of_sums(Sb, n);
 }
 
 // (d source moments / d populations)^T mb

@@ -4,7 +4,7 @@
 //   fine -> coarse  scaleFineToCoarseCompressible_Device (src/gpu/core/GridScaling/scaleFineToCoarseCompressible.cu:76-129)
 //   arithmetic      src/lbm/interpolation/InterpolationCoefficients.h, src/lbm/refinement/Interpolation{CF,FC}.h
 //
-// One thread per interface cell. The 8 source nodes of a cell are the corners of the cell
+// Eight lanes per interface cell. The 8 source nodes of a cell are the corners of the cell
 // whose lowest corner is the listed index, n = ix + 2 iy + 4 iz (M = 0, P = 1); each source
 // node's pre-collision state gives drho, u and the non-equilibrium second moments; every
 // coefficient of the interpolation polynomials is a short combination of the signed sums
@@ -66,47 +66,89 @@ struct ScaleArgs {
     float omegaC, omegaF;
 };
 
-template <class S>
-__global__ __launch_bounds__(64) void scale_cf_kernel(const S* __restrict__ fc, S* __restrict__ ff, Addr ac, Addr af,
-                                                      ScaleArgs s, int evenC)
+// Eight lanes per interface cell (a 64-lane workgroup holds 8 cells): lane q loads and reduces
+// source node q, the 8 lanes trade their source moments through LDS, each lane forms the cell's
+// coefficients (the same arithmetic in every lane), and lane q then rebuilds fine child q (CF)
+// or an eighth of the coarse node's 27 populations (FC). One cell per thread would leave most
+// of the chip idle: an interface has tens of thousands of cells, each a long serial chain.
+constexpr int kScaleBlock = 64;
+constexpr int kScaleCells = kScaleBlock / 8;
+
+struct CellLane {
+    uint32_t i;  // interface cell
+    int q;       // lane within the cell: source corner, fine child, or slice of the 27 directions
+    bool valid;  // cell < number of cells (the other lanes still reach every barrier)
+};
+
+__device__ __forceinline__ CellLane cell_lane(uint32_t n)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
-    SrcMom m[8];
+    CellLane c;
+    c.i = blockIdx.x * kScaleCells + (threadIdx.x >> 3);
+    c.q = threadIdx.x & 7;
+    c.valid = c.i < n;
+    return c;
+}
+
+// the 8 source moments of this lane's cell: lane q computes corner q, LDS hands them round
+template <class S>
+__device__ __forceinline__ void cell_source_moments(const S* f, const Addr& a, uint32_t base, const CellLane& cl,
+                                                    bool even, float omega, SrcMom (&m)[8])
+{
+    __shared__ SrcMom sm[kScaleBlock];
+    if (cl.valid) sm[threadIdx.x] = source_moments(f, a, corner(a, base, cl.q), even, omega);
+    __syncthreads();
+    const int g0 = threadIdx.x & ~7;
 #pragma unroll
-    for (int n = 0; n < 8; ++n) m[n] = source_moments(fc, ac, corner(ac, s.cellsC[i], n), evenC != 0, s.omegaC);
-    const float xo = s.ox ? s.ox[i] : 0.f, yo = s.oy ? s.oy[i] : 0.f, zo = s.oz ? s.oz[i] : 0.f;
-    const Coef c = coefficients(m, xo, yo, zo);
-#pragma unroll 1
-    for (int q = 0; q < 8; ++q) {
-        const NodeMoments nm = cf_node_moments(c, cf_child_offset(q, 0), cf_child_offset(q, 1), cf_child_offset(q, 2),
-                                               0.5f, s.omegaF);
-        float f[27];
-        rebuild_populations(f, nm);
-        uint32_t nn[8];
-        cell_nodes(af, corner(af, s.cellsF[i], q), nn);
-#pragma unroll
-        for (int d = 0; d < 27; ++d) write_pre(ff, af, nn, d, true, f[d]);
-    }
+    for (int n = 0; n < 8; ++n) m[n] = sm[g0 + n];
+}
+
+__device__ __forceinline__ void cell_offsets(const ScaleArgs& s, const CellLane& cl, float& xo, float& yo, float& zo)
+{
+    xo = (s.ox && cl.valid) ? s.ox[cl.i] : 0.f;
+    yo = (s.oy && cl.valid) ? s.oy[cl.i] : 0.f;
+    zo = (s.oz && cl.valid) ? s.oz[cl.i] : 0.f;
 }
 
 template <class S>
-__global__ __launch_bounds__(64) void scale_fc_kernel(S* __restrict__ fc, const S* __restrict__ ff, Addr ac, Addr af,
-                                                      ScaleArgs s, int evenC)
+__global__ __launch_bounds__(kScaleBlock) void scale_cf_kernel(const S* __restrict__ fc, S* __restrict__ ff, Addr ac,
+                                                               Addr af, ScaleArgs s, int evenC)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
+    const CellLane cl = cell_lane(s.n);
     SrcMom m[8];
+    cell_source_moments(fc, ac, cl.valid ? s.cellsC[cl.i] : 0u, cl, evenC != 0, s.omegaC, m);
+    if (!cl.valid) return;
+    float xo, yo, zo;
+    cell_offsets(s, cl, xo, yo, zo);
+    const Coef c = coefficients(m, xo, yo, zo);
+    const int q = cl.q;
+    const NodeMoments nm = cf_node_moments(c, cf_child_offset(q, 0), cf_child_offset(q, 1), cf_child_offset(q, 2),
+                                           0.5f, s.omegaF);
+    float f[27];
+    rebuild_populations(f, nm);
+    uint32_t nn[8];
+    cell_nodes(af, corner(af, s.cellsF[cl.i], q), nn);
 #pragma unroll
-    for (int n = 0; n < 8; ++n) m[n] = source_moments(ff, af, corner(af, s.cellsF[i], n), true, s.omegaF);
-    const float xo = s.ox ? s.ox[i] : 0.f, yo = s.oy ? s.oy[i] : 0.f, zo = s.oz ? s.oz[i] : 0.f;
+    for (int d = 0; d < 27; ++d) write_pre(ff, af, nn, d, true, f[d]);
+}
+
+template <class S>
+__global__ __launch_bounds__(kScaleBlock) void scale_fc_kernel(S* __restrict__ fc, const S* __restrict__ ff, Addr ac,
+                                                               Addr af, ScaleArgs s, int evenC)
+{
+    const CellLane cl = cell_lane(s.n);
+    SrcMom m[8];
+    cell_source_moments(ff, af, cl.valid ? s.cellsF[cl.i] : 0u, cl, true, s.omegaF, m);
+    if (!cl.valid) return;
+    float xo, yo, zo;
+    cell_offsets(s, cl, xo, yo, zo);
     const Coef c = coefficients(m, xo, yo, zo);
     float f[27];
     rebuild_populations(f, fc_node_moments(c, s.omegaC));
     uint32_t nn[8];
-    cell_nodes(ac, s.cellsC[i], nn);
+    cell_nodes(ac, s.cellsC[cl.i], nn);
 #pragma unroll
-    for (int d = 0; d < 27; ++d) write_pre(fc, ac, nn, d, evenC != 0, f[d]);
+    for (int d = 0; d < 27; ++d)
+        if ((d & 7) == cl.q) write_pre(fc, ac, nn, d, evenC != 0, f[d]);
 }
 
 // ---- reverse mode ------------------------------------------------------------------------
@@ -115,29 +157,58 @@ __global__ __launch_bounds__(64) void scale_fc_kernel(S* __restrict__ fc, const 
 // transposed interpolation of their adjoint is added to the source nodes' cells, which
 // neighbouring interface cells share (atomic adds). Linearisation point: the source state the
 // forward kernel read. Per cell the chain is recomputed from that state (nothing is stored).
+// Lanes as in the forward kernels: lane q takes fine child q (CF) and source node q.
 
 __device__ __forceinline__ float* adj_cell(float* lam, const Addr& a, const uint32_t (&n)[8], int d, bool even)
 {
     return slot_ptr(lam, a.N, d, even) + n[node_sel(d)];
 }
 
-template <class S>
-__global__ __launch_bounds__(64) void scale_cf_adjoint_kernel(const S* __restrict__ fc, float* __restrict__ lamC,
-                                                              float* __restrict__ lamF, Addr ac, Addr af, ScaleArgs s,
-                                                              int evenC)
+// sum of a Coef over the 8 lanes of a cell (butterfly within the lane group)
+__device__ __forceinline__ void coef_group_sum(Coef& c)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
-    SrcMom m[8];
+    float* p = &c.a000;
 #pragma unroll
-    for (int n = 0; n < 8; ++n) m[n] = source_moments(fc, ac, corner(ac, s.cellsC[i], n), evenC != 0, s.omegaC);
-    const float xo = s.ox ? s.ox[i] : 0.f, yo = s.oy ? s.oy[i] : 0.f, zo = s.oz ? s.oz[i] : 0.f;
+    for (int k = 0; k < (int)(sizeof(Coef) / sizeof(float)); ++k) {
+        float v = p[k];
+        v += __shfl_xor(v, 1, 8);
+        v += __shfl_xor(v, 2, 8);
+        v += __shfl_xor(v, 4, 8);
+        p[k] = v;
+    }
+}
+
+// adjoint of source node q's moments back to its 27 cells
+template <class S>
+__device__ __forceinline__ void source_node_adjoint(const S* f, float* lam, const Addr& a, uint32_t k, bool even,
+                                                    float omega, const SrcMom& mb)
+{
+    float v[27], fb[27];
+    node_populations(f, a, k, even, v);
+    source_moments_vjp(v, omega, mb, fb);
+    uint32_t nn[8];
+    cell_nodes(a, k, nn);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) atomicAdd(adj_cell(lam, a, nn, d, even), fb[d]);
+}
+
+template <class S>
+__global__ __launch_bounds__(kScaleBlock) void scale_cf_adjoint_kernel(const S* __restrict__ fc,
+                                                                       float* __restrict__ lamC,
+                                                                       float* __restrict__ lamF, Addr ac, Addr af,
+                                                                       ScaleArgs s, int evenC)
+{
+    const CellLane cl = cell_lane(s.n);
+    SrcMom m[8];
+    cell_source_moments(fc, ac, cl.valid ? s.cellsC[cl.i] : 0u, cl, evenC != 0, s.omegaC, m);
+    float xo, yo, zo;
+    cell_offsets(s, cl, xo, yo, zo);
     const Coef c = coefficients(m, xo, yo, zo);
     Coef cb = coef_zero();
-#pragma unroll 1
-    for (int q = 0; q < 8; ++q) {
+    const int q = cl.q;
+    if (cl.valid) {
         uint32_t nn[8];
-        cell_nodes(af, corner(af, s.cellsF[i], q), nn);
+        cell_nodes(af, corner(af, s.cellsF[cl.i], q), nn);
         float g[27];
 #pragma unroll
         for (int d = 0; d < 27; ++d) {
@@ -149,57 +220,42 @@ __global__ __launch_bounds__(64) void scale_cf_adjoint_kernel(const S* __restric
         const NodeMoments nb = rebuild_vjp(cf_node_moments(c, x, y, z, 0.5f, s.omegaF), g);
         cf_node_moments_vjp(c, x, y, z, 0.5f, s.omegaF, nb, cb);
     }
-    SrcMom mb[8];
-    coefficients_vjp(m, xo, yo, zo, cb, mb);
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        const uint32_t k = corner(ac, s.cellsC[i], n);
-        float f[27], fb[27];
-        node_populations(fc, ac, k, evenC != 0, f);
-        source_moments_vjp(f, s.omegaC, mb[n], fb);
-        uint32_t nn[8];
-        cell_nodes(ac, k, nn);
-#pragma unroll
-        for (int d = 0; d < 27; ++d) atomicAdd(adj_cell(lamC, ac, nn, d, evenC != 0), fb[d]);
-    }
+    coef_group_sum(cb);  // every lane of the group takes part (invalid cells add zeros)
+    if (!cl.valid) return;
+    const SrcMom mb = node_of_sums(coefficients_vjp_sums(m, xo, yo, zo, cb), q);
+    source_node_adjoint(fc, lamC, ac, corner(ac, s.cellsC[cl.i], q), evenC != 0, s.omegaC, mb);
 }
 
 template <class S>
-__global__ __launch_bounds__(64) void scale_fc_adjoint_kernel(const S* __restrict__ ff, float* __restrict__ lamC,
-                                                              float* __restrict__ lamF, Addr ac, Addr af, ScaleArgs s,
-                                                              int evenC)
+__global__ __launch_bounds__(kScaleBlock) void scale_fc_adjoint_kernel(const S* __restrict__ ff,
+                                                                       float* __restrict__ lamC,
+                                                                       float* __restrict__ lamF, Addr ac, Addr af,
+                                                                       ScaleArgs s, int evenC)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.n) return;
+    const CellLane cl = cell_lane(s.n);
     SrcMom m[8];
-#pragma unroll
-    for (int n = 0; n < 8; ++n) m[n] = source_moments(ff, af, corner(af, s.cellsF[i], n), true, s.omegaF);
-    const float xo = s.ox ? s.ox[i] : 0.f, yo = s.oy ? s.oy[i] : 0.f, zo = s.oz ? s.oz[i] : 0.f;
-    const Coef c = coefficients(m, xo, yo, zo);
+    cell_source_moments(ff, af, cl.valid ? s.cellsF[cl.i] : 0u, cl, true, s.omegaF, m);
+    // every lane reads the written coarse node's adjoint; after the barrier lane q clears its
+    // slice of the 27 cells
     uint32_t nc[8];
-    cell_nodes(ac, s.cellsC[i], nc);
     float g[27];
+    if (cl.valid) {
+        cell_nodes(ac, s.cellsC[cl.i], nc);
 #pragma unroll
-    for (int d = 0; d < 27; ++d) {
-        float* w = adj_cell(lamC, ac, nc, d, evenC != 0);
-        g[d] = *w;
-        *w = 0.f;
+        for (int d = 0; d < 27; ++d) g[d] = *adj_cell(lamC, ac, nc, d, evenC != 0);
     }
+    __syncthreads();
+    if (!cl.valid) return;
+#pragma unroll
+    for (int d = 0; d < 27; ++d)
+        if ((d & 7) == cl.q) *adj_cell(lamC, ac, nc, d, evenC != 0) = 0.f;
+    float xo, yo, zo;
+    cell_offsets(s, cl, xo, yo, zo);
+    const Coef c = coefficients(m, xo, yo, zo);
     Coef cb = coef_zero();
     fc_node_moments_vjp(c, s.omegaC, rebuild_vjp(fc_node_moments(c, s.omegaC), g), cb);
-    SrcMom mb[8];
-    coefficients_vjp(m, xo, yo, zo, cb, mb);
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        const uint32_t k = corner(af, s.cellsF[i], n);
-        float f[27], fb[27];
-        node_populations(ff, af, k, true, f);
-        source_moments_vjp(f, s.omegaF, mb[n], fb);
-        uint32_t nn[8];
-        cell_nodes(af, k, nn);
-#pragma unroll
-        for (int d = 0; d < 27; ++d) atomicAdd(adj_cell(lamF, af, nn, d, true), fb[d]);
-    }
+    const SrcMom mb = node_of_sums(coefficients_vjp_sums(m, xo, yo, zo, cb), cl.q);
+    source_node_adjoint(ff, lamF, af, corner(af, s.cellsF[cl.i], cl.q), true, s.omegaF, mb);
 }
 
 static int check_level(const vf_lbm_parameter* p)
@@ -239,7 +295,7 @@ static int launch_scale(bool fine_to_coarse, const vf_lbm_parameter* c, const vf
     ScaleArgs s{cells->coarseCellIndices, cells->fineCellIndices, cells->numberOfCells,
                 off ? off->x : nullptr, off ? off->y : nullptr, off ? off->z : nullptr, c->omega, f->omega};
     const Addr ac = make_addr(*c), af = make_addr(*f);
-    const dim3 grid((cells->numberOfCells + 63) / 64), block(64);
+    const dim3 grid((cells->numberOfCells + kScaleCells - 1) / kScaleCells), block(kScaleBlock);
     hipStream_t st = (hipStream_t)stream;
 #define VF_LAUNCH(S)                                                                                                 \
     if (adjoint && fine_to_coarse)                                                                                   \

@@ -101,7 +101,11 @@ class Lattice:
         p.storage = code
         p.isEvenTimestep = 1
         p.numberOfNodes = N
-        p.neighborX, p.neighborY, p.neighborZ = _ptr(self.nbx), _ptr(self.nby), _ptr(self.nbz)
+        # a box lattice addresses every kernel (BCs, interface, reductions) by (x, y, z): its
+        # neighbour arrays are the periodic box's, and recomputing them saves two dependent loads
+        # per node; the indexed lattice hands the arrays to every kernel
+        if not self.box:
+            p.neighborX, p.neighborY, p.neighborZ = _ptr(self.nbx), _ptr(self.nby), _ptr(self.nbz)
         p.typeOfGridNode = _ptr(self.geo)
         p.nx, p.ny, p.nz = grid.nx, grid.ny, grid.nz
         p.fluidMask = _ptr(self.mask)
