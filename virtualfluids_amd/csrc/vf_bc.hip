@@ -33,7 +33,11 @@ __global__ __launch_bounds__(256) void bc_post_kernel(S* __restrict__ f, Addr a,
     const bool even = even_flag != 0;
     uint32_t n[8];
     cell_nodes(a, (uint32_t)b.k[i], n);
-    float post[27];
+    // every load issued before the first use: the link distances decide the branches below,
+    // and loading each inside its branch would serialise 26 memory latencies
+    float qd[27], post[27];
+#pragma unroll
+    for (int d = 1; d < 27; ++d) qd[d] = b.q[(uint64_t)d * b.nbc + i];
 #pragma unroll
     for (int d = 0; d < 27; ++d) post[d] = read_post(f, a, n, d, even);
 
@@ -62,7 +66,7 @@ __global__ __launch_bounds__(256) void bc_post_kernel(S* __restrict__ f, Addr a,
     const float omega = b.omega;
 #pragma unroll
     for (int d = 1; d < 27; ++d) {
-        const float q = b.q[(uint64_t)d * b.nbc + i];
+        const float q = qd[d];
         if (!(q >= 0.0f && q <= 1.0f)) continue;
         const int r = inv(d);
         const float w = weight(d);

@@ -55,19 +55,27 @@ __global__ __launch_bounds__(256) void bc_post_adjoint_kernel(const S* __restric
     }
     MacroOut mo;
     k17_collide(post, b.rates, mo, b.porosity ? 1.0f - 0.5f * b.porosity[b.k[i]] : 1.0f);
-    // adjoint of the written cells (incoming direction inv(d), next parity), then cleared
+    // adjoint of the written cells (incoming direction inv(d), next parity), then cleared; the
+    // 27 candidate cells are loaded unconditionally (every one is a valid cell) so that the
+    // loads overlap instead of waiting one by one behind the link tests
+#pragma unroll
+    for (int d = 0; d < 27; ++d) G[inv(d)] = *lam_cell(lam, a, n, inv(d), !even);
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
-        G[inv(d)] = 0.0f;
-        if (!(q[d] >= 0.0f && q[d] <= 1.0f)) continue;
-        float* W = lam_cell(lam, a, n, inv(d), !even);
-        G[inv(d)] = *W;
-        *W = 0.0f;
+        if (!(q[d] >= 0.0f && q[d] <= 1.0f)) {
+            G[inv(d)] = 0.0f;
+            continue;
+        }
+        *lam_cell(lam, a, n, inv(d), !even) = 0.0f;
     }
     bc_post_vjp<KIND>(post, q, b.omega, G, fb, Ub);
-    // the post-collision value e sits in the cell of pre(inv(e))
+    // the post-collision value e sits in the cell of pre(inv(e)); the 27 cells are distinct, so
+    // all of them are read before any is written (one memory latency instead of 27)
+    float acc[27];
 #pragma unroll
-    for (int e = 0; e < 27; ++e) *lam_cell(lam, a, n, inv(e), even) += fb[e];
+    for (int e = 0; e < 27; ++e) acc[e] = *lam_cell(lam, a, n, inv(e), even);
+#pragma unroll
+    for (int e = 0; e < 27; ++e) *lam_cell(lam, a, n, inv(e), even) = acc[e] + fb[e];
     if (b.ubar) {
         b.ubar[i] += Ub[0];
         b.ubar[b.nbc + i] += Ub[1];
@@ -98,12 +106,16 @@ __global__ __launch_bounds__(256) void pressure_adjoint_kernel(const S* __restri
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
         fn[d] = read_pre(f, a, nn, d, even);
+        G[d] = *lam_cell(lam, a, n, d, even); // loaded unconditionally (valid cells), see above
+    }
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
         const int c = b.axis == 0 ? cx(d) : (b.axis == 1 ? cy(d) : cz(d));
-        G[d] = 0.0f;
-        if (c * b.sign > 0) continue;
-        float* cell = lam_cell(lam, a, n, d, even);
-        G[d] = *cell;
-        *cell = 0.0f;
+        if (c * b.sign > 0) {
+            G[d] = 0.0f;
+            continue;
+        }
+        *lam_cell(lam, a, n, d, even) = 0.0f;
     }
     pressure_vjp(fn, b.rhoBC[i], G, fnb, rb);
 #pragma unroll

@@ -209,13 +209,13 @@ __global__ __launch_bounds__(kScaleBlock) void scale_cf_adjoint_kernel(const S* 
     if (cl.valid) {
         uint32_t nn[8];
         cell_nodes(af, corner(af, s.cellsF[cl.i], q), nn);
+        // the 27 cells are distinct: read them all, then clear them (a clear between the reads
+        // would make every read wait for the previous one)
         float g[27];
 #pragma unroll
-        for (int d = 0; d < 27; ++d) {
-            float* w = adj_cell(lamF, af, nn, d, true);
-            g[d] = *w;
-            *w = 0.f;
-        }
+        for (int d = 0; d < 27; ++d) g[d] = *adj_cell(lamF, af, nn, d, true);
+#pragma unroll
+        for (int d = 0; d < 27; ++d) *adj_cell(lamF, af, nn, d, true) = 0.f;
         const float x = cf_child_offset(q, 0), y = cf_child_offset(q, 1), z = cf_child_offset(q, 2);
         const NodeMoments nb = rebuild_vjp(cf_node_moments(c, x, y, z, 0.5f, s.omegaF), g);
         cf_node_moments_vjp(c, x, y, z, 0.5f, s.omegaF, nb, cb);
