@@ -25,8 +25,14 @@ def _grid(kind):
         return periodic_box(24, 10, 8)
     if kind == "walled":
         return walled_box((22, 8, 6), {})
-    # rows of four waves, the last one partial (the row kernel's LDS hand-over)
-    return walled_box((228, 4, 3), {}) if kind == "long_walled" else periodic_box(200, 4, 3)
+    # long_walled: rows of four waves, the last one partial (the row kernel's LDS hand-over);
+    # long_periodic and split_walled: 130 = 128 + 2 and 258 = 256 + 2, whole waves in the row
+    # kernel and two tail columns in the scalar kernel
+    if kind == "long_walled":
+        return walled_box((228, 4, 3), {})
+    if kind == "split_walled":
+        return walled_box((256, 3, 2), {})
+    return periodic_box(130, 4, 3)
 
 
 # box: the row kernel (buffer addressing); box_global: its global-address form; box_scalar:
@@ -37,14 +43,14 @@ ADJ_ENV = {"box": {}, "box_global": {"VFAMD_ROW_BUFFER": "0"}, "box_scalar": {"V
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("path", ["box", "box_global", "box_scalar", "indexed"])
 @pytest.mark.parametrize("even", [True, False])
-@pytest.mark.parametrize("kind", ["periodic", "walled", "long_periodic", "long_walled"])
+@pytest.mark.parametrize("kind", ["periodic", "walled", "long_periodic", "long_walled", "split_walled"])
 def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, monkeypatch, storage, path, even, kind):
     import torch
 
     from virtualfluids_amd import capi
     from virtualfluids_amd.lattice import Lattice
 
-    if kind.startswith("long") and path in ("box_scalar", "indexed") and storage != "fp32":
+    if kind.startswith(("long", "split")) and path in ("box_scalar", "indexed") and storage != "fp32":
         pytest.skip("long rows exercise the row kernel; the other paths are covered on the short boxes")
     for k, v in ADJ_ENV[path].items():
         monkeypatch.setenv(k, v)

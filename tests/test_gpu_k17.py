@@ -214,3 +214,47 @@ def test_out_of_place_step_matches_in_place(gpu, monkeypatch, storage, dims, sol
         L.collide()
         torch.cuda.synchronize()
         assert torch.equal(out, L.f), f"{storage} {dims} solid={solid} even={even}"
+
+
+# rows whose last wave carries a few lanes (130 at one node per lane; 258 and 260 at two; 194),
+# with solid walls, a porosity field, in place and out of place
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("case", ["periodic130", "periodic260", "walled258", "walled194_porous"])
+def test_partial_rows_match_oracle(gpu, orc, storage, case):
+    import ctypes
+
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.adjoint import _param
+    from virtualfluids_amd.grid import walled_box
+    from virtualfluids_amd.lattice import stream_handle
+
+    grid = {"periodic130": lambda: periodic_box(130, 4, 3), "periodic260": lambda: periodic_box(260, 3, 2),
+            "walled258": lambda: walled_box((256, 3, 2), {}),
+            "walled194_porous": lambda: walled_box((192, 3, 3), {})}[case]()
+    rng = np.random.default_rng(230 + len(case))
+    f0 = near_equilibrium(rng, grid.N, u_amp=0.04, noise=5e-3)
+    phi = rng.uniform(0, 0.1, grid.N) if "porous" in case else None
+    for even in (True, False):
+        L = _lattice(grid, 1.9, storage=storage)
+        if phi is not None:
+            L.set_porosity(torch.from_numpy(phi.astype(np.float32)).cuda())
+        L.even = even
+        L.set_populations(pack_pre(f0, grid, even))
+        stored = L.populations().astype(np.float64)
+        ref = _oracle_lattice(orc, grid)
+        ref.even = even
+        ref.f[:] = stored
+        ref.collide(grid.fluid, 1.9, porosity=None if phi is None else phi.astype(np.float32).astype(np.float64))
+        out = torch.empty_like(L.f)
+        capi.call("vf_k17_run_box_into", ctypes.byref(_param(L, L.f, even)), ctypes.c_void_p(out.data_ptr()),
+                  stream_handle())
+        L.collide()
+        torch.cuda.synchronize()
+        got = L.populations()
+        tol = {"fp32": 2e-7, "fp16": 0.06 * 2.0 ** -11 + 2e-7, "bf16": 0.06 * 2.0 ** -8 + 2e-7}[storage]
+        err = np.abs(got - ref.f).max()
+        print(f"{case} {storage} even={even} nx={grid.nx} err={err:.3e}")
+        assert err < tol, err
+        assert torch.equal(out, L.f), "out-of-place step differs from the in-place step"

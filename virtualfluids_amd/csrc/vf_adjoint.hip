@@ -50,7 +50,28 @@ struct AdjBoxArgs {
     float* shiftAdj;       // 3*N or null
     const float* porosity; // Brinkman porosity or null
     float* porosityAdj;    // N or null
+    uint32_t rowEnd = 0;   // row kernel: columns [0, rowEnd) (0: the whole row)
+    uint32_t x0 = 0;       // scalar kernel: 0 for every node, else the columns [x0, nx) only
 };
+
+// (x, y, z) of this thread's node in the scalar kernel; false for threads beyond the nodes
+__device__ __forceinline__ bool adj_coords(const AdjBoxArgs& a, int& x, int& y, int& z)
+{
+    if (a.x0 == 0) {
+        x = blockIdx.x * blockDim.x + threadIdx.x;
+        y = blockIdx.y;
+        z = blockIdx.z;
+        return x < a.nx;
+    }
+    const uint32_t tw = (uint32_t)a.nx - a.x0;
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= tw * (uint32_t)a.ny * (uint32_t)a.nz) return false;
+    const uint32_t r = id / tw;
+    x = (int)(a.x0 + id % tw);
+    y = (int)(r % (uint32_t)a.ny);
+    z = (int)(r / (uint32_t)a.ny);
+    return true;
+}
 
 // adjoint of the velocity scale s = 1 - porosity/2 goes to the porosity with the factor -1/2
 __device__ __forceinline__ void brinkman_adjoint(const float* porosity, float* porosityAdj, uint32_t k, float sb)
@@ -62,10 +83,8 @@ template <class S, bool EVEN, bool MASK, bool SHIFT>
 __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __restrict__ f, float* __restrict__ lam,
                                                                     AdjBoxArgs a, K17Rates r)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int z = blockIdx.z;
-    if (x >= a.nx) return;
+    int x, y, z;
+    if (!adj_coords(a, x, y, z)) return;
     const int xp = (x + 1 == a.nx) ? 0 : x + 1;
     const int yp = (y + 1 == a.ny) ? 0 : y + 1;
     const int zp = (z + 1 == a.nz) ? 0 : z + 1;
@@ -108,14 +127,18 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
     __shared__ float xlast[9];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
-    const bool active = threadIdx.x < nx;
-    const bool lastLane = threadIdx.x + 1 == nx;
+    // split: the kernel covers the columns [0, rowEnd) and a tail launch of the scalar kernel
+    // the rest; the cells either side of that seam are read and written one by one
+    const uint32_t rowEnd = a.rowEnd ? a.rowEnd : nx;
+    const bool split = rowEnd < nx;
+    const bool active = threadIdx.x < rowEnd;
+    const bool lastLane = threadIdx.x + 1 == rowEnd;
     const uint32_t y = blockIdx.x, z = blockIdx.y;
     const uint32_t yp = (y + 1 == ny) ? 0 : y + 1;
     const uint32_t zp = (z + 1 == (uint32_t)a.nz) ? 0 : z + 1;
     // idle lanes read the row's last node (so that every load is unconditional and in bounds)
     // and drop their results
-    const uint32_t x = active ? threadIdx.x : nx - 1;
+    const uint32_t x = active ? threadIdx.x : rowEnd - 1;
     // this lane's node in the 4 rows its cells touch, indexed by the y/z bits of node_sel
     const uint32_t rows[4] = {nx * (y + ny * z) + x, nx * (yp + ny * z) + x, nx * (y + ny * zp) + x,
                               nx * (yp + ny * zp) + x};
@@ -143,6 +166,13 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
         const uint32_t off = rows[node_sel(d) >> 1] * (uint32_t)sizeof(float);
         if constexpr (BUF) return LO::load(slot<EVEN>(d) >= kRowSplit ? lHi : lLo, off, soff(d, slotL));
         else return LO::load(reinterpret_cast<const float*>(reinterpret_cast<const char*>(lam) + slot<EVEN>(d) * slotL + off));
+    };
+    // direction d's cell at column cx of this lane's row (split rows only)
+    auto cellS = [&](int d, uint32_t cx) {
+        return f + (uint64_t)slot<EVEN>(d) * a.N + (rows[node_sel(d) >> 1] - x + cx);
+    };
+    auto cellL = [&](int d, uint32_t cx) {
+        return lam + (uint64_t)slot<EVEN>(d) * a.N + (rows[node_sel(d) >> 1] - x + cx);
     };
     auto storeL = [&](int d, float val) {
         const uint32_t off = rows[node_sel(d) >> 1] * (uint32_t)sizeof(float);
@@ -176,12 +206,21 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
         nxtL[j] = __shfl(LO::elem(rawL[kShiftedDirs[j]], 0), (lane + 1) & 63, 64);
     }
     if (active && (lane == 63 || lastLane)) {
-        // the next wave's first node, or the row's first node around the periodic wrap
-        const int src = lastLane ? 0 : wave + 1;
+        if (lastLane && split) {
+            // the first tail column
 #pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            nxtS[j] = xinS[src][j];
-            nxtL[j] = xinL[src][j];
+            for (int j = 0; j < 9; ++j) {
+                nxtS[j] = IO::elem_bits(cellS(kShiftedDirs[j], rowEnd));
+                nxtL[j] = *cellL(kShiftedDirs[j], rowEnd);
+            }
+        } else {
+            // the next wave's first node, or the row's first node around the periodic wrap
+            const int src = lastLane ? 0 : wave + 1;
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                nxtS[j] = xinS[src][j];
+                nxtL[j] = xinL[src][j];
+            }
         }
     }
 
@@ -231,8 +270,14 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
         for (int j = 0; j < 9; ++j) xout[wave][j] = fb[kShiftedDirs[j]];
     }
     if (lastLane) {
+        if (split) {
+            // into the first tail column, which the tail launch does not write for these slots
 #pragma unroll
-        for (int j = 0; j < 9; ++j) xlast[j] = fb[kShiftedDirs[j]];
+            for (int j = 0; j < 9; ++j) *cellL(kShiftedDirs[j], rowEnd) = fb[kShiftedDirs[j]];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 9; ++j) xlast[j] = fb[kShiftedDirs[j]];
+        }
     }
     __syncthreads();
     float s0[9];
@@ -241,7 +286,8 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
         s0[j] = __shfl(fb[kShiftedDirs[j]], (lane + 63) & 63, 64);
         if (lane == 0) s0[j] = (wave == 0) ? xlast[j] : xout[wave - 1][j];
     }
-    if (active) {
+    // split: column 0 of these slots belongs to the last tail node
+    if (active && !(split && threadIdx.x == 0)) {
 #pragma unroll
         for (int j = 0; j < 9; ++j) storeL(kShiftedDirs[j], s0[j]);
     }
@@ -310,8 +356,8 @@ int check_adjoint_args(const vf_lbm_parameter* p, const float* adjoint)
     return VF_OK;
 }
 
-// VFAMD_ADJ_ROW=0 selects the scalar box kernel and VFAMD_ROW_BUFFER=0 the global-address form
-// of the row kernel (used by the tests to cover every form)
+// VFAMD_ADJ_ROW=0 selects the scalar box kernel, VFAMD_ROW_BUFFER=0 the global-address form of
+// the row kernel and VFAMD_ROW_SPLIT=0 whole rows (used by the tests to cover every form)
 bool env_off(const char* name)
 {
     const char* e = std::getenv(name);
@@ -321,7 +367,7 @@ bool env_off(const char* name)
 template <class S, bool EVEN, bool BUF>
 void launch_row_b(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift, hipStream_t s)
 {
-    const dim3 grid(a.ny, a.nz), block(((a.nx + 63) / 64) * 64);
+    const dim3 grid(a.ny, a.nz), block(((a.rowEnd + 63) / 64) * 64);
     if (mask && shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, true, BUF>), grid, block, 0, s, f, lam, a, r);
     else if (mask) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, false, BUF>), grid, block, 0, s, f, lam, a, r);
     else if (shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, false, true, BUF>), grid, block, 0, s, f, lam, a, r);
@@ -335,17 +381,34 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj
     const K17Rates r = make_k17_rates(p);
     // A row is one workgroup of ceil(nx / 64) waves. At ~245 VGPRs a CU holds 8 waves of this
     // kernel, so only rows of 1, 2, 4 or 8 waves fill it (a 5-wave row of 258 nodes leaves 3 of
-    // 8 wave slots empty and runs 1.5x slower than the scalar kernel); other widths take the
+    // 8 wave slots empty and runs 1.5x slower than the scalar kernel). A row whose last wave
+    // would carry fewer than 32 lanes is split: the whole waves take the row kernel and the
+    // columns beyond a tail launch of the scalar kernel (258 = 256 + 2). Other widths take the
     // scalar kernel.
-    const int waves = (p.nx + 63) / 64;
-    if (p.nx <= kAdjRowMaxThreads && 8 % waves == 0 && !env_off("VFAMD_ADJ_ROW")) {
+    uint32_t rowEnd = (uint32_t)p.nx;
+    if (p.nx > 64 && p.nx % 64 != 0 && p.nx % 64 < 32 && !env_off("VFAMD_ROW_SPLIT")) rowEnd = p.nx - p.nx % 64;
+    const int waves = (int)((rowEnd + 63) / 64);
+    if (rowEnd <= (uint32_t)kAdjRowMaxThreads && 8 % waves == 0 && !env_off("VFAMD_ADJ_ROW")) {
         // the buffer form needs kRowSplit slots of the adjoint (the larger element) within
         // 32-bit offsets
         const S* f = static_cast<const S*>(p.distributions);
+        AdjBoxArgs ar = a;
+        ar.rowEnd = rowEnd;
         if ((uint64_t)kRowSplit * a.N * sizeof(float) <= 0xffffffffull && !env_off("VFAMD_ROW_BUFFER"))
-            launch_row_b<S, EVEN, true>(a, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
+            launch_row_b<S, EVEN, true>(ar, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
         else
-            launch_row_b<S, EVEN, false>(a, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
+            launch_row_b<S, EVEN, false>(ar, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
+        if (rowEnd == (uint32_t)p.nx) return;
+        // the tail columns, one thread per node over all rows
+        AdjBoxArgs at = a;
+        at.x0 = rowEnd;
+        const uint64_t n = (uint64_t)(p.nx - rowEnd) * p.ny * p.nz;
+        const dim3 tg((unsigned)((n + kAdjBlock - 1) / kAdjBlock)), tb(kAdjBlock);
+        const bool mask = p.fluidMask != nullptr;
+        if (mask && shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, true>), tg, tb, 0, s, f, lam, at, r);
+        else if (mask) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, false>), tg, tb, 0, s, f, lam, at, r);
+        else if (shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, true>), tg, tb, 0, s, f, lam, at, r);
+        else hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, false>), tg, tb, 0, s, f, lam, at, r);
         return;
     }
     const int bx = p.nx >= kAdjBlock ? kAdjBlock : ((p.nx + 63) / 64) * 64;

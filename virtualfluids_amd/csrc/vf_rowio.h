@@ -68,6 +68,13 @@ template <class S, int V> struct RowIO {
     // the word holding element 0 (what the previous lane needs for its shifted directions)
     __device__ static unsigned word0(const Raw& r) { return word<BYTES>(r, 0); }
 
+    // one element's bits in the low part of a word (of_word(w, 0) reads it back)
+    __device__ static unsigned elem_bits(const S* p)
+    {
+        if constexpr (sizeof(S) == 4) return *reinterpret_cast<const unsigned*>(p);
+        else return *reinterpret_cast<const unsigned short*>(p);
+    }
+
     // cache policy of the buffer forms: non-temporal (each population is touched once per step)
     static constexpr int kNT = 2;
 
