@@ -64,7 +64,9 @@ enum vf_collision_template {
  * Addressing: neighborX/Y/Z hold the +x/+y/+z neighbour of every node
  * (Parameter.h:84-86). When nx*ny*nz == numberOfNodes the node index is
  * k = x + nx*(y + ny*z) of a periodic box and vf_k17_run_box() recomputes the
- * neighbours instead of loading them.
+ * neighbours instead of loading them; every other entry point except the indexed
+ * collision (vf_k17_run, vf_k17_run_on_indices) and its adjoint (vf_k17_adjoint) also takes
+ * NULL neighbour arrays for such a box and then recomputes them too.
  */
 typedef struct vf_lbm_parameter {
     void* distributions;            /* 27*numberOfNodes values of type `storage` */
@@ -94,7 +96,7 @@ typedef struct vf_lbm_parameter {
     /* Brinkman porosity per node (NULL: none; no counterpart in the reference): a drag
      * acceleration -porosity * u applied through the collision's half-step velocity shift,
      * so the collision velocity becomes u (1 - porosity/2). Honoured by every collision and
-     * adjoint entry point (the dense-box FP32 row kernel falls back to the box kernel). */
+     * adjoint entry point. */
     const float* porosity;
 } vf_lbm_parameter;
 
