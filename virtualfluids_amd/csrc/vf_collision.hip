@@ -148,7 +148,6 @@ template <class S, int V, bool EVEN, bool MASK, bool MACRO, bool BUF, bool INTO 
 __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, BoxArgs a, K17Rates r)
 {
     using IO = RowIO<S, V>;
-    constexpr bool MIX = Codec<S>::storage == VF_STORAGE_FP16;
     __shared__ unsigned xin[kRowMaxThreads / 64][9];
     __shared__ float xout[kRowMaxThreads / 64][9];
     __shared__ float xlast[9];
@@ -247,7 +246,7 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, 
             MacroOut mo;
             // Brinkman porosity: the velocity scale 1 - porosity / 2 (1 without a field)
             const float vs = a.porosity ? 1.0f - 0.5f * a.porosity[rows[0] + i] : 1.0f;
-            k17_collide<MIX>(fl, r, mo, vs);
+            k17_collide(fl, r, mo, vs);
             if (MACRO) {
                 a.rho[rows[0] + i] = mo.drho;
                 a.vx[rows[0] + i] = mo.vx;

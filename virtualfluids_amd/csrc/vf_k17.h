@@ -32,7 +32,6 @@ struct K17Rates {
     float A4;            // -(1/w - 1/2) (2/3) factorA: diagonal 4th-order equilibrium per unit rho*du
     float B4;            // -(1/w - 1/2) (1/3) factorB * (-3w): off-diagonal 4th order per unit m_ab
     float fx, fy, fz;    // half-step velocity shift (forces * 1/2 * forceFactor)
-    float one;           // 1, as a launch argument: a * one + c lets FP16 sources fold into v_fma_mix_f32
 };
 
 // The collision helpers compile for the host as well (the CPU tests evaluate the same code).
@@ -128,7 +127,6 @@ inline K17Rates make_k17_rates(const vf_lbm_parameter& p)
     r.fx = p.forces[0] * 0.5f * p.forceFactor;
     r.fy = p.forces[1] * 0.5f * p.forceFactor;
     r.fz = p.forces[2] * 0.5f * p.forceFactor;
-    r.one = 1.0f;
     return r;
 }
 
@@ -138,9 +136,6 @@ struct MacroOut {
 
 // f: 27 shifted pre-collision populations in D3Q27 order; replaced by post-collision values.
 // vscale: the Brinkman factor 1 - porosity/2 that scales the collision velocity (1: none)
-// MIX: the first sums are written a * one + c (bit-identical to a + c), so that values widened
-// from FP16 feed v_fma_mix_f32 directly and need no conversion instruction of their own.
-template <bool MIX = false>
 VF_HD void k17_collide(float (&f)[27], const K17Rates& r, MacroOut& mo, float vscale = 1.0f)
 {
     float& fMMM = f[dMMM]; float& f0MM = f[d0MM]; float& fPMM = f[dPMM];
@@ -159,17 +154,10 @@ VF_HD void k17_collide(float (&f)[27], const K17Rates& r, MacroOut& mo, float vs
     float eMM, dMM, sMM, eM0, dM0, sM0, eMP, dMP, sMP;
     float e0M, d0M, s0M, e00, d00, s00, e0P, d0P, s0P;
     float ePM, dPM, sPM, eP0, dP0, sP0, ePP, dPP, sPP;
-    const float one = r.one;
-    auto zsum = [one](float a, float b, float c, float& e, float& d, float& s) {
-        if (MIX) {
-            e = __builtin_fmaf(a, one, c);
-            d = __builtin_fmaf(-a, one, c);
-            s = __builtin_fmaf(b, one, e);
-        } else {
-            e = a + c;
-            d = c - a;
-            s = e + b;
-        }
+    auto zsum = [](float a, float b, float c, float& e, float& d, float& s) {
+        e = a + c;
+        d = c - a;
+        s = e + b;
     };
     zsum(fMMM, fMM0, fMMP, eMM, dMM, sMM);
     zsum(fM0M, fM00, fM0P, eM0, dM0, sM0);

@@ -1,12 +1,12 @@
 // Aligned, non-temporal vector loads and stores of V consecutive populations of one
-// direction array. Used by the row kernel, where every lane owns V consecutive x-nodes and
+// direction array. Used by the row kernels, where every lane owns V consecutive x-nodes and
 // every global access of a wave is aligned and contiguous (the x-shift of the eso-twist
 // layout is resolved with lane shuffles).
 //
-// Loads keep the raw lane word; elements are widened to FP32 where they are used, so FP16
-// values can enter v_fma_mix_f32 without a conversion instruction. Stores narrow two
-// values per instruction (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32, round to nearest even,
-// the rounding of Codec<S>::store).
+// Loads keep the raw lane word (the word, not a widened value, is what lanes trade for the
+// shifted directions); elements are widened to FP32 where they are used. Stores narrow two
+// values per instruction (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32, round to nearest even, the
+// rounding of Codec<S>::store).
 #pragma once
 
 #include "vf_common.h"
