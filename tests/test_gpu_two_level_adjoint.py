@@ -5,7 +5,7 @@ and a random porosity field on the fine level, started from a uniform flow. Obje
 kinetic energy of a coarse plane downstream of the fine box, averaged over the last coarse
 steps. Its gradients with respect
 to the inflow velocities and the fine porosity match FP64 central differences of the oracle's
-two-level run (the north star's 1e-4 bound); they do not depend on the checkpoint interval.
+two-level run (the stated 1e-4 bound); they do not depend on the checkpoint interval.
 """
 import numpy as np
 import pytest
