@@ -44,7 +44,8 @@ def _channel_lattice(box, storage="fp32"):
 def test_single_level_replay_matches_eager_steps(gpu, box):
     import torch
 
-    from virtualfluids_amd.adjoint import Stepper, _StepGraphs
+    from virtualfluids_amd.adjoint import Stepper
+    from virtualfluids_amd.graphs import StepGraphs as _StepGraphs
 
     L = _channel_lattice(box)
     st = Stepper(L, _factory())
@@ -81,7 +82,8 @@ def _pair(storage="fp16"):
 def test_two_level_replay_matches_eager_steps(gpu, storage):
     import torch
 
-    from virtualfluids_amd.adjoint import Stepper, _StepGraphs, _two_level_step_in_place
+    from virtualfluids_amd.adjoint import Stepper, _two_level_step_in_place
+    from virtualfluids_amd.graphs import StepGraphs as _StepGraphs
     from virtualfluids_amd.refinement import DeviceInterfaceCells
 
     nb, Lc, Lf = _pair(storage)
@@ -139,3 +141,22 @@ def test_two_level_steps_same_with_and_without_graphs(gpu):
     for a, b in zip(*out):
         assert torch.equal(a, b)
     assert float(out[0][2]) > 0 and float(out[0][3].abs().sum()) > 0
+
+
+@pytest.mark.parametrize("box", [True, False])
+def test_simulation_run_same_with_and_without_graphs(gpu, box):
+    """Simulation.run replays one graph per parity: chunks of steps (eager work and a
+    synchronisation between them) leave the same state as host-launched steps."""
+    import torch
+
+    from virtualfluids_amd.simulation import Simulation
+
+    A, B = _channel_lattice(box), _channel_lattice(box)
+    sa, sb = Simulation(A, _factory()), Simulation(B, _factory())
+    for chunk in range(4):
+        sa.run(7, graphs=True)
+        sb.run(7, graphs=False)
+        torch.cuda.synchronize()
+        assert A.even == B.even and sa.t == sb.t
+        assert torch.equal(A.f, B.f), f"chunk {chunk}"
+        _ = A.f.float().abs().sum().item()

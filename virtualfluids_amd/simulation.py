@@ -110,7 +110,8 @@ class Simulation:
                 raise RuntimeError(f"The boundary condition {label} was not set!")
         self.t = 0
 
-    def step(self, stream=None):
+    def _launch(self, stream=None):
+        """The launches of one step (UpdateGrid27::updateGrid for one level); swaps the parity."""
         L = self.lattice
         L.collide(stream=stream)
         if self.velocity_fn:
@@ -122,11 +123,34 @@ class Simulation:
         L.swap()
         if self.pressure_fn:
             L.apply_bc(self.pressure_fn, "pressure", stream)
+
+    def step(self, stream=None):
+        self._launch(stream)
         self.t += 1
 
-    def run(self, steps: int, stream=None):
+    def run(self, steps: int, stream=None, graphs: bool = True):
+        """`steps` time steps. With graphs (and no explicit stream) each parity's launches are
+        captured once and replayed (virtualfluids_amd.graphs.StepGraphs): the same kernels
+        with the same arguments, without a host round trip per launch; graphs=False launches
+        every kernel from the host."""
+        if not graphs or stream is not None or steps < 3:
+            for _ in range(steps):
+                self.step(stream)
+            return
+        from .graphs import StepGraphs
+
+        L = self.lattice
+
+        def launch(even, st):
+            L.even = even
+            self._launch(st)
+
+        run = StepGraphs(launch)
         for _ in range(steps):
-            self.step(stream)
+            even = L.even
+            run(even)
+            L.even = not even  # the step's parity swap (host state; a replay does not run it)
+            self.t += 1
 
 
 class MultiLevelSimulation:
