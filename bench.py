@@ -252,6 +252,56 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
     return out
 
 
+# Time-to-solution and NUPS the reference publishes for its GPU DrivenCavity app (K17, 10,000
+# steps; VirtualFluids.pdf p.12, Table 4), in seconds and millions of node updates per second
+PUBLISHED_CAVITY = {"NVIDIA A100-PCIE-40GB": (1.73, 1658.6), "Tesla V100-SXM2-32GB": (1.47, 1950.1),
+                    "Tesla V100-PCIE-16GB": (1.44, 2001.5), "NVIDIA RTX 5090": (0.27, 10852.3)}
+
+
+def published_cavity(steps=10000):
+    """The reference's published GPU benchmark, its DrivenCavity app
+    (apps/gpu/DrivenCavity/DrivenCavity.cpp:68-162): K17, 64 nodes per side (66^3 nodes with the
+    wall layer, the node count its NUPS figures imply), lid on z-max moving diagonally at 0.05
+    (vx = vy = 0.05 / sqrt 2), Re 1000 (nu = 64 * 0.05 / 1000), NoSlipBounceBack and
+    VelocityBounceBack, FP32, 10,000 steps through Simulation.run (HIP-graph replay), no
+    output files. Timed from the first step to the last, initialisation excluded."""
+    import numpy as np
+    import torch
+
+    from virtualfluids_amd.grid import d0M0, d0P0, d00M, d00P, dM00, dP00, walled_box
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory, Simulation
+
+    n, u_lid, re = 64, 0.05, 1000.0
+    walls = {s: ("noslip",) for s in (dP00, dM00, d0P0, d0M0, d00M)}
+    walls[d00P] = ("velocity", (u_lid / np.sqrt(2.0), u_lid / np.sqrt(2.0), 0.0))
+    g = walled_box((n, n, n), walls)
+    nu = n * u_lid / re
+    F = BoundaryConditionFactory
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityBounceBack)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipBounceBack)
+    L = Lattice(g, 1.0 / (3.0 * nu + 0.5))
+    z = np.zeros(g.N)
+    L.init_equilibrium(z, z, z, z)
+    sim = Simulation(L, fac)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    L.macroscopic()
+    umax = float(torch.sqrt(L.vx ** 2 + L.vy ** 2 + L.vz ** 2).max())
+    out = {"case": "K17 lid-driven cavity, 64 nodes per side + wall layer (66^3), diagonal lid 0.05, Re 1000, "
+                   "NoSlipBounceBack + VelocityBounceBack, FP32, 10,000 steps",
+           "nodes": int(g.N), "steps": steps, "time_to_solution_s": round(t, 4),
+           "MNUPS": round(g.N * steps / t / 1e6, 1), "max_speed": round(umax, 5),
+           "published": {k: {"time_to_solution_s": v[0], "MNUPS": v[1]} for k, v in PUBLISHED_CAVITY.items()}}
+    del sim, L
+    torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth():
     """Device-to-device copy of 2 GiB, (read + write bytes) / time, by two kernels: the
     library's aligned non-temporal 16-byte copy (vf_stream_copy) and the runtime's blit copy
@@ -346,6 +396,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200, help="timed adjoint kernel steps")
     ap.add_argument("--no-sphere", action="store_true", help="skip the full-size sphere-channel gradient")
     ap.add_argument("--no-two-level", action="store_true", help="skip the full-size two-level gradient")
+    ap.add_argument("--no-cavity", action="store_true", help="skip the reference's published cavity benchmark")
     ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
                     help="time one storage type only (for profiling runs)")
     args = ap.parse_args()
@@ -422,6 +473,7 @@ def main():
         "adjoint": adjoint,
         "sphere_channel_gradient": None if (args.no_sphere or args.no_adjoint) else sphere_channel_gradient(),
         "two_level_gradient": None if (args.no_two_level or args.no_adjoint) else two_level_gradient(),
+        "published_cavity": None if args.no_cavity else published_cavity(),
         "hbm_copy_GBps": round(copy_gbps, 1),
         "blit_copy_GBps": round(blit_gbps, 1),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),

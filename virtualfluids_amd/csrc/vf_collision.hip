@@ -371,27 +371,45 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
 // near the VALU issue limit), else 2. Rows longer than one workgroup's lanes take the next width
 // that fits; rows no width fits take the scalar kernel.
 // VFAMD_ROW_WIDTH overrides the choice (0, 1, 2 or 4) for benchmarking.
+// VFAMD_ROW_PAIRS=0 keeps one FP32 node per lane (benchmarking)
+static bool row_pairs_allowed()
+{
+    const char* e = std::getenv("VFAMD_ROW_PAIRS");
+    return !(e && *e == '0');
+}
+
 template <class S> static int row_width(int nx)
 {
     const char* e = std::getenv("VFAMD_ROW_WIDTH");
     if (e && *e) return std::atoi(e);
     auto fits = [nx](int v) { return nx % v == 0 && nx / v <= kRowMaxThreads; };
-    if (Codec<S>::storage == VF_STORAGE_FP32) return fits(1) ? 1 : fits(2) ? 2 : fits(4) ? 4 : 0;
+    auto waves = [nx](int v) { return (nx / v + 63) / 64; };
+    if (Codec<S>::storage == VF_STORAGE_FP32) {
+        // two nodes per lane where one node per lane leaves a last wave of a few lanes (66
+        // nodes: one wave of 33 lanes instead of a full wave and one of 2 lanes)
+        const int rem = (nx % 128) % 64;
+        if (fits(1) && fits(2) && rem != 0 && rem < 32 && waves(2) < waves(1) && row_pairs_allowed()) return 2;
+        return fits(1) ? 1 : fits(2) ? 2 : fits(4) ? 4 : 0;
+    }
     if (nx % 256 == 0 && fits(4)) return 4;
     return fits(2) ? 2 : fits(4) ? 4 : 0;
 }
 
 // Resident workgroups per CU for the row kernel (0: as many as registers allow). The FP32
-// kernel moves the most bytes per instruction and runs best with few resident rows: at 256^3
-// on MI355X, 8 resident 4-wave workgroups per CU give 5.72 TB/s, 3-4 give 5.84-5.86 and 2
-// give 5.92-5.93 (fewer concurrent direction streams per memory channel); one gives 3.8. The 16-bit
-// kernels are bound by VALU issue and need every wave they can get. The cap is imposed by
-// reserving LDS at launch. VFAMD_ROW_WGS overrides it for benchmarking.
-template <class S, int V, bool BUF> static int row_wgs_per_cu()
+// kernel moves the most bytes per instruction and runs best with few resident rows when it
+// streams from HBM: at 256^3 on MI355X, 8 resident 4-wave workgroups per CU give 5.72 TB/s, 3-4
+// give 5.84-5.86 and 2 give 5.92-5.93 (fewer concurrent direction streams per memory channel);
+// one gives 3.8. A lattice that fits the 256 MB infinity cache is latency-bound instead and
+// takes every wave (the 66^3 cavity: see the launch below). The 16-bit kernels are bound by
+// VALU issue and need every wave they can get. The cap is imposed by reserving LDS at launch.
+// VFAMD_ROW_WGS overrides it for benchmarking.
+constexpr uint64_t kCacheResidentBytes = 256ull << 20;
+template <class S, int V, bool BUF> static int row_wgs_per_cu(uint64_t N)
 {
     const char* e = std::getenv("VFAMD_ROW_WGS");
     if (e && *e) return std::atoi(e);
-    return (Codec<S>::storage == VF_STORAGE_FP32 && V == 1 && BUF) ? 2 : 0;
+    const bool streams = 27 * N * sizeof(S) > kCacheResidentBytes;
+    return (Codec<S>::storage == VF_STORAGE_FP32 && V == 1 && BUF && streams) ? 2 : 0;
 }
 
 // dynamic LDS bytes that leave room for at most k workgroups of the row kernel on one CU
@@ -419,7 +437,7 @@ static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
 {
     dim3 grid(a.ny, a.nz);
     dim3 block(((a.nx / V + 63) / 64) * 64); // whole waves; lanes beyond the row idle
-    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>());
+    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>(a.N));
     if (out) {
         if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, true>), grid, block, lds, s, f, out, a, r);
         else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF, true>), grid, block, lds, s, f, out, a, r);
