@@ -101,7 +101,9 @@ def test_two_level_replay_matches_eager_steps(gpu, storage):
         even = not even
 
 
-def test_k17_steps_same_with_and_without_graphs(gpu):
+# 13 steps every 4: single-step graphs; 45 steps every 20: graphs of 16 steps inside segments
+@pytest.mark.parametrize("steps,K", [(13, 4), (45, 20)])
+def test_k17_steps_same_with_and_without_graphs(gpu, steps, K):
     import torch
 
     from virtualfluids_amd.adjoint import k17_steps
@@ -112,7 +114,7 @@ def test_k17_steps_same_with_and_without_graphs(gpu):
     for graphs in (True, False):
         x = L.f.detach().clone().requires_grad_(True)
         U = U0.clone().requires_grad_(True)
-        y = k17_steps(x, L, 13, checkpoint_every=4, even=True, bc_factory=_factory(), velocity=U, graphs=graphs)
+        y = k17_steps(x, L, steps, checkpoint_every=K, even=True, bc_factory=_factory(), velocity=U, graphs=graphs)
         w = torch.linspace(-1, 1, y.numel(), device="cuda").reshape(y.shape)
         (y.float() * w).sum().backward()
         out.append((y.detach(), x.grad, U.grad))
@@ -120,7 +122,8 @@ def test_k17_steps_same_with_and_without_graphs(gpu):
         assert torch.equal(a, b)
 
 
-def test_two_level_steps_same_with_and_without_graphs(gpu):
+@pytest.mark.parametrize("steps,K,observe_from", [(9, 4, 5), (41, 20, 37)])
+def test_two_level_steps_same_with_and_without_graphs(gpu, steps, K, observe_from):
     import torch
 
     from virtualfluids_amd.adjoint import KineticEnergyObservation, k17_two_level_steps
@@ -133,13 +136,19 @@ def test_two_level_steps_same_with_and_without_graphs(gpu):
     out = []
     for graphs in (True, False):
         U = U0.clone().requires_grad_(True)
-        C, F, J = k17_two_level_steps(Lc.f.clone(), Lf.f.clone(), Lc, Lf, (nb.cf, nb.fc), 9, checkpoint_every=4,
-                                      even=True, bc_factory=_factory(), velocity=U, observe=obs, observe_from=5,
-                                      graphs=graphs)
+        C, F, J = k17_two_level_steps(Lc.f.clone(), Lf.f.clone(), Lc, Lf, (nb.cf, nb.fc), steps,
+                                      checkpoint_every=K, even=True, bc_factory=_factory(), velocity=U, observe=obs,
+                                      observe_from=observe_from, graphs=graphs)
         J.backward()
         out.append((C.detach(), F.detach(), J.detach(), U.grad))
-    for a, b in zip(*out):
-        assert torch.equal(a, b)
+    for name, a, b in zip(("coarse", "fine", "objective", "inflow gradient"), *out):
+        print(name, float((a.double() - b.double()).abs().max()), float(a.double().abs().max()))
+    # the forward sweep is the same bit for bit; the reverse sweep's interface adjoints add into
+    # shared source cells with atomics, whose order (and last bit) varies from run to run
+    for name, a, b in zip(("coarse", "fine", "objective"), out[0][:3], out[1][:3]):
+        assert torch.equal(a, b), name
+    ga, gb = out[0][3].double(), out[1][3].double()
+    assert float((ga - gb).abs().max()) <= 1e-6 * float(ga.abs().max())
     assert float(out[0][2]) > 0 and float(out[0][3].abs().sum()) > 0
 
 
@@ -153,9 +162,10 @@ def test_simulation_run_same_with_and_without_graphs(gpu, box):
 
     A, B = _channel_lattice(box), _channel_lattice(box)
     sa, sb = Simulation(A, _factory()), Simulation(B, _factory())
-    for chunk in range(4):
-        sa.run(7, graphs=True)
-        sb.run(7, graphs=False)
+    # 7 steps: per-parity graphs only; 37 and 40: graphs of 16 steps, then single steps
+    for chunk, n in enumerate((7, 37, 7, 40)):
+        sa.run(n, graphs=True)
+        sb.run(n, graphs=False)
         torch.cuda.synchronize()
         assert A.even == B.even and sa.t == sb.t
         assert torch.equal(A.f, B.f), f"chunk {chunk}"

@@ -135,11 +135,12 @@ class _K17Steps(torch.autograd.Function):
         run = _StepGraphs(lambda ev, st: stepper.forward(buf, ev, None, st), graphs)
         checkpoints = []
         even = even0
-        for i in range(steps):
-            if i % checkpoint_every == 0:
-                checkpoints.append((i, even, buf.clone()))
-            run(even)
-            even = not even
+        i = 0
+        while i < steps:
+            checkpoints.append((i, even, buf.clone()))
+            n = min(checkpoint_every, steps - i)
+            even = run.run(n, even)
+            i += n
         del run
         ctx.stepper, ctx.steps, ctx.K = stepper, steps, checkpoint_every
         ctx.checkpoints = checkpoints
@@ -226,13 +227,20 @@ class _TwoLevelSteps(torch.autograd.Function):
         checkpoints = []
         even = even0
         J = torch.zeros((), dtype=torch.float64, device=fc0.device)
-        for i in range(steps):
+        i = 0
+        while i < steps:
             if i % K == 0:
                 checkpoints.append((i, even, C.clone(), F.clone()))
-            run(even)
-            even = not even
             if observe is not None and i >= observe_from:
+                run(even)  # one step: the observation follows every step from here on
+                even = not even
                 J += observe(C, even).detach().to(torch.float64)
+                i += 1
+                continue
+            # up to the next checkpoint or the first observed step
+            n = min(K - i % K, steps - i, (observe_from - i) if observe is not None else steps)
+            even = run.run(n, even)
+            i += n
         del run
         ctx.sc, ctx.sf, ctx.cells, ctx.steps, ctx.K = sc, sf, cells, steps, K
         ctx.observe, ctx.observe_from = observe, observe_from

@@ -12,8 +12,9 @@ import torch
 class StepGraphs:
     """Runs a time step that works in place on fixed buffers, replaying one HIP graph per parity.
 
-    step(even, stream) launches one step on `stream`. The first two calls (one of each parity)
-    run eagerly, so that every lazy set-up of the library happens outside a capture; from then
+    step(even, stream) launches one step on `stream`. Calling the object runs one step; run(n,
+    even) runs n, replaying graphs of `chunk` steps where it can. The first two steps (one of
+    each parity) run eagerly, so that every lazy set-up of the library happens outside a capture; from then
     on each parity's launches are captured once (torch.cuda.CUDAGraph on a side stream of
     default priority) and replayed on the current stream. The captured launches read their
     kernel arguments at capture time: the buffers, BC arrays and porosity fields they name must
@@ -22,12 +23,39 @@ class StepGraphs:
     garbage and empty the allocator's cache around every capture, which costs more than a
     thousand replayed steps save.)"""
 
-    def __init__(self, step, enabled: bool = True):
+    def __init__(self, step, enabled: bool = True, chunk: int = 16):
         self.step = step
         self.enabled = enabled
+        self.chunk = chunk if chunk % 2 == 0 else chunk + 1  # whole pairs: the parity comes back
         self.graphs = {}
+        self.chunks = {}
         self.eager_calls = 0
         self.side = None
+
+    def _capture(self, launch):
+        if self.side is None:
+            self.side = torch.cuda.Stream()
+        return capture(launch, self.side)
+
+    def run(self, n: int, even: bool) -> bool:
+        """n steps from parity `even`, in graphs of `chunk` steps (one replay, one host round
+        trip, per chunk) and single steps for the rest; returns the parity after them."""
+        while n > 0:
+            if self.enabled and self.eager_calls >= 2 and self.chunk > 1 and n >= self.chunk:
+                g = self.chunks.get(even)
+                if g is None:
+                    def launch(st, e0=even):
+                        for j in range(self.chunk):
+                            self.step(e0 ^ (j % 2 == 1), st)
+
+                    g = self.chunks[even] = self._capture(launch)
+                g.replay()
+                n -= self.chunk
+            else:
+                self(even)
+                even = not even
+                n -= 1
+        return even
 
     def __call__(self, even: bool):
         if not self.enabled or self.eager_calls < 2:
@@ -36,17 +64,21 @@ class StepGraphs:
             return
         g = self.graphs.get(even)
         if g is None:
-            cur = torch.cuda.current_stream()
-            if self.side is None:
-                self.side = torch.cuda.Stream()
-            self.side.wait_stream(cur)
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.stream(self.side):
-                g.capture_begin()
-                try:
-                    self.step(even, self.side)
-                finally:
-                    g.capture_end()
-            cur.wait_stream(self.side)
-            self.graphs[even] = g
+            g = self.graphs[even] = self._capture(lambda st: self.step(even, st))
         g.replay()
+
+
+def capture(launch, side: torch.cuda.Stream) -> torch.cuda.CUDAGraph:
+    """A HIP graph of the launches launch(stream) makes, captured on `side` (default priority)
+    after the current stream's pending work; replay() then runs them on the current stream."""
+    cur = torch.cuda.current_stream()
+    side.wait_stream(cur)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        g.capture_begin()
+        try:
+            launch(side)
+        finally:
+            g.capture_end()
+    cur.wait_stream(side)
+    return g

@@ -129,10 +129,10 @@ class Simulation:
         self.t += 1
 
     def run(self, steps: int, stream=None, graphs: bool = True):
-        """`steps` time steps. With graphs (and no explicit stream) each parity's launches are
-        captured once and replayed (virtualfluids_amd.graphs.StepGraphs): the same kernels
-        with the same arguments, without a host round trip per launch; graphs=False launches
-        every kernel from the host."""
+        """`steps` time steps. With graphs (and no explicit stream) the steps' launches are
+        captured once, 16 steps to a graph, and replayed (virtualfluids_amd.graphs.StepGraphs):
+        the same kernels with the same arguments, without a host round trip per launch;
+        graphs=False launches every kernel from the host."""
         if not graphs or stream is not None or steps < 3:
             for _ in range(steps):
                 self.step(stream)
@@ -145,12 +145,9 @@ class Simulation:
             L.even = even
             self._launch(st)
 
-        run = StepGraphs(launch)
-        for _ in range(steps):
-            even = L.even
-            run(even)
-            L.even = not even  # the step's parity swap (host state; a replay does not run it)
-            self.t += 1
+        # the steps' parity swaps are host state, which a replay does not run: set it after
+        L.even = StepGraphs(launch).run(steps, L.even)
+        self.t += steps
 
 
 class MultiLevelSimulation:
