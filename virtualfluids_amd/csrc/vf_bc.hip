@@ -155,17 +155,17 @@ template <int KIND> static int launch_bc_post(const vf_lbm_parameter* p, const v
     }
     const Addr a = make_addr(*p);
     BcArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega};
-    const dim3 grid((bc->numberOfBCnodes + 255) / 256);
+    const dim3 grid((bc->numberOfBCnodes + kBcBlock - 1) / kBcBlock);
     hipStream_t s = (hipStream_t)stream;
     switch (p->storage) {
     case VF_STORAGE_FP32:
-        hipLaunchKernelGGL((bc_post_kernel<float, KIND>), grid, dim3(256), 0, s, (float*)p->distributions, a, b, p->isEvenTimestep);
+        hipLaunchKernelGGL((bc_post_kernel<float, KIND>), grid, dim3(kBcBlock), 0, s, (float*)p->distributions, a, b, p->isEvenTimestep);
         break;
     case VF_STORAGE_FP16:
-        hipLaunchKernelGGL((bc_post_kernel<__half, KIND>), grid, dim3(256), 0, s, (__half*)p->distributions, a, b, p->isEvenTimestep);
+        hipLaunchKernelGGL((bc_post_kernel<__half, KIND>), grid, dim3(kBcBlock), 0, s, (__half*)p->distributions, a, b, p->isEvenTimestep);
         break;
     case VF_STORAGE_BF16:
-        hipLaunchKernelGGL((bc_post_kernel<__hip_bfloat16, KIND>), grid, dim3(256), 0, s, (__hip_bfloat16*)p->distributions, a, b,
+        hipLaunchKernelGGL((bc_post_kernel<__hip_bfloat16, KIND>), grid, dim3(kBcBlock), 0, s, (__hip_bfloat16*)p->distributions, a, b,
                            p->isEvenTimestep);
         break;
     default:
@@ -229,17 +229,17 @@ extern "C" int vf_PressureNonEquilibriumCompressible(const vf_lbm_parameter* p, 
     default: return VF_OK; // the reference kernel writes nothing for other directions
     }
     const Addr a = make_addr(*p);
-    const dim3 grid((bc->numberOfBCnodes + 255) / 256);
+    const dim3 grid((bc->numberOfBCnodes + kBcBlock - 1) / kBcBlock);
     hipStream_t s = (hipStream_t)stream;
     switch (p->storage) {
     case VF_STORAGE_FP32:
-        hipLaunchKernelGGL((pressure_neq_kernel<float>), grid, dim3(256), 0, s, (float*)p->distributions, a, b, p->isEvenTimestep);
+        hipLaunchKernelGGL((pressure_neq_kernel<float>), grid, dim3(kBcBlock), 0, s, (float*)p->distributions, a, b, p->isEvenTimestep);
         break;
     case VF_STORAGE_FP16:
-        hipLaunchKernelGGL((pressure_neq_kernel<__half>), grid, dim3(256), 0, s, (__half*)p->distributions, a, b, p->isEvenTimestep);
+        hipLaunchKernelGGL((pressure_neq_kernel<__half>), grid, dim3(kBcBlock), 0, s, (__half*)p->distributions, a, b, p->isEvenTimestep);
         break;
     case VF_STORAGE_BF16:
-        hipLaunchKernelGGL((pressure_neq_kernel<__hip_bfloat16>), grid, dim3(256), 0, s, (__hip_bfloat16*)p->distributions, a, b,
+        hipLaunchKernelGGL((pressure_neq_kernel<__hip_bfloat16>), grid, dim3(kBcBlock), 0, s, (__hip_bfloat16*)p->distributions, a, b,
                            p->isEvenTimestep);
         break;
     default:

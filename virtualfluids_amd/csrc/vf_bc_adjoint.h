@@ -14,6 +14,11 @@ namespace vfa {
 
 enum BcKind : int { kNoSlipBB = 0, kVelocityBB = 1, kNoSlipInterp = 2, kVelocityInterp = 3, kVelocityPressureInterp = 4 };
 
+// One wave per workgroup for the BC kernels (one thread per BC node): a boundary has a few
+// thousand to a few hundred thousand nodes, each a short latency-bound chain, and 64-thread
+// workgroups spread them over every CU (4 096 inflow nodes are 64 workgroups, not 16).
+constexpr int kBcBlock = 64;
+
 template <int KIND>
 VF_HD void bc_post_vjp(const float (&post)[27], const float (&q)[27], float omega, const float (&G)[27],
                        float (&fb)[27], float (&Ub)[3])

@@ -137,8 +137,8 @@ template <int KIND> struct PostAdjL {
     template <class S> struct L {
         static void run(const vf_lbm_parameter& p, const BcAdjArgs& b, float* lam, hipStream_t s)
         {
-            const dim3 grid((b.nbc + 255) / 256);
-            hipLaunchKernelGGL((bc_post_adjoint_kernel<S, KIND>), grid, dim3(256), 0, s, (const S*)p.distributions, lam,
+            const dim3 grid((b.nbc + kBcBlock - 1) / kBcBlock);
+            hipLaunchKernelGGL((bc_post_adjoint_kernel<S, KIND>), grid, dim3(kBcBlock), 0, s, (const S*)p.distributions, lam,
                                make_addr(p), b, p.isEvenTimestep);
         }
     };
@@ -147,8 +147,8 @@ template <int KIND> struct PostAdjL {
 template <class S> struct PressAdjL {
     static void run(const vf_lbm_parameter& p, const PressureAdjArgs& b, float* lam, hipStream_t s)
     {
-        const dim3 grid((b.nbc + 255) / 256);
-        hipLaunchKernelGGL(pressure_adjoint_kernel<S>, grid, dim3(256), 0, s, (const S*)p.distributions, lam,
+        const dim3 grid((b.nbc + kBcBlock - 1) / kBcBlock);
+        hipLaunchKernelGGL(pressure_adjoint_kernel<S>, grid, dim3(kBcBlock), 0, s, (const S*)p.distributions, lam,
                            make_addr(p), b, p.isEvenTimestep);
     }
 };
