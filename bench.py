@@ -126,16 +126,12 @@ def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, 
     idx = {e: [torch.from_numpy(a).cuda() for a in momentum_exchange_index(g, "geometry", e)] for e in (True, False)}
     spin_up = steps - average_last
 
+    def drag(buf, even):
+        return momentum_exchange(buf, *idx[even])[0] / average_last
+
     def objective(phi):
-        s = k17_steps(L.f.detach().clone(), L, spin_up, checkpoint_every=checkpoint_every, even=True,
-                      bc_factory=fac, porosity=phi)
-        even = spin_up % 2 == 0
-        J = 0.0
-        for _ in range(average_last):
-            s = k17_steps(s, L, 1, checkpoint_every=1, even=even, bc_factory=fac, porosity=phi)
-            even = not even
-            J = J + momentum_exchange(s, *idx[even])[0] / average_last
-        return J
+        return k17_steps(L.f.detach().clone(), L, steps, checkpoint_every=checkpoint_every, even=True,
+                         bc_factory=fac, porosity=phi, observe=drag, observe_from=spin_up)[1]
 
     phi = torch.from_numpy(phi0.astype(np.float32)).cuda()
     torch.cuda.synchronize()

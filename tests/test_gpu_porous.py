@@ -70,7 +70,10 @@ def test_porosity_adjoint_matches_reference(gpu, orc, box):
     assert err < 3e-5 and err_p < 1e-4
 
 
-def test_sphere_channel_drag_gradient_wrt_porosity(gpu, orc):
+# per_step: the averaging window as single-step k17_steps calls; observe: one k17_steps call
+# whose observation hook sums the drag over the window
+@pytest.mark.parametrize("form", ["per_step", "observe"])
+def test_sphere_channel_drag_gradient_wrt_porosity(gpu, orc, form):
     import torch
 
     from helpers import ORACLE_POST_BC
@@ -97,14 +100,21 @@ def test_sphere_channel_drag_gradient_wrt_porosity(gpu, orc):
     idx_odd = [torch.from_numpy(a).cuda() for a in momentum_exchange_index(g, "geometry", False)]
     phi = torch.from_numpy(phi0.astype(np.float32)).cuda().requires_grad_(True)
     x = L.f.detach().clone()
-    s = k17_steps(x, L, spin_up, checkpoint_every=10, even=True, bc_factory=fac, porosity=phi)
-    even = spin_up % 2 == 0
-    J = 0.0
-    for _ in range(avg):
-        s = k17_steps(s, L, 1, checkpoint_every=1, even=even, bc_factory=fac, porosity=phi)
-        even = not even
-        i = idx if even else idx_odd
-        J = J + momentum_exchange(s, *i)[0] / avg
+    if form == "observe":
+        def drag(buf, even):
+            return momentum_exchange(buf, *(idx if even else idx_odd))[0] / avg
+
+        _, J = k17_steps(x, L, spin_up + avg, checkpoint_every=10, even=True, bc_factory=fac, porosity=phi,
+                         observe=drag, observe_from=spin_up)
+    else:
+        s = k17_steps(x, L, spin_up, checkpoint_every=10, even=True, bc_factory=fac, porosity=phi)
+        even = spin_up % 2 == 0
+        J = 0.0
+        for _ in range(avg):
+            s = k17_steps(s, L, 1, checkpoint_every=1, even=even, bc_factory=fac, porosity=phi)
+            even = not even
+            i = idx if even else idx_odd
+            J = J + momentum_exchange(s, *i)[0] / avg
     J.backward()
     grad = phi.grad.cpu().numpy().astype(np.float64)
 

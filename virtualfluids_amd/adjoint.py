@@ -117,7 +117,8 @@ class Stepper:
 
 class _K17Steps(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs):
+    def forward(ctx, f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs, observe,
+                observe_from):
         L = stepper.L
         if f0.dtype != L.f.dtype or f0.numel() != L.f.numel() or not f0.is_cuda:
             raise ValueError("f0 must be a device buffer of the lattice's storage type and size")
@@ -135,25 +136,39 @@ class _K17Steps(torch.autograd.Function):
         run = _StepGraphs(lambda ev, st: stepper.forward(buf, ev, None, st), graphs)
         checkpoints = []
         even = even0
+        J = torch.zeros((), dtype=torch.float64, device=f0.device)
         i = 0
         while i < steps:
-            checkpoints.append((i, even, buf.clone()))
-            n = min(checkpoint_every, steps - i)
+            if i % checkpoint_every == 0:
+                checkpoints.append((i, even, buf.clone()))
+            if observe is not None and i >= observe_from:
+                run(even)  # one step: the observation follows every step from here on
+                even = not even
+                J += observe(buf, even).detach().to(torch.float64)
+                i += 1
+                continue
+            # up to the next checkpoint or the first observed step
+            n = min(checkpoint_every - i % checkpoint_every, steps - i,
+                    (observe_from - i) if observe is not None else steps)
             even = run.run(n, even)
             i += n
         del run
         ctx.stepper, ctx.steps, ctx.K = stepper, steps, checkpoint_every
         ctx.checkpoints = checkpoints
+        ctx.observe, ctx.observe_from = observe, observe_from
         ctx.has_velocity, ctx.has_rho = velocity is not None, rho_bc is not None
         ctx.has_porosity = porosity is not None
-        return buf
+        return buf, J
 
     @staticmethod
-    def backward(ctx, grad_out):
+    def backward(ctx, grad_out, gJ):
         stepper, steps, K = ctx.stepper, ctx.steps, ctx.K
         L = stepper.L
         stream = torch.cuda.current_stream()
+        if grad_out is None:
+            grad_out = torch.zeros_like(ctx.checkpoints[0][2])
         lam = grad_out.detach().to(torch.float32).contiguous().clone()
+        observe = ctx.observe if gJ is not None else None
         vel_bc, pr_bc = L.bcs.get("velocity"), L.bcs.get("pressure")
         ubar = torch.zeros(3 * vel_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_velocity else None
         rbar = torch.zeros(pr_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_rho else None
@@ -166,25 +181,35 @@ class _K17Steps(torch.autograd.Function):
                 states.append(stepper.forward(states[-1], even, torch.empty_like(ckpt), stream))
                 even = not even
             for j in reversed(range(n)):
-                stepper.adjoint(states[j], states[j + 1], lam, even_start ^ (j % 2 == 1), ubar, rbar, padj, stream)
+                even_j = even_start ^ (j % 2 == 1)
+                if observe is not None and start + j >= ctx.observe_from:
+                    # the observed objective of the state after this step
+                    _observe_grad(observe, states[j + 1], not even_j, lam, gJ)
+                stepper.adjoint(states[j], states[j + 1], lam, even_j, ubar, rbar, padj, stream)
             del states
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
-        return lam.to(grad_out.dtype), gv, rbar, padj, None, None, None, None, None
+        return lam.to(grad_out.dtype), gv, rbar, padj, None, None, None, None, None, None, None
 
 
 def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: int = 10, even: bool | None = None,
               bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
               rho_bc: torch.Tensor | None = None, porosity: torch.Tensor | None = None,
-              graphs: bool = True) -> torch.Tensor:
+              graphs: bool = True, observe=None, observe_from: int = 0):
     """`steps` time steps of `lattice` from buffer f0, differentiable in f0 and, when given, in
     the velocity BC's wall velocities `velocity` (3, n), the pressure BC's densities `rho_bc`
     and the Brinkman porosity per node `porosity` (N).
 
     The lattice's own buffer and parity are not touched (its BC parameter arrays take the given
     values); `even` is the parity of the first step (default: the lattice's current parity).
-    graphs: the forward sweep replays one HIP graph per parity (False: every launch from the
+    graphs: the forward sweep replays HIP graphs of the steps (False: every launch from the
     host; the results are the same bit for bit).
+    observe(buf, even) -> scalar tensor, a differentiable function of the buffer and its parity
+    (KineticEnergyObservation, or any torch function such as a momentum-exchange drag), is
+    summed over the states after steps observe_from .. steps-1; then the call returns (buffer,
+    observed sum in FP64), and the sum's gradient enters the reverse sweep step by step, so a
+    time average needs no stored states beyond the checkpoints. Without observe it returns the
+    buffer.
     """
     if steps < 0 or checkpoint_every < 1:
         raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
@@ -194,7 +219,9 @@ def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: 
     if rho_bc is not None and lattice.bcs.get("pressure") is None:
         raise ValueError("the lattice has no pressure BC nodes")
     even0 = lattice.even if even is None else bool(even)
-    return _K17Steps.apply(f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs)
+    out, J = _K17Steps.apply(f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs,
+                             observe, observe_from)
+    return out if observe is None else (out, J)
 
 
 class _TwoLevelSteps(torch.autograd.Function):
@@ -302,11 +329,12 @@ def _observe_grad(observe, buf, even, lam, gJ):
             cells = observe.cells(even)
             vals = buf.reshape(-1)[cells].detach().to(torch.float32).requires_grad_(True)
             (g,) = torch.autograd.grad(observe.value(vals, even), vals)
-            lam.index_add_(0, cells.reshape(-1), g.reshape(-1).to(torch.float32), alpha=float(gJ))
+            # gJ stays on the device (no host synchronisation per observed step)
+            lam.index_add_(0, cells.reshape(-1), g.reshape(-1).to(torch.float32) * gJ.to(torch.float32))
         else:
             x = buf.detach().to(torch.float32).requires_grad_(True)
             (g,) = torch.autograd.grad(observe(x, even), x)
-            lam.add_(g.to(torch.float32), alpha=float(gJ))
+            lam.add_(g.to(torch.float32) * gJ.to(torch.float32))
 
 
 def _neigh(cells):
