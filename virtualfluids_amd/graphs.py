@@ -1,8 +1,8 @@
 """HIP-graph replay of time steps that work in place on fixed buffers.
 
-A long run of short kernels is bound by the host launching them; capturing one step per
-parity in a HIP graph and replaying it takes the host out of the loop. No counterpart in the
-reference (its time loop launches every kernel from the host).
+A long run of short kernels is bound by the host launching them; capturing the steps in HIP
+graphs and replaying them takes the host out of the loop. No counterpart in the reference (its
+time loop launches every kernel from the host).
 """
 from __future__ import annotations
 
@@ -10,13 +10,14 @@ import torch
 
 
 class StepGraphs:
-    """Runs a time step that works in place on fixed buffers, replaying one HIP graph per parity.
+    """Runs a time step that works in place on fixed buffers from replayed HIP graphs.
 
     step(even, stream) launches one step on `stream`. Calling the object runs one step; run(n,
-    even) runs n, replaying graphs of `chunk` steps where it can. The first two steps (one of
-    each parity) run eagerly, so that every lazy set-up of the library happens outside a capture; from then
-    on each parity's launches are captured once (torch.cuda.CUDAGraph on a side stream of
-    default priority) and replayed on the current stream. The captured launches read their
+    even) runs n, replaying graphs of `chunk` steps (an even number, so that the parity comes
+    back) where it can. The first two steps (one of each parity) run eagerly, so that every lazy
+    set-up of the library happens outside a capture; from then on each graph (one step of each
+    parity, `chunk` steps from each parity) is captured once (torch.cuda.CUDAGraph on a side
+    stream of default priority) and replayed on the current stream. The captured launches read their
     kernel arguments at capture time: the buffers, BC arrays and porosity fields they name must
     stay where they are while the graphs live (the owner drops them with the run). The step must
     not allocate or clear memory with memset nodes. (torch.cuda.graph would also collect
