@@ -368,7 +368,10 @@ public:
                  const BoundaryConditionFactory* bcFactory, GridScalingFactory* scalingFactory);
     /* one step of `level` (and two of every finer level before it) */
     void updateGrid(int level, unsigned int t);
+    /* `steps` coarse steps; with graphs (the default) pairs of steps replay one HIP graph
+     * (no counterpart in the reference, whose loop launches every kernel from the host) */
     void run(unsigned int steps);
+    void setUseGraphs(bool on) { useGraphs = on; }
 
 private:
     void collision(int level);
@@ -382,6 +385,7 @@ private:
     BoundaryConditionKernelManager bcKernelManager;
     GridScalingKernelManager gridScalingKernelManager;
     unsigned int timestep = 0;
+    bool useGraphs = true;
 };
 
 } // namespace vfamd
