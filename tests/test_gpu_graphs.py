@@ -170,3 +170,30 @@ def test_simulation_run_same_with_and_without_graphs(gpu, box):
         assert A.even == B.even and sa.t == sb.t
         assert torch.equal(A.f, B.f), f"chunk {chunk}"
         _ = A.f.float().abs().sum().item()
+
+
+def test_multilevel_run_same_with_and_without_graphs(gpu):
+    """MultiLevelSimulation.run replays pairs of coarse steps from one graph: runs of 7 and 10
+    coarse steps (eager work and a synchronisation between them) leave the same fields on both
+    levels as host-launched steps."""
+    import torch
+
+    from virtualfluids_amd.refinement import GridScalingFactory
+    from virtualfluids_amd.simulation import MultiLevelSimulation
+
+    runs = []
+    for _ in range(2):
+        nb, C, F = _pair("fp16")
+        sf = GridScalingFactory()
+        sf.set_scaling_factory(GridScalingFactory.GridScaling.ScaleCompressible)
+        runs.append((nb, C, F, MultiLevelSimulation([C, F], [(nb.cf, nb.fc)], [_factory(), None],
+                                                    scaling_factory=sf)))
+    (_, Ca, Fa, ma), (_, Cb, Fb, mb) = runs
+    for n in (7, 10, 3):
+        ma.run(n, graphs=True)
+        mb.run(n, graphs=False)
+        torch.cuda.synchronize()
+        assert ma.t == mb.t and [lv.t for lv in ma.levels] == [lv.t for lv in mb.levels]
+        assert Ca.even == Cb.even and Fa.even == Fb.even
+        assert torch.equal(Ca.f, Cb.f) and torch.equal(Fa.f, Fb.f), f"after {n} steps"
+        _ = Fa.f.float().abs().max().item()

@@ -152,7 +152,7 @@ class _K17Steps(torch.autograd.Function):
                     (observe_from - i) if observe is not None else steps)
             even = run.run(n, even)
             i += n
-        del run
+        run.close()
         ctx.stepper, ctx.steps, ctx.K = stepper, steps, checkpoint_every
         ctx.checkpoints = checkpoints
         ctx.observe, ctx.observe_from = observe, observe_from
@@ -268,7 +268,7 @@ class _TwoLevelSteps(torch.autograd.Function):
             n = min(K - i % K, steps - i, (observe_from - i) if observe is not None else steps)
             even = run.run(n, even)
             i += n
-        del run
+        run.close()
         ctx.sc, ctx.sf, ctx.cells, ctx.steps, ctx.K = sc, sf, cells, steps, K
         ctx.observe, ctx.observe_from = observe, observe_from
         ctx.checkpoints = checkpoints

@@ -33,6 +33,14 @@ class StepGraphs:
         self.eager_calls = 0
         self.side = None
 
+    def close(self):
+        """Wait for the replays in flight, then release the graphs (an executable graph is not
+        released while the GPU may still run it)."""
+        if self.graphs or self.chunks:
+            torch.cuda.current_stream().synchronize()
+        self.graphs.clear()
+        self.chunks.clear()
+
     def _capture(self, launch):
         if self.side is None:
             self.side = torch.cuda.Stream()

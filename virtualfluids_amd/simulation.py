@@ -146,7 +146,9 @@ class Simulation:
             self._launch(st)
 
         # the steps' parity swaps are host state, which a replay does not run: set it after
-        L.even = StepGraphs(launch).run(steps, L.even)
+        run = StepGraphs(launch)
+        L.even = run.run(steps, L.even)
+        run.close()
         self.t += steps
 
 
@@ -195,6 +197,35 @@ class MultiLevelSimulation:
         self._update(0, stream)
         self.t += 1
 
-    def run(self, steps: int, stream=None):
-        for _ in range(steps):
-            self.step(stream)
+    def run(self, steps: int, stream=None, graphs: bool = True):
+        """`steps` coarse steps. With graphs (and no explicit stream) pairs of coarse steps
+        replay one HIP graph after two steps from the host (after a pair every level's parity
+        is back where it started); graphs=False launches every kernel from the host. The
+        fields are the same bit for bit."""
+        if not graphs or stream is not None or steps < 4:
+            for _ in range(steps):
+                self.step(stream)
+            return
+        import torch
+
+        from .graphs import capture
+
+        for _ in range(2):
+            self.step()
+        steps -= 2
+        pairs = steps // 2
+        if pairs:
+            saved_t = [lv.t for lv in self.levels]
+            g = capture(lambda st: (self._update(0, st), self._update(0, st)), torch.cuda.Stream())
+            # the capture ran the host side of a pair (parities are back); no step was taken
+            for lv, t in zip(self.levels, saved_t):
+                lv.t = t
+            for _ in range(pairs):
+                g.replay()
+            torch.cuda.current_stream().synchronize()  # before the graph is released
+            del g
+            for level, lv in enumerate(self.levels):
+                lv.t += 2 * pairs * 2 ** level
+            self.t += 2 * pairs
+        if steps % 2:
+            self.step()
