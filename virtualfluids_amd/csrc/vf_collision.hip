@@ -404,6 +404,8 @@ template <class S> static int row_width(int nx)
 // VALU issue and need every wave they can get. The cap is imposed by reserving LDS at launch.
 // VFAMD_ROW_WGS overrides it for benchmarking.
 constexpr uint64_t kCacheResidentBytes = 256ull << 20;
+// lattices this small stay in the L2 and infinity caches step after step
+constexpr uint64_t kSmallLatticeBytes = 64ull << 20;
 template <class S, int V, bool BUF> static int row_wgs_per_cu(uint64_t N)
 {
     const char* e = std::getenv("VFAMD_ROW_WGS");
@@ -476,6 +478,14 @@ static bool try_launch_row(const vf_lbm_parameter& p, const BoxArgs& a, const K1
     const bool mask = p.fluidMask != nullptr;
     const int V = row_width<S>(p.nx);
     if (V <= 0 || p.nx % V != 0 || p.nx / V > kRowMaxThreads) return false;
+    // A small lattice that stays in the caches is latency-bound, and rows that leave many lanes
+    // idle are better served by the scalar kernel (no shuffles or barriers, more resident
+    // waves): the 66^3 cavity (31 MB, rows of 33 of 64 lanes) runs 15 % faster on it.
+    const int lanes = p.nx / V, waves = (lanes + 63) / 64;
+    // (VFAMD_SMALL_SCALAR=0 keeps the row kernel there; a forced VFAMD_ROW_WIDTH does too)
+    const char* keep = std::getenv("VFAMD_SMALL_SCALAR");
+    const bool scalarAllowed = !(keep && *keep == '0') && !std::getenv("VFAMD_ROW_WIDTH");
+    if (scalarAllowed && 27 * a.N * sizeof(S) <= kSmallLatticeBytes && 4 * lanes < 3 * 64 * waves) return false;
     if constexpr (sizeof(S) == 4) {
         if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, s); return true; }
     }
