@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define VF_ABI_VERSION 2
+#define VF_ABI_VERSION 3
 
 enum vf_status {
     VF_OK = 0,
@@ -98,6 +98,13 @@ typedef struct vf_lbm_parameter {
      * so the collision velocity becomes u (1 - porosity/2). Honoured by every collision and
      * adjoint entry point. */
     const float* porosity;
+    /* Bytes of lattice state that the caller's time step cycles through between two collisions
+     * of this lattice, e.g. every level of a refinement hierarchy (0: this lattice's own
+     * 27 * numberOfNodes values; no counterpart in the reference). Dense-box collisions pick
+     * their kernel from it: up to the 256 MB infinity cache of an MI355X the work is
+     * cache-resident and latency-bound and takes the one-node-per-thread kernel, beyond it the
+     * streaming row kernel. */
+    uint64_t workingSetBytes;
 } vf_lbm_parameter;
 
 /*

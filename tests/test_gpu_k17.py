@@ -195,6 +195,8 @@ def test_out_of_place_step_matches_in_place(gpu, monkeypatch, storage, dims, sol
     from virtualfluids_amd.lattice import stream_handle
 
     monkeypatch.setenv("VFAMD_ROW_BUFFER", "1" if addressing == "buffer" else "0")
+    # a lattice this small would take the scalar kernel (cache-resident working set)
+    monkeypatch.setenv("VFAMD_SMALL_SCALAR", "0")
     grid = periodic_box(*dims)
     rng = np.random.default_rng(300 + sum(dims) + solid)
     if solid:
@@ -230,7 +232,7 @@ def test_partial_rows_match_oracle(gpu, orc, monkeypatch, storage, case):
     from virtualfluids_amd.grid import walled_box
     from virtualfluids_amd.lattice import stream_handle
 
-    # these lattices are small enough for the scalar kernel's small-lattice rule: keep the row kernel
+    # lattices this small would take the scalar kernel (cache-resident working set): keep the row kernel
     monkeypatch.setenv("VFAMD_SMALL_SCALAR", "0")
     grid = {"periodic130": lambda: periodic_box(130, 4, 3), "periodic260": lambda: periodic_box(260, 3, 2),
             "walled258": lambda: walled_box((256, 3, 2), {}),

@@ -409,6 +409,9 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
         raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
     if fine.storage != coarse.storage:
         raise ValueError("both levels must use the same storage type")
+    # a coarse step cycles through both levels' populations
+    for L in (coarse, fine):
+        L.set_working_set(coarse.state_bytes + fine.state_bytes)
     sc = Stepper(coarse, bc_factory)
     sf = Stepper(fine, fine_bc_factory)
     cf, fc = interface

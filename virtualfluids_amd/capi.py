@@ -64,6 +64,7 @@ class LBMParameter(ctypes.Structure):
         ("fluidNodeIndices", _vp),
         ("numberOfFluidNodes", ctypes.c_uint32),
         ("porosity", _vp),
+        ("workingSetBytes", ctypes.c_uint64),
     ]
 
 

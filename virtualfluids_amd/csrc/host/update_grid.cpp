@@ -20,6 +20,17 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> parameter, std::vector<std
         if (!para->getParD(level)->isEvenTimestep)
             throw std::runtime_error("finer levels must start with the even parity (the interface kernels read "
                                      "and write them with it)");
+    // a coarse step cycles through every level's populations; levels the caller left at 0 get
+    // that sum (it picks the collision kernel: cache-resident or streaming)
+    if (para->getMaxLevel() > 0) {
+        uint64_t total = 0;
+        for (int level = para->getCoarse(); level <= para->getFine(); ++level) {
+            const auto d = para->getParD(level);
+            total += 27ull * d->numberOfNodes * (d->storage == VF_STORAGE_FP32 ? 4u : 2u);
+        }
+        for (int level = para->getCoarse(); level <= para->getFine(); ++level)
+            if (para->getParD(level)->workingSetBytes == 0) para->getParD(level)->workingSetBytes = total;
+    }
 }
 
 void UpdateGrid27::updateGrid(int level, unsigned int t)

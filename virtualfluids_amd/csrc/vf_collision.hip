@@ -144,7 +144,9 @@ constexpr int kRowMaxThreads = 256;
 
 // INTO: the stepped state goes to `fout` (same addresses, another buffer) and `f` is only read;
 // solid nodes then copy their cells, so `fout` is the complete state the in-place step leaves.
-template <class S, int V, bool EVEN, bool MASK, bool MACRO, bool BUF, bool INTO = false>
+// XCD: the rows are taken in the XCD block ranges of xcd_block (rows that do not fill whole
+// cache lines)
+template <class S, int V, bool EVEN, bool MASK, bool MACRO, bool BUF, bool INTO = false, bool XCD = false>
 __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, BoxArgs a, K17Rates r)
 {
     using IO = RowIO<S, V>;
@@ -157,7 +159,12 @@ __global__ __launch_bounds__(kRowMaxThreads) void k17_row_kernel(S* f, S* fout, 
     const uint32_t lanes = (uint32_t)a.nx / V;
     const bool active = threadIdx.x < lanes;
     const bool lastLane = threadIdx.x + 1 == lanes;
-    const uint32_t y = blockIdx.x, z = blockIdx.y;
+    uint32_t y = blockIdx.x, z = blockIdx.y;
+    if constexpr (XCD) {
+        const uint32_t row = xcd_block(y + gridDim.x * z, gridDim.x * gridDim.y);
+        y = row % (uint32_t)a.ny;
+        z = row / (uint32_t)a.ny;
+    }
     const uint32_t yp = (y + 1 == (uint32_t)a.ny) ? 0 : y + 1;
     const uint32_t zp = (z + 1 == (uint32_t)a.nz) ? 0 : z + 1;
     const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny;
@@ -399,18 +406,16 @@ template <class S> static int row_width(int nx)
 // kernel moves the most bytes per instruction and runs best with few resident rows when it
 // streams from HBM: at 256^3 on MI355X, 8 resident 4-wave workgroups per CU give 5.72 TB/s, 3-4
 // give 5.84-5.86 and 2 give 5.92-5.93 (fewer concurrent direction streams per memory channel);
-// one gives 3.8. A lattice that fits the 256 MB infinity cache is latency-bound instead and
-// takes every wave (the 66^3 cavity: see the launch below). The 16-bit kernels are bound by
-// VALU issue and need every wave they can get. The cap is imposed by reserving LDS at launch.
-// VFAMD_ROW_WGS overrides it for benchmarking.
+// one gives 3.8. A working set that fits the 256 MB infinity cache is latency-bound instead
+// and takes every wave. The 16-bit kernels are bound by VALU issue and need every wave they can
+// get. The cap is imposed by reserving LDS at launch. VFAMD_ROW_WGS overrides it for
+// benchmarking.
 constexpr uint64_t kCacheResidentBytes = 256ull << 20;
-// lattices this small stay in the L2 and infinity caches step after step
-constexpr uint64_t kSmallLatticeBytes = 64ull << 20;
-template <class S, int V, bool BUF> static int row_wgs_per_cu(uint64_t N)
+template <class S, int V, bool BUF> static int row_wgs_per_cu(uint64_t workingSet)
 {
     const char* e = std::getenv("VFAMD_ROW_WGS");
     if (e && *e) return std::atoi(e);
-    const bool streams = 27 * N * sizeof(S) > kCacheResidentBytes;
+    const bool streams = workingSet > kCacheResidentBytes;
     return (Codec<S>::storage == VF_STORAGE_FP32 && V == 1 && BUF && streams) ? 2 : 0;
 }
 
@@ -435,19 +440,24 @@ static size_t row_lds_reserve(int k)
 
 // out: null for the in-place step, else the buffer the out-of-place step writes
 template <class S, int V, bool EVEN, bool BUF>
-static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, hipStream_t s)
+static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, uint64_t ws,
+                         hipStream_t s)
 {
     dim3 grid(a.ny, a.nz);
     dim3 block(((a.nx / V + 63) / 64) * 64); // whole waves; lanes beyond the row idle
-    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>(a.N));
+    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>(ws));
     if (out) {
         if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, true>), grid, block, lds, s, f, out, a, r);
         else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF, true>), grid, block, lds, s, f, out, a, r);
         return;
     }
     if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true, BUF>), grid, block, lds, s, f, f, a, r);
-    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF>), grid, block, lds, s, f, f, a, r);
     else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true, BUF>), grid, block, lds, s, f, f, a, r);
+    else if (BUF && xcd_rows(a.nx, sizeof(S))) {
+        if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, false, true>), grid, block, lds, s, f, f, a, r);
+        else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF, false, true>), grid, block, lds, s, f, f, a, r);
+    }
+    else if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF>), grid, block, lds, s, f, f, a, r);
     else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF>), grid, block, lds, s, f, f, a, r);
 }
 
@@ -459,13 +469,14 @@ static bool row_buffer_allowed()
 }
 
 template <class S, int V, bool EVEN>
-static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, hipStream_t s)
+static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, uint64_t ws,
+                         hipStream_t s)
 {
     // the larger descriptor spans kRowSplit slots; buffer offsets are 32-bit
     if ((uint64_t)kRowSplit * a.N * sizeof(S) <= 0xffffffffull && row_buffer_allowed())
-        launch_row_b<S, V, EVEN, true>(a, r, mask, macro, f, out, s);
+        launch_row_b<S, V, EVEN, true>(a, r, mask, macro, f, out, ws, s);
     else
-        launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, out, s);
+        launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, out, ws, s);
 }
 
 // The row kernel when the box allows it: a row of nx nodes is nx/V lanes, at most one
@@ -478,19 +489,22 @@ static bool try_launch_row(const vf_lbm_parameter& p, const BoxArgs& a, const K1
     const bool mask = p.fluidMask != nullptr;
     const int V = row_width<S>(p.nx);
     if (V <= 0 || p.nx % V != 0 || p.nx / V > kRowMaxThreads) return false;
-    // A small lattice that stays in the caches is latency-bound, and rows that leave many lanes
-    // idle are better served by the scalar kernel (no shuffles or barriers, more resident
-    // waves): the 66^3 cavity (31 MB, rows of 33 of 64 lanes) runs 15 % faster on it.
-    const int lanes = p.nx / V, waves = (lanes + 63) / 64;
-    // (VFAMD_SMALL_SCALAR=0 keeps the row kernel there; a forced VFAMD_ROW_WIDTH does too)
+    // A working set that stays in the 256 MB infinity cache is latency-bound: the scalar kernel
+    // (no shuffles or barriers, every lane busy whatever nx, more resident waves) serves it
+    // best. Measured on MI355X, one lattice stepped alone: 128^3 FP32 67 against 86 us per step,
+    // FP16 36 against 44; 194 x 98 x 98 FP32 68 against 109; 130^3 FP32 85 against 134; the
+    // 66^3 cavity 17 against 18. Streaming lattices are the row kernel's (256^3 FP32 620
+    // against 682-736 us). VFAMD_SMALL_SCALAR=0 (or a forced VFAMD_ROW_WIDTH) keeps the row
+    // kernel there.
+    const uint64_t ws = p.workingSetBytes ? p.workingSetBytes : 27 * a.N * sizeof(S);
     const char* keep = std::getenv("VFAMD_SMALL_SCALAR");
     const bool scalarAllowed = !(keep && *keep == '0') && !std::getenv("VFAMD_ROW_WIDTH");
-    if (scalarAllowed && 27 * a.N * sizeof(S) <= kSmallLatticeBytes && 4 * lanes < 3 * 64 * waves) return false;
+    if (scalarAllowed && ws <= kCacheResidentBytes) return false;
     if constexpr (sizeof(S) == 4) {
-        if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, s); return true; }
+        if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, ws, s); return true; }
     }
-    if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, out, s); return true; }
-    if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, out, s); return true; }
+    if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, out, ws, s); return true; }
+    if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, out, ws, s); return true; }
     return false;
 }
 

@@ -134,6 +134,18 @@ class Lattice:
         self.porosity.copy_(t)
         self.param.porosity = _ptr(self.porosity)
 
+    @property
+    def state_bytes(self) -> int:
+        """Bytes of the population buffer (27 values per node)."""
+        return self.f.numel() * self.f.element_size()
+
+    def set_working_set(self, nbytes: int):
+        """Bytes of lattice state the caller's time step cycles through between two collisions
+        of this lattice (every level of a hierarchy; 0: this lattice's own). The dense-box
+        collision takes the streaming row kernel when it exceeds the 256 MB infinity cache and
+        the one-node-per-thread kernel otherwise."""
+        self.param.workingSetBytes = int(nbytes)
+
     # ---- parity -------------------------------------------------------------------------
     @property
     def even(self) -> bool:

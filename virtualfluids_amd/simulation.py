@@ -171,6 +171,10 @@ class MultiLevelSimulation:
         facs = bc_factories or [None] * len(lattices)
         self.lattices = list(lattices)
         self.levels = [Simulation(L, f) for L, f in zip(self.lattices, facs)]
+        # a coarse step cycles through every level's populations
+        total = sum(L.state_bytes for L in self.lattices)
+        for L in self.lattices:
+            L.set_working_set(total)
         for L in self.lattices[1:]:
             if not L.even:
                 raise ValueError("finer levels must start with the even parity")
