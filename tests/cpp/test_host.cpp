@@ -232,6 +232,25 @@ static void kernel_factory_tests()
     EXPECT(std::fabs(p->getParD(2)->omega - 1.0f / 0.62f) < 1e-6f);
     EXPECT(p->getParD(2)->forceFactor == 0.25f);
     EXPECT(p->getParD(1)->isEvenTimestep == 1);
+    // UpdateGrid27 hands the levels of a hierarchy the sum of their population bytes (the
+    // collision picks its kernel from it); a level the caller set keeps its value
+    {
+        auto hp = make_parameter(1);
+        hp->getParD(0)->numberOfNodes = 1000;
+        hp->getParD(0)->storage = VF_STORAGE_FP32;
+        hp->getParD(1)->numberOfNodes = 500;
+        hp->getParD(1)->storage = VF_STORAGE_FP16;
+        hp->getParD(1)->workingSetBytes = 7;
+        BoundaryConditionFactory bf;
+        GridScalingFactory gf;
+        UpdateGrid27 ug(hp, kf.makeKernels(hp), &bf, &gf);
+        EXPECT(hp->getParD(0)->workingSetBytes == 27ull * (1000 * 4 + 500 * 2));
+        EXPECT(hp->getParD(1)->workingSetBytes == 7);
+        auto single = make_parameter(0);
+        single->getParD(0)->numberOfNodes = 1000;
+        UpdateGrid27 us(single, kf.makeKernels(single), &bf, &gf);
+        EXPECT(single->getParD(0)->workingSetBytes == 0);
+    }
     // the C ABI rejects a null level before touching the GPU
     EXPECT(vf_k17_run(nullptr, nullptr) == VF_ERR_INVALID_ARGUMENT);
     EXPECT_THROW(check(vf_k17_run(nullptr, nullptr)), std::runtime_error);

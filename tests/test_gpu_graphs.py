@@ -189,6 +189,8 @@ def test_multilevel_run_same_with_and_without_graphs(gpu):
         runs.append((nb, C, F, MultiLevelSimulation([C, F], [(nb.cf, nb.fc)], [_factory(), None],
                                                     scaling_factory=sf)))
     (_, Ca, Fa, ma), (_, Cb, Fb, mb) = runs
+    # a coarse step cycles through both levels: the collisions pick their kernel from the sum
+    assert Ca.param.workingSetBytes == Fa.param.workingSetBytes == Ca.state_bytes + Fa.state_bytes
     for n in (7, 10, 3):
         ma.run(n, graphs=True)
         mb.run(n, graphs=False)
