@@ -107,7 +107,7 @@ def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, 
     import numpy as np
     import torch
 
-    from virtualfluids_amd.adjoint import k17_steps, momentum_exchange, momentum_exchange_index
+    from virtualfluids_amd.adjoint import MomentumExchangeObservation, k17_steps
     from virtualfluids_amd.grid import gaussian_random_field, sphere_channel
     from virtualfluids_amd.lattice import Lattice
     from virtualfluids_amd.simulation import BoundaryConditionFactory
@@ -123,11 +123,9 @@ def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, 
     L = Lattice(g, omega)
     zeros = np.zeros(g.N)
     L.init_equilibrium(zeros, zeros, zeros, zeros)
-    idx = {e: [torch.from_numpy(a).cuda() for a in momentum_exchange_index(g, "geometry", e)] for e in (True, False)}
     spin_up = steps - average_last
-
-    def drag(buf, even):
-        return momentum_exchange(buf, *idx[even])[0] / average_last
+    # x drag by momentum exchange on the sphere's links, averaged over the window
+    drag = MomentumExchangeObservation(g, "geometry", axis=0, scale=1.0 / average_last)
 
     def objective(phi, n=steps, observe_from=spin_up):
         return k17_steps(L.f.detach().clone(), L, n, checkpoint_every=checkpoint_every, even=True,

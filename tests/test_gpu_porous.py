@@ -71,8 +71,9 @@ def test_porosity_adjoint_matches_reference(gpu, orc, box):
 
 
 # per_step: the averaging window as single-step k17_steps calls; observe: one k17_steps call
-# whose observation hook sums the drag over the window
-@pytest.mark.parametrize("form", ["per_step", "observe"])
+# whose observation hook sums the drag over the window; observe_cells: the same with the
+# MomentumExchangeObservation, whose gradient is scattered onto the link cells only
+@pytest.mark.parametrize("form", ["per_step", "observe", "observe_cells"])
 def test_sphere_channel_drag_gradient_wrt_porosity(gpu, orc, form):
     import torch
 
@@ -106,6 +107,12 @@ def test_sphere_channel_drag_gradient_wrt_porosity(gpu, orc, form):
 
         _, J = k17_steps(x, L, spin_up + avg, checkpoint_every=10, even=True, bc_factory=fac, porosity=phi,
                          observe=drag, observe_from=spin_up)
+    elif form == "observe_cells":
+        from virtualfluids_amd.adjoint import MomentumExchangeObservation
+
+        obs = MomentumExchangeObservation(g, "geometry", axis=0, scale=1.0 / avg)
+        _, J = k17_steps(x, L, spin_up + avg, checkpoint_every=10, even=True, bc_factory=fac, porosity=phi,
+                         observe=obs, observe_from=spin_up)
     else:
         s = k17_steps(x, L, spin_up, checkpoint_every=10, even=True, bc_factory=fac, porosity=phi)
         even = spin_up % 2 == 0

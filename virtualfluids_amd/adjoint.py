@@ -486,6 +486,31 @@ class KineticEnergyObservation:
         return self.value(buf.reshape(-1)[self.index[bool(even)]], even)
 
 
+class MomentumExchangeObservation:
+    """scale * component `axis` of the momentum-exchange force on a BC list (momentum_exchange):
+    an observation for k17_steps / k17_two_level_steps that reads only the link cells, so its
+    gradient is scattered into the adjoint without touching the rest of the buffer."""
+
+    def __init__(self, grid, bc_name: str, axis: int = 0, scale: float = 1.0, device="cuda"):
+        self.index, self.weight = {}, {}
+        for e in (True, False):
+            out_idx, in_idx, c = momentum_exchange_index(grid, bc_name, e)
+            self.index[e] = torch.from_numpy(np.concatenate([out_idx, in_idx])).to(device)
+            self.weight[e] = torch.from_numpy(np.concatenate([c[axis], c[axis]])).to(device)
+        self.scale = float(scale)
+
+    def cells(self, even: bool) -> torch.Tensor:
+        """Flat indices of the cells the observation reads in a state of parity `even`."""
+        return self.index[bool(even)]
+
+    def value(self, f: torch.Tensor, even: bool) -> torch.Tensor:
+        """The observation from the values of its cells."""
+        return self.scale * (self.weight[bool(even)] * f.to(torch.float64)).sum()
+
+    def __call__(self, buf: torch.Tensor, even: bool) -> torch.Tensor:
+        return self.value(buf.reshape(-1)[self.index[bool(even)]], even)
+
+
 def _cell_index(grid, k: np.ndarray, d: int, even: bool) -> np.ndarray:
     """Flat buffer index of the cell holding pre-collision direction d of nodes k."""
     loc = k.astype(np.int64)
