@@ -32,8 +32,8 @@ constexpr int kRowSplit = 14;
 // xcd_block the XCDs take contiguous ranges of R blocks instead (XCD j: [j q + min(j, R % 8),
 // ...) with q = R / 8), so that neighbouring rows meet in one L2. Rows whose bytes are not a
 // whole number of 128-byte lines share a line at either end with the rows beside them in y and
-// z; on MI355X this saves those lines' second trip to HBM (258-node FP16 rows 12 % faster,
-// 66-node FP32 rows 21 %), while aligned rows run 2-5 % slower with it.
+// z, which then meet in one L2 instead of two (on MI355X 258-node FP16 rows run 12 % faster
+// with it, 66-node FP32 rows 21 %, while aligned rows run 2-5 % slower).
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t R)
 {
     const uint32_t q = R >> 3, rem = R & 7, j = b & 7;
