@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define VF_ABI_VERSION 3
+#define VF_ABI_VERSION 4
 
 enum vf_status {
     VF_OK = 0,
@@ -51,6 +51,38 @@ enum vf_collision_template {
 #define VF_GEO_SOLID 15u
 #define VF_GEO_VOID 16u
 #define VF_GEO_FLUID 19u
+
+/*
+ * Kernel choice of the dense-box collision (vf_k17_run_box, vf_k17_run_box_into) and its adjoint
+ * (vf_k17_adjoint_box). No counterpart in the reference. All fields zero (the default of a
+ * zero-initialised vf_lbm_parameter) selects automatically, by row length, storage type and
+ * workingSetBytes; any other value pins one form, for benchmarks and tests. A pinned form that
+ * does not fit the box (a row width that does not divide nx, or rows longer than one workgroup)
+ * falls back to the one-node-per-thread kernel. The results agree to FP32 round-off.
+ */
+enum vf_box_kernel {
+    VF_BOX_KERNEL_AUTO = 0,
+    VF_BOX_KERNEL_ROW = 1,  /* one workgroup per x-row (streaming form) */
+    VF_BOX_KERNEL_NODE = 2  /* one node per thread (cache-resident form) */
+};
+enum vf_row_addressing {
+    VF_ROW_ADDRESSING_AUTO = 0,   /* buffer descriptors where 14 direction slots fit in 4 GiB */
+    VF_ROW_ADDRESSING_BUFFER = 1, /* buffer descriptors (only where they fit; else 64-bit) */
+    VF_ROW_ADDRESSING_GLOBAL = 2  /* 64-bit global addresses */
+};
+enum vf_adjoint_kernel {
+    VF_ADJOINT_KERNEL_AUTO = 0,
+    VF_ADJOINT_KERNEL_ROW_WHOLE = 1, /* row kernel over whole rows, without the scalar tail launch */
+    VF_ADJOINT_KERNEL_NODE = 2       /* one node per thread */
+};
+typedef struct vf_kernel_choice {
+    int32_t boxKernel;          /* enum vf_box_kernel */
+    int32_t rowWidth;           /* 0: automatic; 1, 2 or 4 nodes per lane (2 and 4 for FP16/BF16) */
+    int32_t rowWorkgroupsPerCU; /* 0: automatic; -1: no cap; k > 0: at most k row workgroups per CU */
+    int32_t rowAddressing;      /* enum vf_row_addressing (collision and adjoint row kernels) */
+    int32_t rowXcdRanges;       /* 0: automatic; 1: contiguous row ranges per XCD; 2: launch order */
+    int32_t adjointKernel;      /* enum vf_adjoint_kernel */
+} vf_kernel_choice;
 
 /*
  * One grid level: the fields of LBMSimulationParameter the time step reads
@@ -105,6 +137,8 @@ typedef struct vf_lbm_parameter {
      * cache-resident and latency-bound and takes the one-node-per-thread kernel, beyond it the
      * streaming row kernel. */
     uint64_t workingSetBytes;
+    /* Kernel choice of the dense-box collision and its adjoint (zero: automatic; ABI 4). */
+    vf_kernel_choice kernelChoice;
 } vf_lbm_parameter;
 
 /*

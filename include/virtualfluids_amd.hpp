@@ -149,6 +149,10 @@ public:
 
     void attach(LBMSimulationParameter& p) const; /* points p at this level's memory */
     void setBC(vf_q_boundary_conditions& bc, const HostBC& host); /* plain or directional BC list */
+    /* throws std::invalid_argument for a BC list setBC rejects: sizes that do not match, a node
+     * outside [0, numberOfNodes) or a node listed twice (the BC kernels write each
+     * (node, direction) cell from one thread) */
+    static void checkBC(const HostBC& host, uint64_t numberOfNodes);
     /* cells of the interface from this (coarse) level to a finer level of fineNumberOfNodes */
     void setCells(InterpolationCells& cells, ICellNeigh& neigh, bool& hasOffsets, const HostCells& host,
                   uint64_t fineNumberOfNodes);

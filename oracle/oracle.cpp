@@ -90,9 +90,18 @@ template <class T> inline void write_pre(T* const P[27], const u32 n[8], int d, 
 }
 
 // ---- collision over an index list (runCollision) -------------------------------------------
+// Per-node body force of the ApplyBodyForce / AllFeatures / SubDomainBorder templates
+// (RunCollision.cuh:87-99): the half-step shift is (forces + bodyForce[k]) * 1/2 * forceFactor,
+// evaluated in that order, and the node's body force is reset to zero afterwards.
+template <class T> struct BodyForce {
+    T forces[3];   // global force (Parameter::getForcesDev), unscaled
+    T forceFactor; // 2^-level
+    T* bf[3];      // per-node body force, read and cleared
+};
+
 template <class T>
 void collide_indexed(const Lattice<T>& L, const u32* fluid, u64 nfluid, const K17Param<T>& prm0, bool even, T* rho,
-                     T* vx, T* vy, T* vz, const T* porosity = nullptr)
+                     T* vx, T* vy, T* vz, const T* porosity = nullptr, const BodyForce<T>* body = nullptr)
 {
     T* P[27];
     slot_table(L, even, P);
@@ -106,6 +115,12 @@ void collide_indexed(const Lattice<T>& L, const u32* fluid, u64 nfluid, const K1
         Macro<T> m;
         K17Param<T> prm = prm0;
         if (porosity) prm.vscale = T(1) - T(0.5) * porosity[k];
+        if (body) {
+            prm.fx = (body->forces[0] + body->bf[0][k]) * T(0.5) * body->forceFactor;
+            prm.fy = (body->forces[1] + body->bf[1][k]) * T(0.5) * body->forceFactor;
+            prm.fz = (body->forces[2] + body->bf[2][k]) * T(0.5) * body->forceFactor;
+            body->bf[0][k] = body->bf[1][k] = body->bf[2][k] = T(0);
+        }
         k17_collide(f, prm, &m);
         if (rho) {
             rho[k] = m.drho;
@@ -632,6 +647,17 @@ template <class T> K17Param<T> make_param(T omega, const T* lim3, const T* force
     {                                                                                                          \
         Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
         collide_indexed(L, fluid, nfluid, make_param<T>(omega, lim3, force3), even != 0, rho, vx, vy, vz);    \
+    }                                                                                                          \
+    extern "C" void orc_collide_bodyforce_##SFX(T* buf, unsigned long long N, const uint32_t* nbx,            \
+                                                const uint32_t* nby, const uint32_t* nbz, const uint32_t* fluid, \
+                                                unsigned long long nfluid, T omega, const T* lim3,           \
+                                                const T* forces3, T forceFactor, T* bfx, T* bfy, T* bfz,     \
+                                                int even, T* rho, T* vx, T* vy, T* vz)                       \
+    {                                                                                                          \
+        Lattice<T> L{buf, N, nbx, nby, nbz};                                                                   \
+        const BodyForce<T> body{{forces3[0], forces3[1], forces3[2]}, forceFactor, {bfx, bfy, bfz}};           \
+        collide_indexed(L, fluid, nfluid, make_param<T>(omega, lim3, nullptr), even != 0, rho, vx, vy, vz,     \
+                        (const T*)nullptr, &body);                                                              \
     }                                                                                                          \
     extern "C" void orc_collide_porous_##SFX(T* buf, unsigned long long N, const uint32_t* nbx,               \
                                              const uint32_t* nby, const uint32_t* nbz, const uint32_t* fluid,  \

@@ -122,6 +122,23 @@ class Lattice:
                                         _p(lim), _p(frc), ctypes.c_int(int(self.even)), _p(r), _p(vx), _p(vy),
                                         _p(vz))
 
+    def collide_body_force(self, fluid, omega, forces, force_factor, body_force, limiters=(0.01, 0.01, 0.01),
+                           macro=None):
+        """runCollision<..., applyBodyForce=true> (RunCollision.cuh:87-99): per node the half-step
+        shift (forces + body_force[k]) * 1/2 * force_factor; body_force (three arrays of N values,
+        modified in place) is reset to zero at every collided node."""
+        fluid = np.ascontiguousarray(fluid, dtype=np.uint32)
+        lim = np.asarray(limiters, dtype=self.dtype)
+        frc = np.asarray(forces, dtype=self.dtype)
+        for b in body_force:
+            assert b.dtype == self.dtype and b.flags.c_contiguous and b.size == self.N
+        r = vx = vy = vz = None
+        if macro is not None:
+            r, vx, vy, vz = macro
+        self._fn("orc_collide_bodyforce")(*self._nb(), _p(fluid), ctypes.c_ulonglong(fluid.size), self._T(omega),
+                                          _p(lim), _p(frc), self._T(force_factor), *(_p(b) for b in body_force),
+                                          ctypes.c_int(int(self.even)), _p(r), _p(vx), _p(vy), _p(vz))
+
     def bc_post(self, kind, k, q, omega=0.0, velocity=None):
         k = np.ascontiguousarray(k, dtype=np.int32)
         q = np.ascontiguousarray(q, dtype=self.dtype)

@@ -181,6 +181,24 @@ static void bc_manager_tests()
     }
 }
 
+// ---- BC lists checked on the host before any upload -------------------------------------------
+static void bc_list_tests()
+{
+    DeviceLevel::HostBC bc;
+    bc.k = {3, 7, 5};
+    bc.q27.assign(27 * 3, -1.0f);
+    EXPECT_NO_THROW(DeviceLevel::checkBC(bc, 8));
+    EXPECT_THROW(DeviceLevel::checkBC(bc, 7), std::invalid_argument); // node 7 out of range
+    bc.k = {3, 7, 3};
+    EXPECT_THROW(DeviceLevel::checkBC(bc, 8), std::invalid_argument); // node 3 twice
+    bc.k = {3, 7, 5};
+    bc.Vx = {0.1f, 0.2f};
+    EXPECT_THROW(DeviceLevel::checkBC(bc, 8), std::invalid_argument); // one value short
+    bc.Vx.clear();
+    bc.q27.pop_back();
+    EXPECT_THROW(DeviceLevel::checkBC(bc, 8), std::invalid_argument);
+}
+
 // ---- GridScalingKernelManagerTest_Initialization -------------------------------------------
 static void scaling_manager_tests()
 {
@@ -475,6 +493,8 @@ static int print_layout()
     VF_FIELD(vf_lbm_parameter, fluidNodeIndices);
     VF_FIELD(vf_lbm_parameter, numberOfFluidNodes);
     VF_FIELD(vf_lbm_parameter, porosity);
+    VF_FIELD(vf_lbm_parameter, workingSetBytes);
+    VF_FIELD(vf_lbm_parameter, kernelChoice);
     VF_FIELD(vf_q_boundary_conditions, k);
     VF_FIELD(vf_q_boundary_conditions, q27);
     VF_FIELD(vf_q_boundary_conditions, numberOfBCnodes);
@@ -496,6 +516,7 @@ int main(int argc, char** argv)
     if (mode == "factories") {
         bc_factory_tests();
         bc_manager_tests();
+        bc_list_tests();
         scaling_manager_tests();
         kernel_factory_tests();
         std::printf("%s: %d failure(s)\n", mode.c_str(), failures);

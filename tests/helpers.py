@@ -140,3 +140,10 @@ def shear_wave_init(nb, amp):
     init_c = (np.zeros(Nc), amp * np.sin(k * zc), np.zeros(Nc), np.zeros(Nc))
     init_f = (np.zeros(Nf), amp * np.sin(k * zf), np.zeros(Nf), np.zeros(Nf))
     return k, init_c, init_f
+
+
+def conserved_sums(buf, grid, even):
+    """FP64 sums of drho and momentum over the fluid nodes of a host buffer (the quantities
+    vf_MomentSums reduces on the GPU), read with the given parity."""
+    f = unpack_pre(np.asarray(buf, dtype=np.float64), grid, even)[grid.geo == 19]
+    return np.array([f.sum(), (f @ CX).sum(), (f @ CY).sum(), (f @ CZ).sum()])

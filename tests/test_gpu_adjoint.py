@@ -37,14 +37,15 @@ def _grid(kind):
 
 # box: the row kernel (buffer addressing); box_global: its global-address form; box_scalar:
 # the one-node-per-thread kernel; indexed: fluid-node list and neighbour arrays
-ADJ_ENV = {"box": {}, "box_global": {"VFAMD_ROW_BUFFER": "0"}, "box_scalar": {"VFAMD_ADJ_ROW": "0"}, "indexed": {}}
+# (pinned through vf_lbm_parameter.kernelChoice)
+ADJ_CHOICE = {"box": {}, "box_global": {"rowAddressing": 2}, "box_scalar": {"adjointKernel": 2}, "indexed": {}}
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("path", ["box", "box_global", "box_scalar", "indexed"])
 @pytest.mark.parametrize("even", [True, False])
 @pytest.mark.parametrize("kind", ["periodic", "walled", "long_periodic", "long_walled", "split_walled"])
-def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, monkeypatch, storage, path, even, kind):
+def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, storage, path, even, kind):
     import torch
 
     from virtualfluids_amd import capi
@@ -52,13 +53,14 @@ def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, monkeypatch, storage
 
     if kind.startswith(("long", "split")) and path in ("box_scalar", "indexed") and storage != "fp32":
         pytest.skip("long rows exercise the row kernel; the other paths are covered on the short boxes")
-    for k, v in ADJ_ENV[path].items():
-        monkeypatch.setenv(k, v)
+    choice = ADJ_CHOICE[path]
     path = "indexed" if path == "indexed" else "box"
     grid = _grid(kind)
     rng = np.random.default_rng(500 + 2 * even + (path == "box") + 4 * len(storage))
     omega = 1.8
     L = Lattice(grid, omega, storage=storage, box=(path == "box"))
+    for k, v in choice.items():
+        setattr(L.param.kernelChoice, k, v)
     L.even = even
     L.set_populations(pack_pre(near_equilibrium(rng, grid.N, u_amp=0.05, noise=1e-2), grid, even))
     lam0 = rng.normal(0, 1, 27 * grid.N)
@@ -336,3 +338,118 @@ def test_duct_gradients_match_fp64_finite_differences(gpu, orc, velocity_bc, nos
         ad = float((grad.cpu().numpy().astype(np.float64) * v).sum())
         print(f"{velocity_bc}: d J / d {name}: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
         assert abs(ad - fd) < 1e-4 * abs(fd)
+
+
+def _duct_oracle_J(orc, g, buf, omega, segments, fluid, velocity_bc="VelocityBounceBack"):
+    """FP64 kinetic energy after consecutive segments [(steps, U (3, n))] of the duct run
+    (velocity BC, bounce-back walls, pressure outlet at drho = 0)."""
+    from helpers import ORACLE_POST_BC
+    from virtualfluids_amd.adjoint import pre_collision_index
+    from virtualfluids_amd.grid import CX, CY, CZ
+
+    vel, ns, pr = g.bcs["velocity"], g.bcs["noslip"], g.bcs["pressure"]
+    A = orc.Lattice(g.nbx, g.nby, g.nbz)
+    A.f[:] = buf
+    for n, U in segments:
+        for _ in range(n):
+            A.collide(g.fluid, omega)
+            A.bc_post(ORACLE_POST_BC[velocity_bc], vel.k, vel.q27, omega, tuple(U))
+            A.bc_post(ORACLE_POST_BC["NoSlipBounceBack"], ns.k, ns.q27, omega)
+            A.swap()
+            A.bc_pressure(pr.k, pr.kN, np.asarray(pr.rho_bc, dtype=np.float64), pr.direction)
+    f = A.f[pre_collision_index(g, A.even)]
+    j = np.stack([CX, CY, CZ]).astype(np.float64) @ f
+    return (0.5 * (j * j).sum(0) / (1 + f.sum(0)))[fluid].sum()
+
+
+def test_chained_segments_with_different_velocities(gpu, orc):
+    """Two k17_steps segments with different inflow velocities, chained and differentiated
+    together: each segment's backward must replay with its own wall velocities (the lattice holds
+    the second segment's when both run backward). Gradients with respect to the initial state and
+    to both velocity sets against FP64 central differences of the oracle's run."""
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_steps, kinetic_energy, pre_collision_index
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory
+
+    g = _duct(False)
+    vel = g.bcs["velocity"]
+    omega, n1, n2 = 1.8, 9, 8
+    rng = np.random.default_rng(740)
+    f0 = pack_pre(near_equilibrium(rng, g.N, u_amp=0.03, noise=2e-3), g, True)
+    base = np.array(vel.velocity, dtype=np.float64)
+    U1 = (base * 1.5).astype(np.float32).astype(np.float64)
+    U2 = (base * 0.4 + 0.01 * rng.normal(0, 1, base.shape)).astype(np.float32).astype(np.float64)
+    fluid = g.geo == 19
+    F = BoundaryConditionFactory
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityBounceBack)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipBounceBack)
+    fac.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    L = Lattice(g, omega)
+    x = torch.from_numpy(f0.astype(np.float32)).cuda().requires_grad_(True)
+    u1 = torch.from_numpy(U1.astype(np.float32)).cuda().requires_grad_(True)
+    u2 = torch.from_numpy(U2.astype(np.float32)).cuda().requires_grad_(True)
+    mid = k17_steps(x, L, n1, checkpoint_every=4, even=True, bc_factory=fac, velocity=u1)
+    out = k17_steps(mid, L, n2, checkpoint_every=3, even=(n1 % 2 == 0), bc_factory=fac, velocity=u2)
+    idx = torch.from_numpy(pre_collision_index(g, (n1 + n2) % 2 == 0)).cuda()
+    nodes = torch.from_numpy(np.nonzero(fluid)[0]).cuda()
+    J = kinetic_energy(out, idx, nodes)
+    J.backward()
+    # the lattice's velocity arrays hold the second segment's values again after the backward
+    assert np.array_equal(torch.stack(L.bcs["velocity"].vel).cpu().numpy(), U2.astype(np.float32))
+
+    h = 1e-7
+    for name, grad in (("f0", x.grad), ("U1", u1.grad), ("U2", u2.grad)):
+        v = rng.normal(0, 1, f0.shape if name == "f0" else base.shape)
+        if name == "f0":
+            args_p = (f0 + h * v, [(n1, U1), (n2, U2)])
+            args_m = (f0 - h * v, [(n1, U1), (n2, U2)])
+        elif name == "U1":
+            args_p = (f0, [(n1, U1 + h * v), (n2, U2)])
+            args_m = (f0, [(n1, U1 - h * v), (n2, U2)])
+        else:
+            args_p = (f0, [(n1, U1), (n2, U2 + h * v)])
+            args_m = (f0, [(n1, U1), (n2, U2 - h * v)])
+        fd = (_duct_oracle_J(orc, g, args_p[0], omega, args_p[1], fluid)
+              - _duct_oracle_J(orc, g, args_m[0], omega, args_m[1], fluid)) / (2 * h)
+        ad = float((grad.cpu().numpy().astype(np.float64) * v).sum())
+        print(f"chained segments: d J / d {name}: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
+        assert abs(ad - fd) < 1e-4 * abs(fd)
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16", 2e-2), ("bf16", 1.5e-1)])
+def test_sixteen_bit_lattice_takes_and_returns_fp32(gpu, orc, storage, tol):
+    """A 16-bit lattice stepped from an FP32 f0: the final buffer and the gradient with respect to
+    f0 are FP32 (no 16-bit rounding of the gradient). The gradient is the adjoint of the stepping
+    at the 16-bit states, so against FP64 central differences of the oracle (no storage rounding)
+    it agrees to the storage precision: tolerance 2e-2 (FP16), 0.15 (BF16) relative."""
+    import torch
+
+    from virtualfluids_amd.adjoint import k17_steps, kinetic_energy, pre_collision_index
+    from virtualfluids_amd.lattice import Lattice
+
+    steps = 12
+    grid = periodic_box(16, 12, 10)
+    rng = np.random.default_rng(750)
+    f0 = pack_pre(near_equilibrium(rng, grid.N, u_amp=0.05, noise=2e-3), grid, True)
+    omega = 1.9
+    L = Lattice(grid, omega, storage=storage)
+    idx = torch.from_numpy(pre_collision_index(grid, steps % 2 == 0)).cuda()
+    x = torch.from_numpy(f0.astype(np.float32)).cuda().requires_grad_(True)
+    out = k17_steps(x, L, steps, checkpoint_every=5, even=True)
+    assert out.dtype == torch.float32
+    J = kinetic_energy(out, idx)
+    J.backward()
+    assert x.grad.dtype == torch.float32
+    grad = x.grad.cpu().numpy().astype(np.float64)
+    v = rng.normal(0, 1, f0.shape)
+    h = 1e-7
+    Jp = _oracle_J(orc, grid, f0 + h * v, omega, steps, steps % 2 == 0)[0]
+    Jm = _oracle_J(orc, grid, f0 - h * v, omega, steps, steps % 2 == 0)[0]
+    fd = (Jp - Jm) / (2 * h)
+    ad = float(grad @ v)
+    rel = abs(ad - fd) / abs(fd)
+    print(f"{storage} lattice, FP32 f0: <grad, v> adjoint {ad:.8e} FD {fd:.8e} rel {rel:.2e}")
+    assert rel < tol

@@ -19,6 +19,13 @@ def _lattice(grid, omega, storage="fp32", box=True, limiters=(0.01, 0.01, 0.01))
     return Lattice(grid, omega, storage=storage, box=box, limiters=limiters)
 
 
+def _choose(L, **kw):
+    """Pin a kernel form through vf_lbm_parameter.kernelChoice (capi.KernelChoice fields)."""
+    for k, v in kw.items():
+        setattr(L.param.kernelChoice, k, v)
+    return L
+
+
 def _oracle_lattice(orc, grid):
     return orc.Lattice(grid.nbx, grid.nby, grid.nbz)
 
@@ -52,25 +59,30 @@ def test_one_step_matches_oracle(gpu, orc, dims, box, even):
 # row kernel widths: 0 = scalar kernel; 1/2/4 nodes per lane (rows of 1-4 waves, the last one
 # possibly partial: 40, 130 and 260 nodes); the row
 # kernel addresses the direction arrays through buffer descriptors up to 4 GiB per 14 slots
-# and through 64-bit global addresses beyond (forced here with VFAMD_ROW_BUFFER=0)
+# and through 64-bit global addresses beyond (pinned here through kernelChoice.rowAddressing)
 @pytest.mark.parametrize("addressing", ["buffer", "global"])
 @pytest.mark.parametrize("width", [0, 1, 2, 4])
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("dims", [(64, 5, 3), (128, 4, 3), (256, 3, 2), (512, 2, 2), (40, 5, 3), (130, 4, 3),
                                   (260, 3, 2)])
-def test_row_kernel_widths_match_oracle(gpu, orc, monkeypatch, width, storage, dims, addressing):
+def test_row_kernel_widths_match_oracle(gpu, orc, width, storage, dims, addressing):
     """Every row-kernel width, storage type, addressing form and row length (1 to 4 waves per
     row, with the periodic wrap) against the FP64 oracle applied to the same stored (rounded)
     input."""
+    from virtualfluids_amd import capi
+
     if width == 0 and addressing == "global":
         pytest.skip("the scalar kernel has one addressing form")
-    monkeypatch.setenv("VFAMD_ROW_WIDTH", str(width))
-    monkeypatch.setenv("VFAMD_ROW_BUFFER", "1" if addressing == "buffer" else "0")
+    choice = dict(rowAddressing=capi.ROW_ADDRESSING_BUFFER if addressing == "buffer" else capi.ROW_ADDRESSING_GLOBAL)
+    if width == 0:
+        choice["boxKernel"] = capi.BOX_KERNEL_NODE
+    else:
+        choice.update(boxKernel=capi.BOX_KERNEL_ROW, rowWidth=width)
     grid = periodic_box(*dims)
     rng = np.random.default_rng(200 + width + sum(dims))
     f0 = near_equilibrium(rng, grid.N, u_amp=0.04, noise=5e-3)
     for even in (True, False):
-        L = _lattice(grid, 1.9, storage=storage)
+        L = _choose(_lattice(grid, 1.9, storage=storage), **choice)
         L.even = even
         L.set_populations(pack_pre(f0, grid, even))
         stored = L.populations().astype(np.float64)  # input after storage rounding
@@ -183,7 +195,7 @@ def test_mass_and_momentum_sums_fp64(gpu, orc):
 @pytest.mark.parametrize("addressing", ["buffer", "global"])
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("dims,solid", [((128, 4, 3), False), ((128, 4, 3), True), ((40, 6, 5), True)])
-def test_out_of_place_step_matches_in_place(gpu, monkeypatch, storage, dims, solid, addressing):
+def test_out_of_place_step_matches_in_place(gpu, storage, dims, solid, addressing):
     """vf_k17_run_box_into (row kernel for rows of whole waves, box kernel otherwise; with and
     without solid nodes) writes exactly the state the in-place step leaves, and leaves its
     input untouched."""
@@ -194,9 +206,9 @@ def test_out_of_place_step_matches_in_place(gpu, monkeypatch, storage, dims, sol
     from virtualfluids_amd import capi
     from virtualfluids_amd.lattice import stream_handle
 
-    monkeypatch.setenv("VFAMD_ROW_BUFFER", "1" if addressing == "buffer" else "0")
-    # a lattice this small would take the scalar kernel (cache-resident working set)
-    monkeypatch.setenv("VFAMD_SMALL_SCALAR", "0")
+    # a lattice this small would take the scalar kernel (cache-resident working set): pin the row kernel
+    choice = dict(boxKernel=capi.BOX_KERNEL_ROW,
+                  rowAddressing=capi.ROW_ADDRESSING_BUFFER if addressing == "buffer" else capi.ROW_ADDRESSING_GLOBAL)
     grid = periodic_box(*dims)
     rng = np.random.default_rng(300 + sum(dims) + solid)
     if solid:
@@ -204,7 +216,7 @@ def test_out_of_place_step_matches_in_place(gpu, monkeypatch, storage, dims, sol
         grid.fluid = np.nonzero(grid.geo == 19)[0].astype(np.uint32)
     f0 = near_equilibrium(rng, grid.N, u_amp=0.04, noise=5e-3)
     for even in (True, False):
-        L = _lattice(grid, 1.8, storage=storage)
+        L = _choose(_lattice(grid, 1.8, storage=storage), **choice)
         assert (L.mask is not None) == solid
         L.even = even
         L.set_populations(pack_pre(f0, grid, even))
@@ -222,7 +234,7 @@ def test_out_of_place_step_matches_in_place(gpu, monkeypatch, storage, dims, sol
 # with solid walls, a porosity field, in place and out of place
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("case", ["periodic130", "periodic260", "walled258", "walled194_porous"])
-def test_partial_rows_match_oracle(gpu, orc, monkeypatch, storage, case):
+def test_partial_rows_match_oracle(gpu, orc, storage, case):
     import ctypes
 
     import torch
@@ -232,8 +244,6 @@ def test_partial_rows_match_oracle(gpu, orc, monkeypatch, storage, case):
     from virtualfluids_amd.grid import walled_box
     from virtualfluids_amd.lattice import stream_handle
 
-    # lattices this small would take the scalar kernel (cache-resident working set): keep the row kernel
-    monkeypatch.setenv("VFAMD_SMALL_SCALAR", "0")
     grid = {"periodic130": lambda: periodic_box(130, 4, 3), "periodic260": lambda: periodic_box(260, 3, 2),
             "walled258": lambda: walled_box((256, 3, 2), {}),
             "walled194_porous": lambda: walled_box((192, 3, 3), {})}[case]()
@@ -241,7 +251,8 @@ def test_partial_rows_match_oracle(gpu, orc, monkeypatch, storage, case):
     f0 = near_equilibrium(rng, grid.N, u_amp=0.04, noise=5e-3)
     phi = rng.uniform(0, 0.1, grid.N) if "porous" in case else None
     for even in (True, False):
-        L = _lattice(grid, 1.9, storage=storage)
+        # a lattice this small would take the scalar kernel (cache-resident working set): pin the row kernel
+        L = _choose(_lattice(grid, 1.9, storage=storage), boxKernel=capi.BOX_KERNEL_ROW)
         if phi is not None:
             L.set_porosity(torch.from_numpy(phi.astype(np.float32)).cuda())
         L.even = even

@@ -122,6 +122,74 @@ def test_two_level_steps_match_oracle(gpu, orc):
     assert err_c < 5e-7 and err_f < 5e-7
 
 
+def _oracle_multilevel_rounded(orc, nb, steps, buf_c, buf_f, rounding):
+    """The two-level update of helpers.oracle_multilevel in FP64 arithmetic, with every level's
+    buffer rounded to the 16-bit storage type after each kernel (`rounding`: numpy dtype), as
+    the GPU stores it."""
+    C, F = _oracle_levels(orc, nb)
+    C.f[:] = buf_c
+    F.f[:] = buf_f
+
+    def rnd(L):
+        L.f[:] = L.f.astype(rounding).astype(np.float64)
+
+    zc = [np.zeros(nb.cf.n)] * 3
+    zf = [np.zeros(nb.fc.n)] * 3
+    for _ in range(steps):
+        for _ in range(2):
+            F.collide(nb.fine.fluid, OMEGA_F)
+            F.swap()
+            rnd(F)
+        C.collide(nb.coarse.fluid, OMEGA_C)
+        C.swap()
+        rnd(C)
+        orc.scale(True, C, F, C.even, nb.fc.coarse, nb.fc.fine, zf, OMEGA_C, OMEGA_F)
+        rnd(C)
+        orc.scale(False, C, F, C.even, nb.cf.coarse, nb.cf.fine, zc, OMEGA_C, OMEGA_F)
+        rnd(F)
+    return C, F
+
+
+def test_two_level_steps_match_oracle_fp16(gpu, orc):
+    """Five coarse steps of the two-level update with FP16 shifted storage on both levels (the
+    storage of the two-level channel benchmark) against the oracle from the same stored initial
+    state: (a) the oracle rounding each level to FP16 after every kernel, as the GPU stores it,
+    agrees to 4 FP16 units in the last place of the largest population (0.06 * 2^-10: FP32
+    against FP64 arithmetic flips a rounding now and then, and a flipped unit spreads through
+    the later steps); (b) the oracle without storage rounding agrees to 3e-4 absolute (ten fine
+    and five coarse roundings of at most half a unit, 0.06 * 2^-12, with margin) and to 2e-3 in
+    the relative L2 norm of the velocity field."""
+    from virtualfluids_amd.refinement import GridScalingFactory
+    from virtualfluids_amd.simulation import MultiLevelSimulation
+
+    nb = nested_box((12, 12, 16), (2, 2, 3), (9, 9, 12))
+    k, init_c, init_f = shear_wave_init(nb, 0.03)
+    C, F = _levels(nb, "fp16")
+    C.init_equilibrium(*init_c)
+    F.init_equilibrium(*init_f)
+    buf_c, buf_f = C.populations().astype(np.float64), F.populations().astype(np.float64)
+    fac = GridScalingFactory()
+    fac.set_scaling_factory(GridScalingFactory.GridScaling.ScaleCompressible)
+    MultiLevelSimulation([C, F], [(nb.cf, nb.fc)], scaling_factory=fac).run(5)
+    got_c, got_f = C.populations(), F.populations()
+    rc, rf = _oracle_multilevel_rounded(orc, nb, 5, buf_c, buf_f, np.float16)
+    err_c, err_f = np.abs(got_c - rc.f).max(), np.abs(got_f - rf.f).max()
+    print(f"two-level FP16, rounding oracle: err coarse {err_c:.3e} fine {err_f:.3e}")
+    assert err_c < 0.06 * 2.0 ** -10 * 4 and err_f < 0.06 * 2.0 ** -10 * 4
+    oc, of = _oracle_multilevel_rounded(orc, nb, 5, buf_c, buf_f, np.float64)
+    err_c, err_f = np.abs(got_c - oc.f).max(), np.abs(got_f - of.f).max()
+    print(f"two-level FP16, FP64 oracle: err coarse {err_c:.3e} fine {err_f:.3e}")
+    assert err_c < 3e-4 and err_f < 3e-4
+    valid = nb.coarse_valid()
+    for L, R, sel in ((C, oc, valid), (F, of, slice(None))):
+        g = L.macroscopic()
+        r = R.macroscopic()
+        du = np.sqrt(sum(((g[i] - r[i])[sel] ** 2).sum() for i in (1, 2, 3)))
+        un = np.sqrt(sum((r[i][sel] ** 2).sum() for i in (1, 2, 3)))
+        print(f"  rel L2(u) {du / un:.2e}")
+        assert du / un < 2e-3
+
+
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
 def test_uniform_flow_stays_uniform(gpu, storage):
     from virtualfluids_amd.refinement import GridScalingFactory

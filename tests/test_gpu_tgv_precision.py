@@ -80,10 +80,14 @@ def test_tgv_storage_precision_against_fp64(gpu, orc):
     # same resolution: the storage type's own velocity error against FP64
     u64, ke64 = _oracle_tgv(orc, 128, 500, 10)
     same = {}
+    decay64 = 1.0 - ke64[-1] / ke64[0]
     for storage in ("fp32", "fp16", "bf16"):
         u, ke = _gpu_tgv(128, storage, 500, 10)
         same[storage] = {"u_rel_l2": _rel_l2(u, u64),
-                         "ke_max_rel": float(np.abs(ke / ke64 - 1.0).max())}
+                         "ke_max_rel": float(np.abs(ke / ke64 - 1.0).max()),
+                         "ke_decay": float(1.0 - ke[-1] / ke[0]),
+                         "ke_decay_rel_err_vs_fp64": float(abs((1.0 - ke[-1] / ke[0]) / decay64 - 1.0))}
+    same["fp64_ke_decay"] = float(decay64)
     report["128^3_500_steps_vs_fp64_128^3"] = same
     print(json.dumps(report), flush=True)
     # 512^3 at the same convective time
@@ -96,10 +100,14 @@ def test_tgv_storage_precision_against_fp64(gpu, orc):
         print(f"512^3 {storage} done", flush=True)
     del grid, fields
     for storage, (u, ke) in runs.items():
+        decay = 1.0 - ke[-1] / ke[0]
         big[storage] = {"u_rel_l2_vs_fp64_128": _rel_l2(u, u64),
                         "u_rel_l2_vs_fp32_512": _rel_l2(u, runs["fp32"][0]),
                         "ke_max_rel_vs_fp32_512": float(np.abs(ke / runs["fp32"][1] - 1.0).max()),
-                        "ke_end_over_start": float(ke[-1] / ke[0])}
+                        "ke_end_over_start": float(ke[-1] / ke[0]),
+                        # the storage noise against the physical decay: resolved when the decay
+                        # has the sign and half the size of the FP32 run's
+                        "resolves_ke_decay": bool(decay > 0.5 * (1.0 - runs["fp32"][1][-1] / runs["fp32"][1][0]))}
     big["fp64_128_ke_end_over_start"] = float(ke64[-1] / ke64[0])
     report["512^3_2000_steps"] = big
     print(json.dumps(report), flush=True)
@@ -112,6 +120,16 @@ def test_tgv_storage_precision_against_fp64(gpu, orc):
     assert same["fp16"]["u_rel_l2"] < 1e-2
     assert same["bf16"]["u_rel_l2"] < 1.5e-1
     assert same["fp32"]["ke_max_rel"] < 2e-4
+    # at 128^3 FP16 storage resolves the kinetic-energy decay (0.46 % over 500 steps): the sign,
+    # and the size within 10 % of FP64's (FP32 within 1 %); BF16 does not (its energy error of
+    # ~5e-3 is the size of the decay itself)
+    assert decay64 > 0
+    assert same["fp32"]["ke_decay"] > 0 and same["fp32"]["ke_decay_rel_err_vs_fp64"] < 1e-2
+    assert same["fp16"]["ke_decay"] > 0 and same["fp16"]["ke_decay_rel_err_vs_fp64"] < 1e-1
+    # at 512^3 and 2000 steps the FP32 run resolves the same 0.46 % decay; FP16 storage does not
+    # (its rounding noise adds energy and the curve ends above its start: reported as
+    # resolves_ke_decay, not asserted); FP16 stays within 2 % of FP32 in kinetic energy
+    assert big["fp32"]["resolves_ke_decay"] and abs(big["fp32"]["ke_end_over_start"] - ke64[-1] / ke64[0]) < 2e-3
     assert big["fp32"]["u_rel_l2_vs_fp64_128"] < 3e-3
     assert big["fp16"]["u_rel_l2_vs_fp32_512"] < 5e-2
     assert big["bf16"]["u_rel_l2_vs_fp32_512"] < 2e-1

@@ -134,3 +134,63 @@ def test_two_level_gradients_match_fp64_differences(gpu, orc, K, observe):
     ad = float(gphi @ w)
     print(f"  dJ/dphi_fine . w: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
     assert abs(ad - fd) < 1e-4 * abs(fd)
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16", 2e-2)])
+def test_two_level_sixteen_bit_gradients_match_fp64_differences(gpu, orc, storage, tol):
+    """The two-level gradients with FP16 shifted storage on both levels (the storage of the
+    two-level channel benchmark), from FP32 initial buffers: the adjoint linearises the
+    FP16-stored trajectory, whose states differ from the FP64 run by the storage rounding
+    (2^-11 relative per store), so against FP64 central differences of the oracle the gradients
+    agree to the stated tolerance (2e-2 relative), not to the FP32 bound of 1e-4."""
+    import torch
+
+    from virtualfluids_amd.adjoint import KineticEnergyObservation, k17_two_level_steps
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory
+
+    nb, plane, phi0 = _case()
+    Fac = BoundaryConditionFactory
+    fac = Fac()
+    fac.set_velocity_boundary_condition(Fac.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(Fac.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(Fac.PressureBC.PressureNonEquilibriumCompressible)
+    # FP32 initial buffers (the uniform flow), stepped on 16-bit lattices
+    init = []
+    for g, om, lev in ((nb.coarse, OMEGA_C, 0), (nb.fine, OMEGA_F, 1)):
+        L32 = Lattice(g, om, level=lev)
+        z = np.zeros(L32.N)
+        L32.init_equilibrium(z, np.full(L32.N, 0.03), z, z)
+        init.append(L32.f.detach().clone())
+    C = Lattice(nb.coarse, OMEGA_C, storage=storage, level=0)
+    F = Lattice(nb.fine, OMEGA_F, storage=storage, level=1)
+    f0c, f0f = (t.cpu().numpy().astype(np.float64) for t in init)
+    vel = nb.coarse.bcs["velocity"]
+    u0 = np.stack([np.asarray(v, dtype=np.float64) for v in vel.velocity])
+    U = torch.from_numpy(u0.astype(np.float32)).cuda().requires_grad_(True)
+    phi = torch.from_numpy(phi0.astype(np.float32)).cuda().requires_grad_(True)
+    x0c = init[0].clone().requires_grad_(True)
+    obs = KineticEnergyObservation(nb.coarse, plane, scale=1.0 / AVG)
+    c, f, J = k17_two_level_steps(x0c, init[1].clone(), C, F, (nb.cf, nb.fc), SPIN + AVG, checkpoint_every=5,
+                                  even=True, bc_factory=fac, velocity=U, porosity_fine=phi, observe=obs,
+                                  observe_from=SPIN)
+    assert c.dtype == f.dtype == torch.float32
+    J.backward()
+    assert x0c.grad.dtype == torch.float32
+    J_or = _oracle_objective(orc, nb, plane, f0c, f0f, u0[0], phi0)
+    print(f"{storage}: J gpu {float(J.detach()):.10e} oracle {J_or:.10e}")
+    assert abs(float(J.detach()) - J_or) < tol * abs(J_or)
+    rng = np.random.default_rng(1502)
+    h = 1e-7
+    v = rng.normal(size=u0.shape[1])
+    fd = (_oracle_objective(orc, nb, plane, f0c, f0f, u0[0] + h * v, phi0) -
+          _oracle_objective(orc, nb, plane, f0c, f0f, u0[0] - h * v, phi0)) / (2 * h)
+    ad = float(U.grad.cpu().numpy().astype(np.float64)[0] @ v)
+    print(f"  {storage} dJ/dU_x . v: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) < tol * abs(fd)
+    w = rng.normal(size=phi0.size)
+    fd = (_oracle_objective(orc, nb, plane, f0c, f0f, u0[0], phi0 + h * w) -
+          _oracle_objective(orc, nb, plane, f0c, f0f, u0[0], phi0 - h * w)) / (2 * h)
+    ad = float(phi.grad.cpu().numpy().astype(np.float64) @ w)
+    print(f"  {storage} dJ/dphi_fine . w: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
+    assert abs(ad - fd) < tol * abs(fd)

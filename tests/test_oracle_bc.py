@@ -5,7 +5,9 @@
   half a spacing outside the last fluid nodes, with q-interpolated walls at distance q;
 * bounce-back in a closed box conserves total mass;
 * a uniform inflow through a channel with a pressure outlet reaches the inflow velocity
-  and the prescribed outlet density.
+  and the prescribed outlet density;
+* the velocity BC with pressure term returns zero on every link of a resting uniform state,
+  so the first step's mass loss is a closed form, and it relaxes a closed box to drho = 0.
 """
 import numpy as np
 import pytest
@@ -94,3 +96,68 @@ def test_cavity_lid_drives_flow(orc):
     top = (grid.geo == 19) & (z == grid.nz - 2)
     bottom = (grid.geo == 19) & (z == 1)
     assert ux[top].mean() > 0.01 and ux[bottom].mean() < 0.0
+
+
+def _vwp_box(q_rng=None):
+    """Closed box: interpolated no-slip on five sides, a resting VelocityWithPressure wall on
+    z-max (random sub-grid distances when q_rng is given)."""
+    from virtualfluids_amd.grid import dM00, dP00, d0M0, d0P0
+
+    walls = {s: ("noslip",) for s in (dP00, dM00, d0P0, d0M0, d00M)}
+    walls[d00P] = ("velocity", (0.0, 0.0, 0.0))
+    grid = walled_box((6, 5, 4), walls)
+    if q_rng is not None:
+        vel = grid.bcs["velocity"]
+        q = vel.q27.copy()
+        act = (q >= 0) & (q <= 1)
+        q[act] = q_rng.uniform(0.05, 1.0, int(act.sum()))
+        vel.q27 = q
+    return grid
+
+
+@pytest.mark.parametrize("drho0", [0.02, -0.015])
+def test_velocity_with_pressure_first_step_mass_closed_form(orc, drho0):
+    """VelocityWithPressureInterpolatedCompressible (KernelUtilities.h:208-214) is the velocity
+    BC minus w_d drho. On a resting uniform state (f = f_inv = f_eq = w drho, u = 0) the velocity
+    BC returns w_d drho for every q, so the pressure term makes each active link return 0: after
+    one step the fluid mass differs from the velocity BC's by exactly -drho0 * sum_links w_d. A
+    flipped sign gives +, a dropped term 0, a factor error a multiple."""
+    from virtualfluids_amd.grid import W
+
+    grid = _vwp_box(np.random.default_rng(11))
+    N = grid.N
+    init = (np.full(N, drho0), np.zeros(N), np.zeros(N), np.zeros(N))
+    vel = grid.bcs["velocity"]
+    act = ((vel.q27 >= 0) & (vel.q27 <= 1)).reshape(27, vel.n)
+    wsum = float((W[:, None] * act).sum())
+    m = {}
+    for kind in ("VelocityInterpolatedCompressible", "VelocityWithPressureInterpolatedCompressible"):
+        L = oracle_run(orc, grid, 1.8, 1, velocity_bc=kind, noslip_bc="NoSlipInterpolatedCompressible", init=init)
+        m[kind] = L.macroscopic(grid.geo)[0][grid.geo == 19].sum()
+    m0 = drho0 * grid.fluid.size
+    assert abs(m["VelocityInterpolatedCompressible"] - m0) < 1e-14  # a resting wall conserves mass
+    dm = m["VelocityWithPressureInterpolatedCompressible"] - m["VelocityInterpolatedCompressible"]
+    assert abs(dm - (-drho0 * wsum)) < 1e-14, (dm, -drho0 * wsum)
+
+
+@pytest.mark.parametrize("drho0", [0.02, -0.015])
+def test_velocity_with_pressure_wall_relaxes_density_to_reference(orc, drho0):
+    """The pressure term pins the wall at drho = 0: a closed box at rest with uniform drho0 drains
+    (or fills) monotonically towards drho = 0 (omega 1.8, q = 1/2), where a flipped sign would
+    drive it away."""
+    grid = _vwp_box()
+    N = grid.N
+    L = orc.Lattice(grid.nbx, grid.nby, grid.nbz)
+    L.init_eq(np.full(N, drho0), np.zeros(N), np.zeros(N), np.zeros(N), geo=grid.geo)
+    vel, ns = grid.bcs["velocity"], grid.bcs["noslip"]
+    fl = grid.geo == 19
+    mass = [L.macroscopic(grid.geo)[0][fl].sum()]
+    for _ in range(300):
+        L.collide(grid.fluid, 1.8)
+        L.bc_post(orc.BC_VELOCITY_PRESSURE_INTERPOLATED, vel.k, vel.q27, 1.8, vel.velocity)
+        L.bc_post(orc.BC_NOSLIP_INTERPOLATED, ns.k, ns.q27, 1.8)
+        L.swap()
+        mass.append(L.macroscopic(grid.geo)[0][fl].sum())
+    mass = np.abs(np.array(mass))
+    assert np.all(np.diff(mass) < 0)
+    assert mass[-1] < 1e-5 * mass[0]

@@ -44,6 +44,37 @@ def _vp(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def _bc_snapshot(L: Lattice):
+    """Detached copies of the parameter arrays a step reads from the lattice: the velocity BC's
+    wall velocities, the pressure BC's outlet densities and the porosity field (None where the
+    lattice has none)."""
+    vel, pr = L.bcs.get("velocity"), L.bcs.get("pressure")
+    por = getattr(L, "porosity", None)
+    return (None if vel is None or vel.vel is None else [v.detach().clone() for v in vel.vel],
+            None if pr is None or pr.rho_bc is None else pr.rho_bc.detach().clone(),
+            None if por is None else por.detach().clone())
+
+
+def _bc_restore(L: Lattice, snap):
+    """Write a _bc_snapshot back into the lattice's parameter arrays."""
+    vel, rho, por = snap
+    if vel is not None:
+        for dst, src in zip(L.bcs["velocity"].vel, vel):
+            dst.copy_(src)
+    if rho is not None:
+        L.bcs["pressure"].rho_bc.copy_(rho)
+    if por is not None or getattr(L, "porosity", None) is not None:
+        L.set_porosity(por)
+
+
+def _storage_input(t: torch.Tensor, L: Lattice, what: str) -> torch.Tensor:
+    """A working copy of an initial buffer in the lattice's storage type. The buffer may hold the
+    storage type itself or FP32 (for 16-bit lattices: then the outputs and the gradient stay FP32)."""
+    if t.numel() != L.f.numel() or not t.is_cuda or t.dtype not in (L.f.dtype, torch.float32):
+        raise ValueError(f"{what} must be a device buffer of the lattice's size, in its storage type or FP32")
+    return t.detach().to(L.f.dtype, copy=True)
+
+
 class Stepper:
     """One time step of a Lattice on any buffer, and its adjoint, with the BC functions a
     BoundaryConditionFactory selects (the same selection and checks as Simulation)."""
@@ -120,9 +151,8 @@ class _K17Steps(torch.autograd.Function):
     def forward(ctx, f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs, observe,
                 observe_from):
         L = stepper.L
-        if f0.dtype != L.f.dtype or f0.numel() != L.f.numel() or not f0.is_cuda:
-            raise ValueError("f0 must be a device buffer of the lattice's storage type and size")
-        stream = torch.cuda.current_stream()
+        # in place on one buffer: the same arithmetic as the out-of-place replay of the backward
+        buf = _storage_input(f0, L, "f0")
         if velocity is not None:
             bc = L.bcs["velocity"]
             for i in range(3):
@@ -131,8 +161,9 @@ class _K17Steps(torch.autograd.Function):
             L.bcs["pressure"].rho_bc.copy_(rho_bc.detach().to(torch.float32))
         if porosity is not None:
             L.set_porosity(porosity.detach())
-        # in place on one buffer: the same arithmetic as the out-of-place replay of the backward
-        buf = f0.detach().clone()
+        # the values this forward used: the backward replays with them, whatever the lattice
+        # holds by then (another segment or objective may have set its own)
+        ctx.params = _bc_snapshot(L)
         run = _StepGraphs(lambda ev, st: stepper.forward(buf, ev, None, st), graphs)
         checkpoints = []
         even = even0
@@ -158,15 +189,26 @@ class _K17Steps(torch.autograd.Function):
         ctx.observe, ctx.observe_from = observe, observe_from
         ctx.has_velocity, ctx.has_rho = velocity is not None, rho_bc is not None
         ctx.has_porosity = porosity is not None
-        return buf, J
+        ctx.in_dtype = f0.dtype
+        return buf.to(f0.dtype), J
 
     @staticmethod
     def backward(ctx, grad_out, gJ):
+        L = ctx.stepper.L
+        current = _bc_snapshot(L)
+        _bc_restore(L, ctx.params)
+        try:
+            return _K17Steps._backward(ctx, grad_out, gJ)
+        finally:
+            _bc_restore(L, current)
+
+    @staticmethod
+    def _backward(ctx, grad_out, gJ):
         stepper, steps, K = ctx.stepper, ctx.steps, ctx.K
         L = stepper.L
         stream = torch.cuda.current_stream()
         if grad_out is None:
-            grad_out = torch.zeros_like(ctx.checkpoints[0][2])
+            grad_out = torch.zeros(ctx.checkpoints[0][2].shape, dtype=ctx.in_dtype, device=L.device)
         lam = grad_out.detach().to(torch.float32).contiguous().clone()
         observe = ctx.observe if gJ is not None else None
         vel_bc, pr_bc = L.bcs.get("velocity"), L.bcs.get("pressure")
@@ -189,7 +231,7 @@ class _K17Steps(torch.autograd.Function):
             del states
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
-        return lam.to(grad_out.dtype), gv, rbar, padj, None, None, None, None, None, None, None
+        return lam.to(ctx.in_dtype), gv, rbar, padj, None, None, None, None, None, None, None
 
 
 def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: int = 10, even: bool | None = None,
@@ -199,6 +241,11 @@ def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: 
     """`steps` time steps of `lattice` from buffer f0, differentiable in f0 and, when given, in
     the velocity BC's wall velocities `velocity` (3, n), the pressure BC's densities `rho_bc`
     and the Brinkman porosity per node `porosity` (N).
+
+    f0 holds the lattice's storage type or FP32: a 16-bit lattice then stores f0 rounded, steps
+    in its storage type and returns the final buffer and the gradient with respect to f0 in
+    FP32. The backward replays with the wall velocities, outlet densities and porosity this
+    call used, so segments with different values can be chained and differentiated together.
 
     The lattice's own buffer and parity are not touched (its BC parameter arrays take the given
     values); `even` is the parity of the first step (default: the lattice's current parity).
@@ -235,10 +282,8 @@ class _TwoLevelSteps(torch.autograd.Function):
     def forward(ctx, fc0, ff0, velocity, rho_bc, por_c, por_f, sc, sf, cells, steps, K, even0, observe, observe_from,
                 graphs):
         Lc, Lf = sc.L, sf.L
-        for t, L in ((fc0, Lc), (ff0, Lf)):
-            if t.dtype != L.f.dtype or t.numel() != L.f.numel() or not t.is_cuda:
-                raise ValueError("initial buffers must be device buffers of each level's storage type and size")
-        stream = torch.cuda.current_stream()
+        # in place on one buffer per level: the same arithmetic as the out-of-place replay
+        C, F = _storage_input(fc0, Lc, "fc0"), _storage_input(ff0, Lf, "ff0")
         if velocity is not None:
             for i in range(3):
                 Lc.bcs["velocity"].vel[i].copy_(velocity[i].detach().to(torch.float32))
@@ -248,8 +293,7 @@ class _TwoLevelSteps(torch.autograd.Function):
             Lc.set_porosity(por_c.detach())
         if por_f is not None:
             Lf.set_porosity(por_f.detach())
-        # in place on one buffer per level: the same arithmetic as the out-of-place replay
-        C, F = fc0.detach().clone(), ff0.detach().clone()
+        ctx.params = (_bc_snapshot(Lc), _bc_snapshot(Lf))
         run = _StepGraphs(lambda ev, st: _two_level_step_in_place(sc, sf, cells, C, F, ev, st), graphs)
         checkpoints = []
         even = even0
@@ -273,14 +317,31 @@ class _TwoLevelSteps(torch.autograd.Function):
         ctx.observe, ctx.observe_from = observe, observe_from
         ctx.checkpoints = checkpoints
         ctx.flags = (velocity is not None, rho_bc is not None, por_c is not None, por_f is not None)
-        return C, F, J
+        ctx.in_dtypes = (fc0.dtype, ff0.dtype)
+        return C.to(fc0.dtype), F.to(ff0.dtype), J
 
     @staticmethod
     def backward(ctx, gC, gF, gJ):
+        Lc, Lf = ctx.sc.L, ctx.sf.L
+        current = (_bc_snapshot(Lc), _bc_snapshot(Lf))
+        _bc_restore(Lc, ctx.params[0])
+        _bc_restore(Lf, ctx.params[1])
+        try:
+            return _TwoLevelSteps._backward(ctx, gC, gF, gJ)
+        finally:
+            _bc_restore(Lc, current[0])
+            _bc_restore(Lf, current[1])
+
+    @staticmethod
+    def _backward(ctx, gC, gF, gJ):
         sc, sf, cells, steps, K = ctx.sc, ctx.sf, ctx.cells, ctx.steps, ctx.K
         Lc, Lf = sc.L, sf.L
         stream = torch.cuda.current_stream()
-        dev = gC.device
+        dev = Lc.device
+        if gC is None:
+            gC = torch.zeros(Lc.f.shape, dtype=torch.float32, device=dev)
+        if gF is None:
+            gF = torch.zeros(Lf.f.shape, dtype=torch.float32, device=dev)
         lc = gC.detach().to(torch.float32).contiguous().clone()
         lf = gF.detach().to(torch.float32).contiguous().clone()
         has_u, has_r, has_pc, has_pf = ctx.flags
@@ -317,8 +378,8 @@ class _TwoLevelSteps(torch.autograd.Function):
             del states
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
-        return (lc.to(Lc.f.dtype), lf.to(Lf.f.dtype), gv, rbar, pcb, pfb, None, None, None, None, None, None, None,
-                None, None)
+        return (lc.to(ctx.in_dtypes[0]), lf.to(ctx.in_dtypes[1]), gv, rbar, pcb, pfb, None, None, None, None, None,
+                None, None, None, None)
 
 
 def _observe_grad(observe, buf, even, lam, gJ):
@@ -395,6 +456,10 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
     porosity of either level. interface: (cf, fc) InterfaceCells of the pair (NestedBox.cf,
     NestedBox.fc). The fine level takes its steps with the even parity first, as after every
     coarse step; `even` is the coarse parity of the first step (default: the lattice's).
+
+    fc0 / ff0 hold each level's storage type or FP32 (16-bit levels then return their final
+    buffers and gradients in FP32). The backward replays with the BC values and porosities this
+    call used, whatever the lattices hold by then.
 
     observe(buf, even) -> scalar tensor, a differentiable function of a coarse buffer and its
     parity, is summed over the coarse states after steps observe_from .. steps-1 and returned as

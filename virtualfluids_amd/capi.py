@@ -33,6 +33,25 @@ GEO_FLUID = 19
 _vp = ctypes.c_void_p
 
 
+# vf_box_kernel, vf_row_addressing, vf_adjoint_kernel
+BOX_KERNEL_AUTO, BOX_KERNEL_ROW, BOX_KERNEL_NODE = 0, 1, 2
+ROW_ADDRESSING_AUTO, ROW_ADDRESSING_BUFFER, ROW_ADDRESSING_GLOBAL = 0, 1, 2
+ADJOINT_KERNEL_AUTO, ADJOINT_KERNEL_ROW_WHOLE, ADJOINT_KERNEL_NODE = 0, 1, 2
+
+
+class KernelChoice(ctypes.Structure):
+    """vf_kernel_choice: pins a form of the dense-box collision and its adjoint (zero: automatic)."""
+
+    _fields_ = [
+        ("boxKernel", ctypes.c_int32),
+        ("rowWidth", ctypes.c_int32),
+        ("rowWorkgroupsPerCU", ctypes.c_int32),
+        ("rowAddressing", ctypes.c_int32),
+        ("rowXcdRanges", ctypes.c_int32),
+        ("adjointKernel", ctypes.c_int32),
+    ]
+
+
 class LBMParameter(ctypes.Structure):
     """vf_lbm_parameter: one grid level (LBMSimulationParameter, Parameter.h:67-256)."""
 
@@ -65,6 +84,7 @@ class LBMParameter(ctypes.Structure):
         ("numberOfFluidNodes", ctypes.c_uint32),
         ("porosity", _vp),
         ("workingSetBytes", ctypes.c_uint64),
+        ("kernelChoice", KernelChoice),
     ]
 
 

@@ -1,6 +1,7 @@
 // Parameter levels and device memory of a level (host side of the C++ layer).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "virtualfluids_amd.hpp"
@@ -134,28 +135,41 @@ void DeviceLevel::attach(LBMSimulationParameter& p) const
     p.numberOfFluidNodes = (uint32_t)shape.fluidNodeIndices.size();
 }
 
-void DeviceLevel::setBC(vf_q_boundary_conditions& bc, const HostBC& host)
+void DeviceLevel::checkBC(const HostBC& host, uint64_t numberOfNodes)
 {
     const size_t n = host.k.size();
     if (host.q27.size() != 27 * n) throw std::invalid_argument("setBC: q27 must hold 27 entries per BC node");
     if (!host.kN.empty() && host.kN.size() != n) throw std::invalid_argument("setBC: kN must hold one node per BC node");
     for (int32_t k : host.k)
-        if (k < 0 || (uint64_t)k >= shape.numberOfNodes) throw std::invalid_argument("setBC: BC node out of range");
+        if (k < 0 || (uint64_t)k >= numberOfNodes) throw std::invalid_argument("setBC: BC node out of range");
     for (int32_t k : host.kN)
-        if (k < 0 || (uint64_t)k >= shape.numberOfNodes) throw std::invalid_argument("setBC: kN node out of range");
-    auto opt = [&](const std::vector<float>& v, const char* name) -> const float* {
-        if (v.empty()) return nullptr;
-        if (v.size() != n) throw std::invalid_argument(std::string("setBC: ") + name + " must hold one value per BC node");
-        return (const float*)upload(v.data(), n * 4);
+        if (k < 0 || (uint64_t)k >= numberOfNodes) throw std::invalid_argument("setBC: kN node out of range");
+    for (const std::vector<float>* v : {&host.Vx, &host.Vy, &host.Vz, &host.RhoBC})
+        if (!v->empty() && v->size() != n)
+            throw std::invalid_argument("setBC: Vx, Vy, Vz and RhoBC must hold one value per BC node");
+    // the BC kernels write each (node, direction) cell from one thread: a node listed twice would
+    // be written by two threads in the same launch
+    std::vector<int32_t> sorted(host.k);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+        throw std::invalid_argument("setBC: a BC node is listed twice");
+}
+
+void DeviceLevel::setBC(vf_q_boundary_conditions& bc, const HostBC& host)
+{
+    checkBC(host, shape.numberOfNodes);
+    const size_t n = host.k.size();
+    auto opt = [&](const std::vector<float>& v) -> const float* {
+        return v.empty() ? nullptr : (const float*)upload(v.data(), n * 4);
     };
     bc.k = (const int32_t*)upload(host.k.data(), n * 4);
     bc.kN = host.kN.empty() ? nullptr : (const int32_t*)upload(host.kN.data(), n * 4);
     bc.q27 = (const float*)upload(host.q27.data(), 27 * n * 4);
     bc.numberOfBCnodes = (uint32_t)n;
-    bc.Vx = opt(host.Vx, "Vx");
-    bc.Vy = opt(host.Vy, "Vy");
-    bc.Vz = opt(host.Vz, "Vz");
-    bc.RhoBC = opt(host.RhoBC, "RhoBC");
+    bc.Vx = opt(host.Vx);
+    bc.Vy = opt(host.Vy);
+    bc.Vz = opt(host.Vz);
+    bc.RhoBC = opt(host.RhoBC);
     bc.direction = host.direction;
 }
 

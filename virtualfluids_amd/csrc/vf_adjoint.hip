@@ -356,14 +356,6 @@ int check_adjoint_args(const vf_lbm_parameter* p, const float* adjoint)
     return VF_OK;
 }
 
-// VFAMD_ADJ_ROW=0 selects the scalar box kernel, VFAMD_ROW_BUFFER=0 the global-address form of
-// the row kernel and VFAMD_ROW_SPLIT=0 whole rows (used by the tests to cover every form)
-bool env_off(const char* name)
-{
-    const char* e = std::getenv(name);
-    return e && *e == '0';
-}
-
 template <class S, bool EVEN, bool BUF>
 void launch_row_b(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift, hipStream_t s)
 {
@@ -385,16 +377,19 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj
     // would carry fewer than 32 lanes is split: the whole waves take the row kernel and the
     // columns beyond a tail launch of the scalar kernel (258 = 256 + 2). Other widths take the
     // scalar kernel.
+    // kernelChoice.adjointKernel pins the scalar kernel or whole rows without the split.
+    const vf_kernel_choice& kc = p.kernelChoice;
     uint32_t rowEnd = (uint32_t)p.nx;
-    if (p.nx > 64 && p.nx % 64 != 0 && p.nx % 64 < 32 && !env_off("VFAMD_ROW_SPLIT")) rowEnd = p.nx - p.nx % 64;
+    if (p.nx > 64 && p.nx % 64 != 0 && p.nx % 64 < 32 && kc.adjointKernel != VF_ADJOINT_KERNEL_ROW_WHOLE)
+        rowEnd = p.nx - p.nx % 64;
     const int waves = (int)((rowEnd + 63) / 64);
-    if (rowEnd <= (uint32_t)kAdjRowMaxThreads && 8 % waves == 0 && !env_off("VFAMD_ADJ_ROW")) {
+    if (rowEnd <= (uint32_t)kAdjRowMaxThreads && 8 % waves == 0 && kc.adjointKernel != VF_ADJOINT_KERNEL_NODE) {
         // the buffer form needs kRowSplit slots of the adjoint (the larger element) within
         // 32-bit offsets
         const S* f = static_cast<const S*>(p.distributions);
         AdjBoxArgs ar = a;
         ar.rowEnd = rowEnd;
-        if ((uint64_t)kRowSplit * a.N * sizeof(float) <= 0xffffffffull && !env_off("VFAMD_ROW_BUFFER"))
+        if ((uint64_t)kRowSplit * a.N * sizeof(float) <= 0xffffffffull && kc.rowAddressing != VF_ROW_ADDRESSING_GLOBAL)
             launch_row_b<S, EVEN, true>(ar, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);
         else
             launch_row_b<S, EVEN, false>(ar, r, f, lam, p.fluidMask != nullptr, shift != nullptr, s);

@@ -17,7 +17,6 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
-#include <cstdlib>
 
 #include "vf_k17.h"
 #include "vf_rowio.h"
@@ -327,7 +326,8 @@ struct IndexedArgs {
     float* bfx;
     float* bfy;
     float* bfz;
-    float forceScale; // 1/2 * forceFactor, applied to the per-node body force
+    float forces[3];  // global force, unscaled (Parameter::getForcesDev)
+    float forceFactor; // 2^-level
     const float* porosity;
 };
 
@@ -350,10 +350,11 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
     n[7] = a.nbz[n[3]];
     K17Rates rr = r;
     if (BODYFORCE) {
-        // RunCollision.cuh:104-110: (forces + bodyForce[k]) * 1/2 * forceFactor, then reset
-        rr.fx += a.bfx[k] * a.forceScale;
-        rr.fy += a.bfy[k] * a.forceScale;
-        rr.fz += a.bfz[k] * a.forceScale;
+        // RunCollision.cuh:87-94: (forces + bodyForce[k]) * 1/2 * forceFactor in that order,
+        // then the node's body force is reset
+        rr.fx = (a.forces[0] + a.bfx[k]) * 0.5f * a.forceFactor;
+        rr.fy = (a.forces[1] + a.bfy[k]) * 0.5f * a.forceFactor;
+        rr.fz = (a.forces[2] + a.bfz[k]) * 0.5f * a.forceFactor;
         a.bfx[k] = 0.0f;
         a.bfy[k] = 0.0f;
         a.bfz[k] = 0.0f;
@@ -376,26 +377,17 @@ __global__ __launch_bounds__(kIndexedBlock) void k17_indexed_kernel(S* __restric
 // Measured on MI355X at 256^3: FP32 1 node/lane (5.9 TB/s; 2 nodes/lane 5.8, scalar 4.9);
 // FP16 and BF16 4 nodes/lane where rows are whole waves of it (1-2 % above 2 nodes/lane; both
 // near the VALU issue limit), else 2. Rows longer than one workgroup's lanes take the next width
-// that fits; rows no width fits take the scalar kernel.
-// VFAMD_ROW_WIDTH overrides the choice (0, 1, 2 or 4) for benchmarking.
-// VFAMD_ROW_PAIRS=0 keeps one FP32 node per lane (benchmarking)
-static bool row_pairs_allowed()
+// that fits; rows no width fits take the scalar kernel. kernelChoice.rowWidth pins a width.
+template <class S> static int row_width(int nx, const vf_kernel_choice& kc)
 {
-    const char* e = std::getenv("VFAMD_ROW_PAIRS");
-    return !(e && *e == '0');
-}
-
-template <class S> static int row_width(int nx)
-{
-    const char* e = std::getenv("VFAMD_ROW_WIDTH");
-    if (e && *e) return std::atoi(e);
+    if (kc.rowWidth != 0) return kc.rowWidth;
     auto fits = [nx](int v) { return nx % v == 0 && nx / v <= kRowMaxThreads; };
     auto waves = [nx](int v) { return (nx / v + 63) / 64; };
     if (Codec<S>::storage == VF_STORAGE_FP32) {
-        // two nodes per lane where one node per lane leaves a last wave of a few lanes (66
-        // nodes: one wave of 33 lanes instead of a full wave and one of 2 lanes)
+        // two nodes per lane where one node per lane leaves a last wave of a few lanes (a row
+        // of 130 nodes: one full wave and one of 33 lanes instead of two full waves and one of 2)
         const int rem = (nx % 128) % 64;
-        if (fits(1) && fits(2) && rem != 0 && rem < 32 && waves(2) < waves(1) && row_pairs_allowed()) return 2;
+        if (fits(1) && fits(2) && rem != 0 && rem < 32 && waves(2) < waves(1)) return 2;
         return fits(1) ? 1 : fits(2) ? 2 : fits(4) ? 4 : 0;
     }
     if (nx % 256 == 0 && fits(4)) return 4;
@@ -408,13 +400,11 @@ template <class S> static int row_width(int nx)
 // give 5.84-5.86 and 2 give 5.92-5.93 (fewer concurrent direction streams per memory channel);
 // one gives 3.8. A working set that fits the 256 MB infinity cache is latency-bound instead
 // and takes every wave. The 16-bit kernels are bound by VALU issue and need every wave they can
-// get. The cap is imposed by reserving LDS at launch. VFAMD_ROW_WGS overrides it for
-// benchmarking.
+// get. The cap is imposed by reserving LDS at launch; kernelChoice.rowWorkgroupsPerCU pins it.
 constexpr uint64_t kCacheResidentBytes = 256ull << 20;
-template <class S, int V, bool BUF> static int row_wgs_per_cu(uint64_t workingSet)
+template <class S, int V, bool BUF> static int row_wgs_per_cu(uint64_t workingSet, const vf_kernel_choice& kc)
 {
-    const char* e = std::getenv("VFAMD_ROW_WGS");
-    if (e && *e) return std::atoi(e);
+    if (kc.rowWorkgroupsPerCU != 0) return kc.rowWorkgroupsPerCU < 0 ? 0 : kc.rowWorkgroupsPerCU;
     const bool streams = workingSet > kCacheResidentBytes;
     return (Codec<S>::storage == VF_STORAGE_FP32 && V == 1 && BUF && streams) ? 2 : 0;
 }
@@ -441,11 +431,11 @@ static size_t row_lds_reserve(int k)
 // out: null for the in-place step, else the buffer the out-of-place step writes
 template <class S, int V, bool EVEN, bool BUF>
 static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, uint64_t ws,
-                         hipStream_t s)
+                         const vf_kernel_choice& kc, hipStream_t s)
 {
     dim3 grid(a.ny, a.nz);
     dim3 block(((a.nx / V + 63) / 64) * 64); // whole waves; lanes beyond the row idle
-    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>(ws));
+    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>(ws, kc));
     if (out) {
         if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, true>), grid, block, lds, s, f, out, a, r);
         else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF, true>), grid, block, lds, s, f, out, a, r);
@@ -453,7 +443,7 @@ static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
     }
     if (mask && macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, true, BUF>), grid, block, lds, s, f, f, a, r);
     else if (macro) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, true, BUF>), grid, block, lds, s, f, f, a, r);
-    else if (BUF && xcd_rows(a.nx, sizeof(S))) {
+    else if (BUF && xcd_rows(a.nx, sizeof(S), kc.rowXcdRanges)) {
         if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, false, true>), grid, block, lds, s, f, f, a, r);
         else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF, false, true>), grid, block, lds, s, f, f, a, r);
     }
@@ -461,22 +451,15 @@ static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
     else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF>), grid, block, lds, s, f, f, a, r);
 }
 
-// VFAMD_ROW_BUFFER=0 forces the global-address form (used by the tests to cover both forms).
-static bool row_buffer_allowed()
-{
-    const char* e = std::getenv("VFAMD_ROW_BUFFER");
-    return !(e && *e == '0');
-}
-
 template <class S, int V, bool EVEN>
 static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, uint64_t ws,
-                         hipStream_t s)
+                         const vf_kernel_choice& kc, hipStream_t s)
 {
     // the larger descriptor spans kRowSplit slots; buffer offsets are 32-bit
-    if ((uint64_t)kRowSplit * a.N * sizeof(S) <= 0xffffffffull && row_buffer_allowed())
-        launch_row_b<S, V, EVEN, true>(a, r, mask, macro, f, out, ws, s);
+    if ((uint64_t)kRowSplit * a.N * sizeof(S) <= 0xffffffffull && kc.rowAddressing != VF_ROW_ADDRESSING_GLOBAL)
+        launch_row_b<S, V, EVEN, true>(a, r, mask, macro, f, out, ws, kc, s);
     else
-        launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, out, ws, s);
+        launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, out, ws, kc, s);
 }
 
 // The row kernel when the box allows it: a row of nx nodes is nx/V lanes, at most one
@@ -485,26 +468,26 @@ template <class S, bool EVEN>
 static bool try_launch_row(const vf_lbm_parameter& p, const BoxArgs& a, const K17Rates& r, bool macro, S* out,
                            hipStream_t s)
 {
+    const vf_kernel_choice& kc = p.kernelChoice;
+    if (kc.boxKernel == VF_BOX_KERNEL_NODE) return false;
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
-    const int V = row_width<S>(p.nx);
+    const int V = row_width<S>(p.nx, kc);
     if (V <= 0 || p.nx % V != 0 || p.nx / V > kRowMaxThreads) return false;
     // A working set that stays in the 256 MB infinity cache is latency-bound: the scalar kernel
     // (no shuffles or barriers, every lane busy whatever nx, more resident waves) serves it
     // best. Measured on MI355X, one lattice stepped alone: 128^3 FP32 67 against 86 us per step,
     // FP16 36 against 44; 194 x 98 x 98 FP32 68 against 109; 130^3 FP32 85 against 134; the
     // 66^3 cavity 17 against 18. Streaming lattices are the row kernel's (256^3 FP32 620
-    // against 682-736 us). VFAMD_SMALL_SCALAR=0 (or a forced VFAMD_ROW_WIDTH) keeps the row
-    // kernel there.
+    // against 682-736 us). A pinned row kernel or row width keeps the row kernel there.
     const uint64_t ws = p.workingSetBytes ? p.workingSetBytes : 27 * a.N * sizeof(S);
-    const char* keep = std::getenv("VFAMD_SMALL_SCALAR");
-    const bool scalarAllowed = !(keep && *keep == '0') && !std::getenv("VFAMD_ROW_WIDTH");
-    if (scalarAllowed && ws <= kCacheResidentBytes) return false;
+    const bool pinned = kc.boxKernel == VF_BOX_KERNEL_ROW || kc.rowWidth != 0;
+    if (!pinned && ws <= kCacheResidentBytes) return false;
     if constexpr (sizeof(S) == 4) {
-        if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, ws, s); return true; }
+        if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, ws, kc, s); return true; }
     }
-    if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, out, ws, s); return true; }
-    if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, out, ws, s); return true; }
+    if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, out, ws, kc, s); return true; }
+    if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, out, ws, kc, s); return true; }
     return false;
 }
 
@@ -531,7 +514,7 @@ static void launch_indexed_t(const vf_lbm_parameter& p, const uint32_t* idx, uin
 {
     IndexedArgs a{p.numberOfNodes, p.neighborX, p.neighborY, p.neighborZ, idx, n,
                   p.rho, p.velocityX, p.velocityY, p.velocityZ, p.bodyForceX, p.bodyForceY, p.bodyForceZ,
-                  0.5f * p.forceFactor, p.porosity};
+                  {p.forces[0], p.forces[1], p.forces[2]}, p.forceFactor, p.porosity};
     const K17Rates r = make_k17_rates(p);
     dim3 grid((n + kIndexedBlock - 1) / kIndexedBlock);
     S* f = static_cast<S*>(p.distributions);

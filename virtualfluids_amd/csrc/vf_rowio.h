@@ -33,19 +33,18 @@ constexpr int kRowSplit = 14;
 // ...) with q = R / 8), so that neighbouring rows meet in one L2. Rows whose bytes are not a
 // whole number of 128-byte lines share a line at either end with the rows beside them in y and
 // z, which then meet in one L2 instead of two (on MI355X 258-node FP16 rows run 12 % faster
-// with it, 66-node FP32 rows 21 %, while aligned rows run 2-5 % slower).
+// with it, 130-node FP32 rows 13 %, while aligned rows run 2-5 % slower).
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t R)
 {
     const uint32_t q = R >> 3, rem = R & 7, j = b & 7;
     return j * q + (j < rem ? j : rem) + (b >> 3);
 }
 
-// host: whether a launch over rows of nx elements of elemBytes takes the XCD block ranges;
-// VFAMD_ROW_XCD=0 or 1 forces the choice
-inline uint32_t xcd_rows(int nx, size_t elemBytes)
+// host: whether a launch over rows of nx elements of elemBytes takes the XCD block ranges
+// (choice: 0 automatic, 1 always, 2 never; vf_kernel_choice.rowXcdRanges)
+inline uint32_t xcd_rows(int nx, size_t elemBytes, int choice)
 {
-    const char* e = std::getenv("VFAMD_ROW_XCD");
-    if (e && (*e == '0' || *e == '1')) return *e == '1';
+    if (choice == 1 || choice == 2) return choice == 1;
     return ((size_t)nx * elemBytes) % 128 != 0;
 }
 

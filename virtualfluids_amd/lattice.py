@@ -43,7 +43,14 @@ def stream_handle(stream=None):
 class DeviceBC:
     """A boundary-condition node list resident on the device (QforBoundaryConditions)."""
 
-    def __init__(self, nodes, device):
+    def __init__(self, nodes, device, number_of_nodes: int | None = None):
+        k = np.asarray(nodes.k)
+        if number_of_nodes is not None and k.size and (k.min() < 0 or k.max() >= number_of_nodes):
+            raise ValueError("BC node index out of range")
+        # the BC kernels write each (node, direction) cell from one thread: a node listed twice
+        # would be written by two threads in the same launch
+        if np.unique(k).size != k.size:
+            raise ValueError("a BC node is listed twice")
         self.n = nodes.n
         self.k = torch.from_numpy(nodes.k.astype(np.int32)).to(device)
         self.q27 = torch.from_numpy(nodes.q27.astype(np.float32)).to(device)
@@ -94,7 +101,7 @@ class Lattice:
         self.sums = torch.zeros(5, dtype=torch.float64, device=self.device)
         # the dense-box path needs no neighbour loads; default to it for boxes
         self.box = True if box is None else box
-        self.bcs = {name: DeviceBC(nodes, self.device) for name, nodes in grid.bcs.items()}
+        self.bcs = {name: DeviceBC(nodes, self.device, N) for name, nodes in grid.bcs.items()}
 
         p = capi.LBMParameter()
         p.distributions = _ptr(self.f)
@@ -167,8 +174,30 @@ class Lattice:
     def pref(self):
         return ctypes.byref(self.param)
 
-    def collide(self, write_macro: bool = False, stream=None):
+    def set_body_force(self, bfx, bfy, bfz):
+        """Per-node body force read (and reset to zero) by the ApplyBodyForce, AllFeatures and
+        SubDomainBorder collision templates (numpy or torch, N values each; None removes it)."""
+        if bfx is None:
+            self.body_force = None
+            self.param.bodyForceX = self.param.bodyForceY = self.param.bodyForceZ = None
+            return
+        self.body_force = [torch.as_tensor(np.asarray(b, dtype=np.float32) if not torch.is_tensor(b) else b,
+                                           dtype=torch.float32).reshape(-1).to(self.device).clone()
+                           for b in (bfx, bfy, bfz)]
+        if any(b.numel() != self.N for b in self.body_force):
+            raise ValueError("body force needs one value per node")
+        self.param.bodyForceX, self.param.bodyForceY, self.param.bodyForceZ = (_ptr(b) for b in self.body_force)
+
+    def collide(self, write_macro: bool = False, stream=None, template: int | None = None):
+        """One collide + stream step. template: a CollisionTemplate (capi.COLLISION_*); the
+        default is Default, or WriteMacroVars with write_macro. The body-force templates run on
+        the fluid-node list (runOnIndices)."""
         tmpl = capi.COLLISION_WRITE_MACRO_VARS if write_macro else capi.COLLISION_DEFAULT
+        if template is not None:
+            tmpl = int(template)
+        if tmpl in (capi.COLLISION_APPLY_BODY_FORCE, capi.COLLISION_ALL_FEATURES, capi.COLLISION_SUBDOMAIN_BORDER):
+            if self.box:
+                raise ValueError("the body-force templates need the indexed lattice (box=False)")
         if self.box:
             capi.call("vf_k17_run_box", self.pref(), tmpl, stream_handle(stream))
         else:
