@@ -246,7 +246,7 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
                    f"storage, {steps} coarse steps, checkpoint every {checkpoint_every}, outlet kinetic energy "
                    f"averaged over the last {average_last}; gradient w.r.t. {len(a0)} inflow Fourier coefficients",
            "fluid_nodes": nodes, "objective": J0, "grad_l2": float(np.linalg.norm(g)),
-           "grad_first4": [float(v) for v in g[:4]],
+           "grad_first4": [float(v) for v in g[:4]], "_grad": g,
            "forward_s": round(t_forward, 3), "gradient_s": round(t_gradient, 3),
            "gradient_vs_forward": round(t_gradient / t_forward, 3),
            "forward_MLUPS": round((nb.coarse.fluid.size + 2 * nb.fine.N) * steps / t_forward / 1e6, 1)}
@@ -350,32 +350,81 @@ def copy_bandwidth():
     return tuple(out)
 
 
-def cpu_baseline(target_s=12.0):
-    """The FP64 CPU oracle (OpenMP over z-slabs) on a 128^3 Taylor-Green sample."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def inplace27_bandwidth(n=256, reps=20):
+    """The in-place 27-stream read-modify-write yardstick (vf_stream_inplace27) on a 27 * n^3 FP32
+    buffer, the population buffer of the headline workload: 216 B per node / time."""
+    import ctypes
+
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import stream_handle
+
+    N = n ** 3
+    buf = torch.zeros(27 * N, dtype=torch.float32, device="cuda")
+
+    def run():
+        capi.call("vf_stream_inplace27", ctypes.c_void_p(buf.data_ptr()), ctypes.c_uint64(4 * N), stream_handle())
+
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        run()
+    e.record()
+    torch.cuda.synchronize()
+    del buf
+    torch.cuda.empty_cache()
+    return 216.0 * N / (s.elapsed_time(e) / 1000.0 / reps) / 1e9
+
+
+def cpu_baseline(n=256, target_s=8.0):
+    """The CPU oracle (the C++ restatement of the reference's K17 step, OpenMP over z-slabs) on the
+    headline workload, Taylor-Green 256^3 periodic, in its FP64 build (the reference's CPU
+    precision) and its FP32 build, each timed for about `target_s` seconds of steps after one
+    untimed step."""
     import numpy as np
 
     import oracle
     from virtualfluids_amd.grid import periodic_box, taylor_green_fields, taylor_green_omega
 
-    n = 128
     grid = periodic_box(n, n, n)
     omega = taylor_green_omega(n, 0.05, 1600.0)
-    L = oracle.Lattice(grid.nbx, grid.nby, grid.nbz)
-    L.init_eq(*taylor_green_fields(grid, 0.05))
-    even = True
-    for _ in range(2):
+    fields = taylor_green_fields(grid, 0.05)
+    out = {}
+    for dtype in (np.float64, np.float32):
+        L = oracle.Lattice(grid.nbx, grid.nby, grid.nbz, dtype=dtype)
+        L.init_eq(*fields)
+        even = True
         oracle.collide_box(L.f, n, n, n, omega, even, limiters=(1e6, 1e6, 1e6))
         even = not even
-    steps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < target_s and steps < 400:
-        oracle.collide_box(L.f, n, n, n, omega, even, limiters=(1e6, 1e6, 1e6))
-        even = not even
-        steps += 1
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    cores = min(cores, os.cpu_count() or cores)
-    return {"value": round(grid.N * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": f"FP64 CPU oracle, Taylor-Green 128^3 periodic, {steps} K17 steps ({dt:.1f} s)"}
+        steps, t0 = 0, time.perf_counter()
+        while steps < 2 or (time.perf_counter() - t0 < target_s and steps < 400):
+            oracle.collide_box(L.f, n, n, n, omega, even, limiters=(1e6, 1e6, 1e6))
+            even = not even
+            steps += 1
+        dt = time.perf_counter() - t0
+        out[np.dtype(dtype).name] = (grid.N * steps / dt / 1e6, steps, dt)
+        del L
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = min(threads, os.cpu_count() or threads)
+    f64, f32 = out["float64"], out["float32"]
+    return {"value": round(f64[0], 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"CPU oracle (C++ restatement of the reference's K17 step, OpenMP), Taylor-Green {n}^3 periodic: "
+                      f"FP64 build {f64[1]} steps in {f64[2]:.1f} s; FP32 build {f32[1]} steps in {f32[2]:.1f} s",
+            "fp32_build_MLUPS": round(f32[0], 3), "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
 
 
 def traffic_from_profile():
@@ -387,13 +436,61 @@ def traffic_from_profile():
     return float(m.group(1)) if m else None
 
 
+def velocity_field(L):
+    """(3, N) FP32 velocity of the current state on the device (macroscopic output kernel)."""
+    import torch
+
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.lattice import stream_handle
+
+    capi.call("vf_CalculateMacroscopicQuantitiesCompressible", L.pref(), stream_handle())
+    return torch.stack([L.vx, L.vy, L.vz]).clone()
+
+
+def rel_l2(a, b) -> float:
+    return float(((a.double() - b.double()) ** 2).sum().sqrt() / (b.double() ** 2).sum().sqrt())
+
+
+def storage_precision(n=64, steps=200):
+    """Velocity error of each storage type against the FP64 oracle on the same lattice: Taylor-Green
+    n^3, Re 1600, u0 0.05, limiter off, `steps` steps; relative L2 of the velocity field."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from virtualfluids_amd.grid import periodic_box, taylor_green_fields, taylor_green_omega
+    from virtualfluids_amd.lattice import Lattice
+
+    grid = periodic_box(n, n, n)
+    omega = taylor_green_omega(n, 0.05, 1600.0)
+    fields = taylor_green_fields(grid, 0.05)
+    ref = oracle.Lattice(grid.nbx, grid.nby, grid.nbz)
+    ref.init_eq(*fields)
+    for _ in range(steps):
+        oracle.collide_box(ref.f, n, n, n, omega, ref.even, limiters=(1e6, 1e6, 1e6))
+        ref.swap()
+    _, vx, vy, vz, _ = ref.macroscopic()
+    u64 = torch.from_numpy(np.stack([vx, vy, vz]))
+    out = {"case": f"Taylor-Green {n}^3, Re 1600, u0 0.05, {steps} steps, velocity relative L2 against the FP64 oracle"}
+    for storage in ("fp32", "fp16", "bf16"):
+        L = Lattice(grid, omega, storage=storage, limiters=(1e6, 1e6, 1e6))
+        L.init_equilibrium(*fields)
+        for _ in range(steps):
+            L.collide()
+            L.swap()
+        out[storage] = rel_l2(velocity_field(L).cpu(), u64)
+        del L
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--variant-steps", type=int, default=500, help="timed steps of the FP16/BF16 variants")
+    ap.add_argument("--steady-steps", type=int, default=500,
+                    help="a longer steady window per storage type, reported beside the main one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint timing")
     ap.add_argument("--adjoint-steps", type=int, default=200, help="timed adjoint kernel steps")
@@ -421,35 +518,70 @@ def main():
     fields = taylor_green_fields(grid, u0)
     lim = (1e6, 1e6, 1e6)  # the Taylor-Green app switches the limiter off
 
+    # the copy and in-place yardsticks first: they also bring the GPU to its clocks before the
+    # timed steps
+    copy_gbps, blit_gbps = copy_bandwidth()
+    rmw_gbps = inplace27_bandwidth()
+
     results = {}
+    velocity = {}
     adjoint = None
     for storage in (["fp32", "fp16", "bf16"] if args.only is None else [args.only]):
         L = Lattice(grid, omega, storage=storage, limiters=lim)
         L.init_equilibrium(*fields)
-        steps = args.steps if storage == "fp32" else args.variant_steps
-        t = time_steps(L, steps, args.warmup)
+        # the same window for every storage type: --warmup untimed steps, then --steps timed
+        t = time_steps(L, args.steps, args.warmup)
+        velocity[storage] = velocity_field(L)
         bytes_per_node = 2 * 27 * L.f.element_size()
-        results[storage] = {"t": t, "steps": steps, "mlups": N * steps / t / 1e6,
-                            "ms": 1e3 * t / steps, "gbps": bytes_per_node * N * steps / t / 1e9,
-                            "bytes_per_node": bytes_per_node}
-        s = L.moment_sums()
-        results[storage]["mass_drift"] = float(s[0])
+        r = {"t": t, "steps": args.steps, "mlups": N * args.steps / t / 1e6, "ms": 1e3 * t / args.steps,
+             "gbps": bytes_per_node * N * args.steps / t / 1e9, "bytes_per_node": bytes_per_node}
+        if args.steady_steps > 0:
+            ts = time_steps(L, args.steady_steps, 0)
+            r["steady"] = {"steps": args.steady_steps, "MLUPS": round(N * args.steady_steps / ts / 1e6, 1),
+                           "ms_per_step": round(1e3 * ts / args.steady_steps, 4)}
+        r["mass_drift"] = float(L.moment_sums()[0])
+        results[storage] = r
         if storage == "fp32" and not args.no_adjoint:
             ta = time_adjoint(L, args.adjoint_steps, 5)
             tg = time_gradient(L, 40, 10)
+            t_fwd = t / args.steps
             adjoint = {"kernel_ms_per_step": round(1e3 * ta / args.adjoint_steps, 4),
-                       "kernel_vs_forward": round((ta / args.adjoint_steps) / (t / steps), 3),
+                       "kernel_vs_forward": round((ta / args.adjoint_steps) / t_fwd, 3),
                        "kernel_GBps": round(3 * 27 * 4 * N * args.adjoint_steps / ta / 1e9, 1),
                        "gradient_ms_per_step": round(1e3 * tg / 40, 4),
-                       "gradient_vs_forward": round((tg / 40) / (t / steps), 3),
+                       "gradient_vs_forward": round((tg / 40) / t_fwd, 3),
                        "gradient_run": "k17_steps 40 steps, checkpoint every 10: forward, replay, adjoint"}
         del L
         torch.cuda.empty_cache()
 
     main_key = "fp32" if "fp32" in results else args.only
     r = results[main_key]
-    copy_gbps, blit_gbps = copy_bandwidth()
     traffic = traffic_from_profile() if main_key == "fp32" else None
+    variants = {}
+    for k, v in results.items():
+        variants[k] = {"MLUPS": round(v["mlups"], 1), "GBps": round(v["gbps"], 1), "ms_per_step": round(v["ms"], 4),
+                       "steps": v["steps"], "warmup": args.warmup,
+                       "vs_fp32": round(v["mlups"] / results["fp32"]["mlups"], 3) if "fp32" in results else None}
+        if "steady" in v:
+            variants[k]["steady"] = v["steady"]
+            if "fp32" in results and "steady" in results["fp32"]:
+                variants[k]["steady"]["vs_fp32"] = round(v["steady"]["MLUPS"] / results["fp32"]["steady"]["MLUPS"], 3)
+        if "fp32" in velocity and k != "fp32":
+            # the storage type's velocity error after the same steps from the same initial field
+            variants[k]["u_rel_l2_vs_fp32_storage"] = rel_l2(velocity[k], velocity["fp32"])
+    del velocity
+    precision = storage_precision() if args.only is None else None
+    two_level = None
+    if not (args.no_two_level or args.no_adjoint):
+        two_level = two_level_gradient()
+        ref32 = two_level_gradient(storage="fp32")
+        g16, g32 = two_level.pop("_grad"), ref32.pop("_grad")
+        import numpy as np
+
+        two_level["fp32_storage"] = {k: ref32[k] for k in ("objective", "grad_l2", "forward_s", "gradient_s",
+                                                           "gradient_vs_forward", "forward_MLUPS")}
+        two_level["fp16_vs_fp32_gradient_rel_l2"] = float(np.linalg.norm(g16 - g32) / np.linalg.norm(g32))
+        two_level["fp16_vs_fp32_objective_rel"] = abs(two_level["objective"] / ref32["objective"] - 1.0)
     line = {
         "metric": METRIC,
         "value": round(r["mlups"], 1),
@@ -468,17 +600,15 @@ def main():
                    "omega": round(omega, 6), "storage": main_key},
         "roofline": {"bound": "hbm", "achieved": round(r["gbps"], 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(r["gbps"] / HBM_PEAK_GBPS, 4), "traffic": traffic},
-        "storage_variants": {k: {"MLUPS": round(v["mlups"], 1), "GBps": round(v["gbps"], 1),
-                                 "ms_per_step": round(v["ms"], 4), "steps": v["steps"],
-                                 "vs_fp32": (round(v["mlups"] / results["fp32"]["mlups"], 3)
-                                             if "fp32" in results else None)}
-                             for k, v in results.items()},
+        "storage_variants": variants,
+        "storage_precision_vs_fp64": precision,
         "adjoint": adjoint,
         "sphere_channel_gradient": None if (args.no_sphere or args.no_adjoint) else sphere_channel_gradient(),
-        "two_level_gradient": None if (args.no_two_level or args.no_adjoint) else two_level_gradient(),
+        "two_level_gradient": two_level,
         "published_cavity": None if args.no_cavity else published_cavity(),
         "hbm_copy_GBps": round(copy_gbps, 1),
         "blit_copy_GBps": round(blit_gbps, 1),
+        "inplace27_GBps": round(rmw_gbps, 1),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

@@ -182,6 +182,14 @@ const char* vf_get_last_error(void);
  * bandwidth the collision kernels are measured against. */
 int vf_stream_copy(const void* src, void* dst, uint64_t bytes, void* stream);
 
+/* In-place read-modify-write of 27 streams of streamBytes each (stream d at buf + d*streamBytes;
+ * streamBytes a multiple of 1024 and below 2^34): every element is read and written into the
+ * opposite stream at the same offset (d <-> inv(d) of the D3Q27 order, stream 0 in place), the
+ * memory pattern of the in-place collision step without its arithmetic (27 reads and 27 writes
+ * per node). No counterpart in the reference: the yardstick the collision kernels' bandwidth is
+ * measured against. */
+int vf_stream_inplace27(void* buf, uint64_t streamBytes, void* stream);
+
 /* ---------------------------------------------------------------------------------
  * K17 collide + eso-twist stream
  * --------------------------------------------------------------------------------- */
@@ -205,6 +213,32 @@ int vf_k17_run_box(const vf_lbm_parameter* para, int32_t collisionTemplate, void
  * cells are copied unchanged, so `out` equals what the in-place step leaves. No counterpart in
  * the reference: it lets the adjoint's replay keep every intermediate state without copies. */
 int vf_k17_run_box_into(const vf_lbm_parameter* para, void* out, void* stream);
+
+/* Post-collision bounce-back lists that the dense-box collision applies in the same launch
+ * (vf_k17_run_box_bounce_back; no counterpart in the reference). lists[l] are BC lists in the
+ * order their separate launches would run (NULL where unused); velocity[l] selects
+ * VelocityBounceBack (1: the list needs Vx, Vy, Vz) or NoSlipBounceBack (0). nodeEntry (device,
+ * numberOfNodes values) holds -1 or the node's entry; per entry, entryIndex (device, 3 values)
+ * holds the node's index in list l or -1, and entryLinks (device, 3 values) the bitmask of the
+ * links d with 0 <= q_d <= 1 the node has in list l. Every such link must lead to a node the
+ * collision does not update (fluidMask 0, as at box walls): the epilogue then writes exactly what
+ * the separate kernels would. */
+typedef struct vf_post_bounce_back {
+    const vf_q_boundary_conditions* lists[3]; /* host pointers */
+    int32_t velocity[3];
+    const int32_t* nodeEntry;
+    const int32_t* entryIndex;
+    const uint32_t* entryLinks;
+    uint32_t numberOfEntries;
+} vf_post_bounce_back;
+
+/* vf_k17_run_box followed by the bounce-back lists of bb in their order (VelocityBounceBack or
+ * NoSlipBounceBack, Velocity.cu:44-63 / NoSlip.cu:44-60), with identical results bit for bit.
+ * Where the one-node-per-thread kernel serves the box it writes the bounce-back values in its own
+ * epilogue, from the post-collision values it holds, so the BC launches and their reads of the
+ * post-collision state disappear; where the row kernel serves it, the BC kernels run after it. */
+int vf_k17_run_box_bounce_back(const vf_lbm_parameter* para, int32_t collisionTemplate, const vf_post_bounce_back* bb,
+                               void* stream);
 
 /* ---------------------------------------------------------------------------------
  * Boundary conditions: the free launchers BoundaryConditionFactory hands out

@@ -103,3 +103,11 @@ def test_stream_copy_validates_alignment():
     assert lib.vf_stream_copy(ctypes.c_void_p(0x18), ctypes.c_void_p(0x20), 32, None) == capi.VF_ERR_INVALID_ARGUMENT
     assert lib.vf_stream_copy(None, ctypes.c_void_p(0x20), 32, None) == capi.VF_ERR_INVALID_ARGUMENT
     assert lib.vf_stream_copy(None, None, 0, None) == capi.VF_OK  # nothing to copy
+
+
+def test_stream_inplace27_validates_sizes():
+    lib = capi.load()
+    assert lib.vf_stream_inplace27(None, 1024, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1000, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1 << 34, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(None, 0, None) == capi.VF_OK  # nothing to do

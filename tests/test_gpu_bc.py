@@ -110,3 +110,39 @@ def test_lid_driven_cavity_64_well_posed_rates(gpu, orc):
     e = _errors(grid, L.macroscopic(), ref64, 0.05)
     print("cavity64 omega=1.8", e)
     assert e["u_L2"] < 2e-5 and e["u_max"] < 2e-4 and e["rho_max"] < 2e-6
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("row_kernel", [False, True])
+def test_fused_bounce_back_matches_separate_launches(gpu, storage, graphs, row_kernel):
+    """Simulation on a walled box with VelocityBounceBack and NoSlipBounceBack: the collision that
+    applies the bounce-back lists in its own launch (vf_k17_run_box_bounce_back) leaves every
+    population bit-identical to the collision followed by the two BC launches; pinned to the row
+    kernel, the entry point takes the separate launches itself."""
+    from virtualfluids_amd import capi
+    from virtualfluids_amd.grid import walled_box, d0M0, d0P0, d00M, d00P, dM00, dP00
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory as F, Simulation
+
+    walls = {s: ("noslip",) for s in (dP00, dM00, d0P0, d0M0, d00M)}
+    walls[d00P] = ("velocity", (0.035, 0.02, 0.0))
+    grid = walled_box((30, 20, 12), walls)
+    rng = np.random.default_rng(17)
+    N = grid.N
+    init = (rng.normal(0, 0.01, N), rng.normal(0, 0.02, N), rng.normal(0, 0.02, N), rng.normal(0, 0.02, N))
+    f = F()
+    f.set_velocity_boundary_condition(F.VelocityBC.VelocityBounceBack)
+    f.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipBounceBack)
+    out = []
+    for fuse in (True, False):
+        L = Lattice(grid, 1.7, storage=storage)
+        if row_kernel:
+            L.param.kernelChoice.boxKernel = capi.BOX_KERNEL_ROW
+        L.init_equilibrium(*init)
+        sim = Simulation(L, f, fuse_bounce_back=fuse)
+        assert (sim.fused is not None) == fuse
+        print(f"{storage} graphs={graphs} row_kernel={row_kernel} fused={fuse}", flush=True)
+        sim.run(37, graphs=graphs)
+        out.append(L.populations())
+    assert np.array_equal(out[0], out[1])

@@ -104,6 +104,19 @@ class QBoundaryConditions(ctypes.Structure):
     ]
 
 
+class PostBounceBack(ctypes.Structure):
+    """vf_post_bounce_back: bounce-back lists the dense-box collision applies in its own launch."""
+
+    _fields_ = [
+        ("lists", ctypes.POINTER(QBoundaryConditions) * 3),
+        ("velocity", ctypes.c_int32 * 3),
+        ("nodeEntry", _vp),
+        ("entryIndex", _vp),
+        ("entryLinks", _vp),
+        ("numberOfEntries", ctypes.c_uint32),
+    ]
+
+
 class InterpolationCells(ctypes.Structure):
     _fields_ = [("fineCellIndices", _vp), ("coarseCellIndices", _vp), ("numberOfCells", ctypes.c_uint32)]
 
@@ -119,10 +132,12 @@ _Q = ctypes.POINTER(QBoundaryConditions)
 SIGNATURES = {
     "vf_abi_version": [],
     "vf_stream_copy": [_vp, _vp, ctypes.c_uint64, _vp],
+    "vf_stream_inplace27": [_vp, ctypes.c_uint64, _vp],
     "vf_k17_run": [_P, _vp],
     "vf_k17_run_on_indices": [_P, _vp, ctypes.c_uint32, ctypes.c_int32, _vp],
     "vf_k17_run_box": [_P, ctypes.c_int32, _vp],
     "vf_k17_run_box_into": [_P, _vp, _vp],
+    "vf_k17_run_box_bounce_back": [_P, ctypes.c_int32, ctypes.POINTER(PostBounceBack), _vp],
     "vf_NoSlipBounceBack": [_P, _Q, _vp],
     "vf_NoSlipInterpolatedCompressible": [_P, _Q, _vp],
     "vf_VelocityBounceBack": [_P, _Q, _vp],

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void bc_post_kernel(S* __restrict__ f, Addr a,
         if (KIND == kNoSlipBB) {
             v = post[d];
         } else if (KIND == kVelocityBB) {
-            v = post[d] - 6.0f * w * c_dot(d, Ux, Uy, Uz);
+            v = velocity_bounce_back(post[d], d, Ux, Uy, Uz);
         } else {
             const float cu = c_dot(d, ux, uy, uz);
             const float feq = w * (drho + 4.5f * cu * cu * (1.0f + drho) - cu_sq); // getEquilibriumForBC

@@ -19,6 +19,7 @@
 #include <cstdio>
 
 #include "vf_k17.h"
+#include "vf_lattice.h"
 #include "vf_rowio.h"
 
 namespace vfa {
@@ -63,8 +64,55 @@ __device__ __forceinline__ float brinkman_scale(const float* porosity, uint32_t 
 
 constexpr int kBoxBlock = 256;
 
-template <class S, bool EVEN, bool MASK, bool MACRO>
-__global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, BoxArgs a, K17Rates r)
+// Bounce-back lists applied by the collision itself (vf_post_bounce_back): for a node with an
+// entry, list l bounces back the links of entryLinks[3 e + l] (velocity lists subtract
+// 6 w_d c_d.U of the node's wall velocity).
+struct FusedBB {
+    const int32_t* nodeEntry;
+    const int32_t* entryIndex;
+    const uint32_t* entryLinks;
+    const float* vx[3];
+    const float* vy[3];
+    const float* vz[3];
+};
+
+// The post-collision bounce-back of one node, from its post-collision values v (as the
+// collision left them in registers): exactly what NoSlipBounceBack / VelocityBounceBack compute
+// from the stored values, list by list, writing the incoming population inv(d) of every wall
+// link d into the cell the next step reads (slot d with this step's parity, at the node of
+// inv(d)'s cell).
+template <class S, bool EVEN>
+__device__ __forceinline__ void bounce_back_epilogue(S* __restrict__ f, uint64_t N, const uint32_t (&n)[8],
+                                                     const float (&v)[27], const FusedBB& bb)
+{
+    const int32_t e = bb.nodeEntry[n[0]];
+    if (e < 0) return;
+    float post[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) post[d] = Codec<S>::round(v[d]); // the stored post-collision values
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        const uint32_t links = bb.entryLinks[3 * e + l];
+        if (!links) continue;
+        float Ux = 0.0f, Uy = 0.0f, Uz = 0.0f;
+        const bool velocity = bb.vx[l] != nullptr;
+        if (velocity) {
+            const int32_t i = bb.entryIndex[3 * e + l];
+            Ux = bb.vx[l][i];
+            Uy = bb.vy[l][i];
+            Uz = bb.vz[l][i];
+        }
+#pragma unroll
+        for (int d = 1; d < 27; ++d) {
+            if (!(links >> d & 1u)) continue;
+            const float val = velocity ? velocity_bounce_back(post[d], d, Ux, Uy, Uz) : post[d];
+            Codec<S>::store(f + (uint64_t)slot<EVEN>(d) * N + n[node_sel(inv(d))], val);
+        }
+    }
+}
+
+template <class S, bool EVEN, bool MASK, bool MACRO, bool BB = false>
+__global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, BoxArgs a, K17Rates r, FusedBB bb)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
@@ -91,6 +139,7 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, B
         a.vz[n[0]] = mo.vz;
     }
     scatter<S, EVEN>(f, a.N, n, v);
+    if (BB) bounce_back_epilogue<S, EVEN>(f, a.N, n, v, bb);
 }
 
 // Out-of-place variant: reads `in`, writes the stepped state to `out` (same layout). Masked
@@ -462,33 +511,44 @@ static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
         launch_row_b<S, V, EVEN, false>(a, r, mask, macro, f, out, ws, kc, s);
 }
 
-// The row kernel when the box allows it: a row of nx nodes is nx/V lanes, at most one
-// workgroup. Returns false when it does not apply.
+// Nodes per lane of the row kernel the dense-box collision takes for this box, or 0 when it
+// takes the one-node-per-thread kernel: a row of nx nodes is nx/V lanes, at most one workgroup.
+// A working set that stays in the 256 MB infinity cache is latency-bound: the node kernel (no
+// shuffles or barriers, every lane busy whatever nx, more resident waves) serves it best.
+// Measured on MI355X, one lattice stepped alone: 128^3 FP32 67 against 86 us per step, FP16 36
+// against 44; 194 x 98 x 98 FP32 68 against 109; 130^3 FP32 85 against 134; the 66^3 cavity 17
+// against 18. Streaming lattices are the row kernel's (256^3 FP32 620 against 682-736 us). A
+// pinned row kernel or row width keeps the row kernel there.
+template <class S> static int box_row_width(const vf_lbm_parameter& p)
+{
+    const vf_kernel_choice& kc = p.kernelChoice;
+    if (kc.boxKernel == VF_BOX_KERNEL_NODE) return 0;
+    const int V = row_width<S>(p.nx, kc);
+    if (V <= 0 || p.nx % V != 0 || p.nx / V > kRowMaxThreads) return 0;
+    if (V == 1 && sizeof(S) != 4) return 0;
+    if (V != 1 && V != 2 && V != 4) return 0;
+    const uint64_t ws = p.workingSetBytes ? p.workingSetBytes : 27 * p.numberOfNodes * sizeof(S);
+    const bool pinned = kc.boxKernel == VF_BOX_KERNEL_ROW || kc.rowWidth != 0;
+    return (!pinned && ws <= kCacheResidentBytes) ? 0 : V;
+}
+
+// The row kernel when box_row_width selects it. Returns false when it does not apply.
 template <class S, bool EVEN>
 static bool try_launch_row(const vf_lbm_parameter& p, const BoxArgs& a, const K17Rates& r, bool macro, S* out,
                            hipStream_t s)
 {
+    const int V = box_row_width<S>(p);
+    if (V == 0) return false;
     const vf_kernel_choice& kc = p.kernelChoice;
-    if (kc.boxKernel == VF_BOX_KERNEL_NODE) return false;
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
-    const int V = row_width<S>(p.nx, kc);
-    if (V <= 0 || p.nx % V != 0 || p.nx / V > kRowMaxThreads) return false;
-    // A working set that stays in the 256 MB infinity cache is latency-bound: the scalar kernel
-    // (no shuffles or barriers, every lane busy whatever nx, more resident waves) serves it
-    // best. Measured on MI355X, one lattice stepped alone: 128^3 FP32 67 against 86 us per step,
-    // FP16 36 against 44; 194 x 98 x 98 FP32 68 against 109; 130^3 FP32 85 against 134; the
-    // 66^3 cavity 17 against 18. Streaming lattices are the row kernel's (256^3 FP32 620
-    // against 682-736 us). A pinned row kernel or row width keeps the row kernel there.
     const uint64_t ws = p.workingSetBytes ? p.workingSetBytes : 27 * a.N * sizeof(S);
-    const bool pinned = kc.boxKernel == VF_BOX_KERNEL_ROW || kc.rowWidth != 0;
-    if (!pinned && ws <= kCacheResidentBytes) return false;
     if constexpr (sizeof(S) == 4) {
         if (V == 1) { launch_row_t<S, 1, EVEN>(a, r, mask, macro, f, out, ws, kc, s); return true; }
     }
     if (V == 2) { launch_row_t<S, 2, EVEN>(a, r, mask, macro, f, out, ws, kc, s); return true; }
-    if (V == 4) { launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, out, ws, kc, s); return true; }
-    return false;
+    launch_row_t<S, 4, EVEN>(a, r, mask, macro, f, out, ws, kc, s);
+    return true;
 }
 
 template <class S, bool EVEN>
@@ -502,10 +562,31 @@ static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
     if (try_launch_row<S, EVEN>(p, a, r, macro, nullptr, s)) return;
     const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
     dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
-    if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r);
-    else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r);
-    else if (macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, true>), grid, dim3(bx), 0, s, f, a, r);
-    else hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, false>), grid, dim3(bx), 0, s, f, a, r);
+    const FusedBB none{};
+    if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r, none);
+    else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r, none);
+    else if (macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, true>), grid, dim3(bx), 0, s, f, a, r, none);
+    else hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, false>), grid, dim3(bx), 0, s, f, a, r, none);
+}
+
+// The node kernel with the bounce-back epilogue. Returns false where the dense-box collision
+// takes the row kernel (the caller then launches the collision and the bounce-back kernels).
+template <class S, bool EVEN>
+static bool launch_box_bb_t(const vf_lbm_parameter& p, bool macro, const FusedBB& bb, hipStream_t s)
+{
+    if (box_row_width<S>(p) != 0) return false;
+    BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, p.rho, p.velocityX, p.velocityY, p.velocityZ,
+              p.porosity};
+    const K17Rates r = make_k17_rates(p);
+    S* f = static_cast<S*>(p.distributions);
+    const bool mask = p.fluidMask != nullptr;
+    const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
+    dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
+    if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true, true>), grid, dim3(bx), 0, s, f, a, r, bb);
+    else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false, true>), grid, dim3(bx), 0, s, f, a, r, bb);
+    else if (macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, true, true>), grid, dim3(bx), 0, s, f, a, r, bb);
+    else hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, false, true>), grid, dim3(bx), 0, s, f, a, r, bb);
+    return true;
 }
 
 template <class S, bool EVEN>
@@ -652,7 +733,8 @@ extern "C" int vf_k17_run_box_into(const vf_lbm_parameter* p, void* out, void* s
     return check_launch("K17CompressibleNavierStokes (box, out of place)");
 }
 
-extern "C" int vf_k17_run_box(const vf_lbm_parameter* p, int32_t collisionTemplate, void* stream)
+// argument checks of the dense-box collision; macro: the template writes the macroscopic values
+static int check_box(const vf_lbm_parameter* p, int32_t collisionTemplate, bool& macro)
 {
     int rc = check_common(p);
     if (rc) return rc;
@@ -661,7 +743,7 @@ extern "C" int vf_k17_run_box(const vf_lbm_parameter* p, int32_t collisionTempla
         set_last_error("box path needs nx*ny*nz == numberOfNodes (ny, nz <= 65535)");
         return VF_ERR_INVALID_ARGUMENT;
     }
-    bool macro = false;
+    macro = false;
     if (collisionTemplate == VF_COLLISION_WRITE_MACRO_VARS) {
         macro = true;
         if (!p->rho || !p->velocityX || !p->velocityY || !p->velocityZ) {
@@ -672,7 +754,69 @@ extern "C" int vf_k17_run_box(const vf_lbm_parameter* p, int32_t collisionTempla
         set_last_error("box path supports the Default and WriteMacroVars templates");
         return VF_ERR_UNSUPPORTED;
     }
+    return VF_OK;
+}
+
+extern "C" int vf_k17_run_box(const vf_lbm_parameter* p, int32_t collisionTemplate, void* stream)
+{
+    bool macro = false;
+    int rc = check_box(p, collisionTemplate, macro);
+    if (rc) return rc;
     rc = dispatch_storage<BoxL>(*p, macro, (hipStream_t)stream);
     if (rc) return rc;
     return check_launch("K17CompressibleNavierStokes (box)");
+}
+
+template <class S, bool EVEN> struct BoxBBL {
+    static void run(const vf_lbm_parameter& p, bool macro, const FusedBB& bb, bool& fused, hipStream_t s)
+    {
+        fused = launch_box_bb_t<S, EVEN>(p, macro, bb, s);
+    }
+};
+
+extern "C" int vf_k17_run_box_bounce_back(const vf_lbm_parameter* p, int32_t collisionTemplate,
+                                          const vf_post_bounce_back* bb, void* stream)
+{
+    bool macro = false;
+    int rc = check_box(p, collisionTemplate, macro);
+    if (rc) return rc;
+    if (!bb) {
+        set_last_error("run_box_bounce_back: null bounce-back lists");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (bb->numberOfEntries > 0 && (!bb->nodeEntry || !bb->entryIndex || !bb->entryLinks)) {
+        set_last_error("run_box_bounce_back: nodeEntry, entryIndex and entryLinks are required");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    FusedBB fb{bb->nodeEntry, bb->entryIndex, bb->entryLinks, {}, {}, {}};
+    for (int l = 0; l < 3; ++l) {
+        const vf_q_boundary_conditions* q = bb->lists[l];
+        if (!q || q->numberOfBCnodes == 0) continue;
+        if (!q->k || !q->q27) {
+            set_last_error("run_box_bounce_back: a list needs k and q27");
+            return VF_ERR_INVALID_ARGUMENT;
+        }
+        if (!bb->velocity[l]) continue;
+        if (!q->Vx || !q->Vy || !q->Vz) {
+            set_last_error("run_box_bounce_back: a velocity list needs Vx, Vy, Vz");
+            return VF_ERR_INVALID_ARGUMENT;
+        }
+        fb.vx[l] = q->Vx;
+        fb.vy[l] = q->Vy;
+        fb.vz[l] = q->Vz;
+    }
+    if (bb->numberOfEntries > 0) {
+        bool fused = false;
+        rc = dispatch_storage<BoxBBL>(*p, macro, fb, fused, (hipStream_t)stream);
+        if (rc) return rc;
+        if (fused) return check_launch("K17CompressibleNavierStokes (box, bounce-back epilogue)");
+    }
+    // the row kernel serves this box (or no node has a wall link): the collision, then each list
+    rc = vf_k17_run_box(p, collisionTemplate, stream);
+    for (int l = 0; l < 3 && rc == VF_OK; ++l) {
+        const vf_q_boundary_conditions* q = bb->lists[l];
+        if (!q || q->numberOfBCnodes == 0) continue;
+        rc = bb->velocity[l] ? vf_VelocityBounceBack(p, q, stream) : vf_NoSlipBounceBack(p, q, stream);
+    }
+    return rc;
 }

@@ -71,16 +71,19 @@ __host__ __device__ constexpr float weight(int d)
 // Populations are stored shifted (f_i - w_i) in FP32, FP16 or BF16; all arithmetic is FP32.
 template <class S> struct Codec;
 
+// round(v): the value a store followed by a load gives back
 template <> struct Codec<float> {
     static constexpr int storage = VF_STORAGE_FP32;
     __device__ static float load(const float* p) { return *p; }
     __device__ static void store(float* p, float v) { *p = v; }
+    __device__ static float round(float v) { return v; }
 };
 
 template <> struct Codec<__half> {
     static constexpr int storage = VF_STORAGE_FP16;
     __device__ static float load(const __half* p) { return __half2float(*p); }
     __device__ static void store(__half* p, float v) { *p = __float2half_rn(v); }
+    __device__ static float round(float v) { return __half2float(__float2half_rn(v)); }
 };
 
 // bfloat16 is the upper half of an FP32 word; stores round to nearest even.
@@ -93,6 +96,10 @@ template <> struct Codec<__hip_bfloat16> {
     __device__ static void store(__hip_bfloat16* p, float v)
     {
         *reinterpret_cast<unsigned short*>(p) = __bfloat16_as_ushort(__float2bfloat16(v));
+    }
+    __device__ static float round(float v)
+    {
+        return __uint_as_float(((unsigned)__bfloat16_as_ushort(__float2bfloat16(v))) << 16);
     }
 };
 

@@ -73,6 +73,14 @@ __device__ __forceinline__ float c_dot(int d, float ux, float uy, float uz)
     return s;
 }
 
+// VelocityBounceBack: the incoming population inv(d) from the post-collision value of the wall
+// link d, f_inv(d) = f_d - 6 w_d c_d.U (getBounceBackDistributionForVeloBC,
+// src/gpu/core/Utilities/KernelUtilities.h:186-190); NoSlipBounceBack returns f_d itself
+__device__ __forceinline__ float velocity_bounce_back(float post_d, int d, float Ux, float Uy, float Uz)
+{
+    return post_d - 6.0f * weight(d) * c_dot(d, Ux, Uy, Uz);
+}
+
 // Full shifted equilibrium w_d (drho + (1+drho)(3 c.u + 9/2 (c.u)^2 - 3/2 u^2))
 // (InitNavierStokesCompressible_Device.cu:173-201).
 __device__ __forceinline__ float feq_shifted(int d, float drho, float ux, float uy, float uz, float cu_sq)

@@ -1,5 +1,5 @@
-// Library-wide bookkeeping (ABI version, last error message) and the copy kernel that
-// yardsticks HBM bandwidth.
+// Library-wide bookkeeping (ABI version, last error message) and the copy and in-place kernels
+// that yardstick HBM bandwidth.
 #include <cstdio>
 #include <cstring>
 
@@ -35,6 +35,20 @@ __global__ __launch_bounds__(256) void stream_copy_kernel(const copy_u4* __restr
         __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// In-place read-modify-write of 27 streams: every lane reads one element of each stream and
+// writes it back into the opposite stream at the same offset (d <-> inv(d), stream 0 in place),
+// the memory pattern of the in-place collision without its arithmetic. A workgroup covers 256
+// consecutive elements of every stream, as the FP32 row kernel covers one 256-node row.
+__global__ __launch_bounds__(256) void stream_inplace27_kernel(float* f, uint64_t n)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    float v[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) v[d] = __builtin_nontemporal_load(f + d * n + k);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) __builtin_nontemporal_store(v[inv(d)], f + d * n + k);
+}
+
 } // namespace vfa
 
 extern "C" int vf_abi_version(void) { return VF_ABI_VERSION; }
@@ -54,4 +68,22 @@ extern "C" int vf_stream_copy(const void* src, void* dst, uint64_t bytes, void* 
     hipLaunchKernelGGL(vfa::stream_copy_kernel, dim3(grid), dim3(256), 0, (hipStream_t)stream,
                        static_cast<const vfa::copy_u4*>(src), static_cast<vfa::copy_u4*>(dst), n);
     return vfa::check_launch("stream_copy_kernel execution failed");
+}
+
+extern "C" int vf_stream_inplace27(void* buf, uint64_t streamBytes, void* stream)
+{
+    if (streamBytes == 0) return VF_OK;
+    if (!buf || streamBytes % 1024 != 0 || (uintptr_t)buf % 16 != 0) {
+        vfa::set_last_error("stream_inplace27: null buffer, or streamBytes not a multiple of 1024");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    const uint64_t n = streamBytes / 4;
+    const uint64_t blocks = n / 256;
+    if (blocks >= (1ull << 24)) { // a launch holds fewer than 2^32 work-items
+        vfa::set_last_error("stream_inplace27: streams longer than 2^32 - 256 elements");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    hipLaunchKernelGGL(vfa::stream_inplace27_kernel, dim3((uint32_t)blocks), dim3(256), 0, (hipStream_t)stream,
+                       static_cast<float*>(buf), n);
+    return vfa::check_launch("stream_inplace27_kernel execution failed");
 }

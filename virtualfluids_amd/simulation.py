@@ -91,10 +91,92 @@ class BoundaryConditionFactory:
         return self._resolve(table, self.geometry)
 
 
-class Simulation:
-    """Single-level time stepping of one Lattice."""
+class FusedBounceBack:
+    """The bounce-back BC lists of a dense-box lattice in the form vf_k17_run_box_bounce_back
+    applies them inside the collision launch: a per-node entry, and per entry and list the node's
+    index and its bounce-back links. lists: (name, velocity) pairs in launch order, velocity True
+    for VelocityBounceBack and False for NoSlipBounceBack. Raises ValueError where the lists do
+    not allow it: a link that leads to a node the collision updates (the epilogue would race
+    with that node's own writes), or a lattice without a fluid mask."""
 
-    def __init__(self, lattice: Lattice, bc_factory: BoundaryConditionFactory | None = None):
+    def __init__(self, lattice: Lattice, lists):
+        import ctypes
+
+        import numpy as np
+        import torch
+
+        from . import capi
+        from .grid import CX, CY, CZ
+
+        if not lattice.box or lattice.mask is None:
+            raise ValueError("fused bounce-back needs a dense box with solid nodes")
+        if len(lists) > 3:
+            raise ValueError("at most three bounce-back lists")
+        g = lattice.grid
+        fluid = g.fluid_mask
+        used = [(name, bool(vel)) for name, vel in lists
+                if lattice.bcs.get(name) is not None and lattice.bcs[name].n > 0]
+        all_k = np.unique(np.concatenate([g.bcs[name].k.astype(np.int64) for name, _ in used])) if used else \
+            np.zeros(0, np.int64)
+        entry = np.full(g.N, -1, dtype=np.int32)
+        entry[all_k] = np.arange(all_k.size, dtype=np.int32)
+        index = np.full((all_k.size, 3), -1, dtype=np.int32)
+        links = np.zeros((all_k.size, 3), dtype=np.uint32)
+        x, y, z = g.coords()
+        for l, (name, _) in enumerate(used):
+            bc = g.bcs[name]
+            k = bc.k.astype(np.int64)
+            q = np.asarray(bc.q27).reshape(27, bc.n)
+            e = entry[k]
+            index[e, l] = np.arange(bc.n, dtype=np.int32)
+            for d in range(1, 27):
+                act = (q[d] >= 0) & (q[d] <= 1)
+                if not np.any(act):
+                    continue
+                kk = k[act]
+                m = g.index((x[kk] + CX[d]) % g.nx, (y[kk] + CY[d]) % g.ny, (z[kk] + CZ[d]) % g.nz)
+                if np.any(fluid[m]):
+                    raise ValueError(f"a bounce-back link of list '{name}' leads to a fluid node")
+                links[e[act], l] |= np.uint32(1 << d)
+        dev = lattice.device
+        self.entry = torch.from_numpy(entry).to(dev)
+        self.index = torch.from_numpy(index.reshape(-1)).to(dev)
+        self.links = torch.from_numpy(links.reshape(-1).view(np.int32)).to(dev)
+        st = capi.PostBounceBack()
+        for l, (name, vel) in enumerate(used):
+            st.lists[l] = ctypes.pointer(lattice.bcs[name].struct)
+            st.velocity[l] = 1 if vel else 0
+        st.nodeEntry = ctypes.c_void_p(self.entry.data_ptr())
+        st.entryIndex = ctypes.c_void_p(self.index.data_ptr())
+        st.entryLinks = ctypes.c_void_p(self.links.data_ptr())
+        st.numberOfEntries = int(all_k.size)
+        self.struct = st
+        self.lattice = lattice
+
+    def run(self, collision_template: int = 0, stream=None):
+        """One collision of the lattice followed by the lists (vf_k17_run_box_bounce_back)."""
+        import ctypes
+
+        from . import capi
+        from .lattice import stream_handle
+
+        capi.call("vf_k17_run_box_bounce_back", self.lattice.pref(), int(collision_template),
+                  ctypes.byref(self.struct), stream_handle(stream))
+
+
+_BOUNCE_BACK = {"vf_VelocityBounceBack": True, "vf_NoSlipBounceBack": False}
+
+
+class Simulation:
+    """Single-level time stepping of one Lattice.
+
+    On a dense box whose post-collision BCs are all bounce-back (NoSlipBounceBack,
+    VelocityBounceBack) at solid walls, the collision launch applies them itself
+    (FusedBounceBack, vf_k17_run_box_bounce_back: the same values bit for bit, no BC launches);
+    fuse_bounce_back=False keeps the separate launches."""
+
+    def __init__(self, lattice: Lattice, bc_factory: BoundaryConditionFactory | None = None,
+                 fuse_bounce_back: bool = True):
         self.lattice = lattice
         f = bc_factory or BoundaryConditionFactory()
         self.velocity_fn = f.get_velocity_boundary_condition_post()
@@ -109,9 +191,28 @@ class Simulation:
             if bc is not None and bc.n > 0 and fn is None:
                 raise RuntimeError(f"The boundary condition {label} was not set!")
         self.t = 0
+        self.fused = None
+        post = [("velocity", self.velocity_fn), ("noslip", self.noslip_fn), ("geometry", self.geometry_fn)]
+        post = [(name, fn) for name, fn in post if fn and lattice.bcs.get(name) is not None and lattice.bcs[name].n]
+        if fuse_bounce_back and getattr(lattice, "box", False) and post and all(fn in _BOUNCE_BACK for _, fn in post):
+            try:
+                self.fused = FusedBounceBack(lattice, [(name, _BOUNCE_BACK[fn]) for name, fn in post])
+            except ValueError:
+                self.fused = None  # the separate launches serve these lists
 
     def _launch(self, stream=None):
         """The launches of one step (UpdateGrid27::updateGrid for one level); swaps the parity."""
+        L = self.lattice
+        if self.fused is not None:
+            self.fused.run(0, stream)
+        else:
+            self._post_collision(stream)
+        L.swap()
+        if self.pressure_fn:
+            L.apply_bc(self.pressure_fn, "pressure", stream)
+
+    def _post_collision(self, stream):
+        """Collision and the post-collision BCs as separate launches."""
         L = self.lattice
         L.collide(stream=stream)
         if self.velocity_fn:
@@ -120,9 +221,6 @@ class Simulation:
             L.apply_bc(self.noslip_fn, "noslip", stream)
         if self.geometry_fn:
             L.apply_bc(self.geometry_fn, "geometry", stream)
-        L.swap()
-        if self.pressure_fn:
-            L.apply_bc(self.pressure_fn, "pressure", stream)
 
     def step(self, stream=None):
         self._launch(stream)
