@@ -360,9 +360,10 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def inplace27_bandwidth(n=256, reps=20):
+def inplace27_bandwidth(n=256, reps=20, caps=(0, 2, 3, 4)):
     """The in-place 27-stream read-modify-write yardstick (vf_stream_inplace27) on a 27 * n^3 FP32
-    buffer, the population buffer of the headline workload: 216 B per node / time."""
+    buffer, the population buffer of the headline workload, at several caps of resident
+    workgroups per CU (0: none): {cap: 216 B per node / time} and the best cap."""
     import ctypes
 
     import torch
@@ -372,22 +373,27 @@ def inplace27_bandwidth(n=256, reps=20):
 
     N = n ** 3
     buf = torch.zeros(27 * N, dtype=torch.float32, device="cuda")
+    out = {}
+    for cap in caps:
+        def run():
+            capi.call("vf_stream_inplace27", ctypes.c_void_p(buf.data_ptr()), ctypes.c_uint64(4 * N), cap,
+                      stream_handle())
 
-    def run():
-        capi.call("vf_stream_inplace27", ctypes.c_void_p(buf.data_ptr()), ctypes.c_uint64(4 * N), stream_handle())
-
-    for _ in range(3):
-        run()
-    torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record()
-    for _ in range(reps):
-        run()
-    e.record()
-    torch.cuda.synchronize()
+        for _ in range(3):
+            run()
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            run()
+        e.record()
+        torch.cuda.synchronize()
+        out[cap] = round(216.0 * N / (s.elapsed_time(e) / 1000.0 / reps) / 1e9, 1)
     del buf
     torch.cuda.empty_cache()
-    return 216.0 * N / (s.elapsed_time(e) / 1000.0 / reps) / 1e9
+    best = max(out, key=out.get)
+    return {"GBps_by_workgroups_per_CU": {str(k): v for k, v in out.items()}, "best_workgroups_per_CU": best,
+            "best_GBps": out[best]}
 
 
 def cpu_baseline(n=256, target_s=8.0):
@@ -483,6 +489,21 @@ def storage_precision(n=64, steps=200):
     return out
 
 
+def run_case(case, steps=None):
+    """One full-size case alone (profiling runs): its record, with `steps` steps when given (the
+    sphere channel averages its drag over the last tenth, the two-level channel its outlet energy
+    over the last quarter; checkpoint intervals as in the full runs)."""
+    if case == "sphere":
+        return sphere_channel_gradient() if steps is None else \
+            sphere_channel_gradient(steps=steps, average_last=max(1, steps // 10))
+    if case == "two-level":
+        out = two_level_gradient() if steps is None else \
+            two_level_gradient(steps=steps, average_last=max(1, steps // 4))
+        out.pop("_grad")
+        return out
+    return published_cavity() if steps is None else published_cavity(steps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -499,11 +520,18 @@ def main():
     ap.add_argument("--no-cavity", action="store_true", help="skip the reference's published cavity benchmark")
     ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
                     help="time one storage type only (for profiling runs)")
+    ap.add_argument("--only-case", choices=["sphere", "two-level", "cavity"], default=None,
+                    help="run one full-size case alone and print its record (for profiling runs)")
+    ap.add_argument("--case-steps", type=int, default=None,
+                    help="steps of --only-case (default: the case's full length)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (one GPU owns the domain); got --gpus {args.gpus}",
               file=sys.stderr)
         sys.exit(2)
+    if args.only_case is not None:
+        print(json.dumps(run_case(args.only_case, args.case_steps)), flush=True)
+        return
 
     import torch
 
@@ -518,27 +546,34 @@ def main():
     fields = taylor_green_fields(grid, u0)
     lim = (1e6, 1e6, 1e6)  # the Taylor-Green app switches the limiter off
 
-    # the copy and in-place yardsticks first: they also bring the GPU to its clocks before the
-    # timed steps
-    copy_gbps, blit_gbps = copy_bandwidth()
-    rmw_gbps = inplace27_bandwidth()
-
     results = {}
     velocity = {}
     adjoint = None
+    copy_gbps = blit_gbps = rmw = None
     for storage in (["fp32", "fp16", "bf16"] if args.only is None else [args.only]):
         L = Lattice(grid, omega, storage=storage, limiters=lim)
         L.init_equilibrium(*fields)
+        if copy_gbps is None:
+            # the copy and in-place yardsticks right before the first timed steps, with the
+            # lattice already set up: the GPU then enters the timed window busy, not from idle
+            copy_gbps, blit_gbps = copy_bandwidth()
+            rmw = inplace27_bandwidth()
+        # the longer steady window first: after idle the GPU's power management takes some 30
+        # launches of this kernel to settle (per-launch times rise by up to 30 % and come back),
+        # so the main window below starts in the settled state
+        steady = None
+        if args.steady_steps > 0:
+            ts = time_steps(L, args.steady_steps, 0)
+            steady = {"steps": args.steady_steps, "MLUPS": round(N * args.steady_steps / ts / 1e6, 1),
+                      "ms_per_step": round(1e3 * ts / args.steady_steps, 4)}
         # the same window for every storage type: --warmup untimed steps, then --steps timed
         t = time_steps(L, args.steps, args.warmup)
         velocity[storage] = velocity_field(L)
         bytes_per_node = 2 * 27 * L.f.element_size()
         r = {"t": t, "steps": args.steps, "mlups": N * args.steps / t / 1e6, "ms": 1e3 * t / args.steps,
              "gbps": bytes_per_node * N * args.steps / t / 1e9, "bytes_per_node": bytes_per_node}
-        if args.steady_steps > 0:
-            ts = time_steps(L, args.steady_steps, 0)
-            r["steady"] = {"steps": args.steady_steps, "MLUPS": round(N * args.steady_steps / ts / 1e6, 1),
-                           "ms_per_step": round(1e3 * ts / args.steady_steps, 4)}
+        if steady is not None:
+            r["steady"] = steady
         r["mass_drift"] = float(L.moment_sums()[0])
         results[storage] = r
         if storage == "fp32" and not args.no_adjoint:
@@ -608,7 +643,8 @@ def main():
         "published_cavity": None if args.no_cavity else published_cavity(),
         "hbm_copy_GBps": round(copy_gbps, 1),
         "blit_copy_GBps": round(blit_gbps, 1),
-        "inplace27_GBps": round(rmw_gbps, 1),
+        "inplace27": rmw,
+        "vs_inplace27": round(r["gbps"] / rmw["best_GBps"], 4) if main_key == "fp32" else None,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)

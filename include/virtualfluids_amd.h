@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define VF_ABI_VERSION 4
+#define VF_ABI_VERSION 5
 
 enum vf_status {
     VF_OK = 0,
@@ -73,7 +73,9 @@ enum vf_row_addressing {
 enum vf_adjoint_kernel {
     VF_ADJOINT_KERNEL_AUTO = 0,
     VF_ADJOINT_KERNEL_ROW_WHOLE = 1, /* row kernel over whole rows, without the scalar tail launch */
-    VF_ADJOINT_KERNEL_NODE = 2       /* one node per thread */
+    VF_ADJOINT_KERNEL_NODE = 2,      /* one node per thread */
+    VF_ADJOINT_KERNEL_ROW = 3        /* row kernel over the whole waves of every row, whatever their
+                                        number, and the scalar tail launch for the columns beyond */
 };
 typedef struct vf_kernel_choice {
     int32_t boxKernel;          /* enum vf_box_kernel */
@@ -156,6 +158,12 @@ typedef struct vf_q_boundary_conditions {
     const float* Vz;
     const float* RhoBC;       /* pressure BC: prescribed drho per BC node */
     int32_t direction;        /* pressure BC: D3Q27 index of the outflow side (dP00, dM00, ...) */
+    /* Optional (NULL: unused; no counterpart in the reference; ABI 5): 27*numberOfBCnodes floats,
+     * postCollision[d*numberOfBCnodes + i]. The post-collision BC kernels (vf_NoSlip*, vf_Velocity*)
+     * store there the 27 post-collision values each BC node's links were computed from; their
+     * adjoints then read the linearisation point from it instead of recomputing each BC node's
+     * collision. */
+    float* postCollision;
 } vf_q_boundary_conditions;
 
 /* ICells / ICellNeigh (src/gpu/core/Calculation/Calculation.h:72-89) */
@@ -186,9 +194,11 @@ int vf_stream_copy(const void* src, void* dst, uint64_t bytes, void* stream);
  * streamBytes a multiple of 1024 and below 2^34): every element is read and written into the
  * opposite stream at the same offset (d <-> inv(d) of the D3Q27 order, stream 0 in place), the
  * memory pattern of the in-place collision step without its arithmetic (27 reads and 27 writes
- * per node). No counterpart in the reference: the yardstick the collision kernels' bandwidth is
- * measured against. */
-int vf_stream_inplace27(void* buf, uint64_t streamBytes, void* stream);
+ * per node), in workgroups of 256 elements per stream; workgroupsPerCU (0: no cap, else 1-32)
+ * caps the resident workgroups per CU as the FP32 row kernel's residency cap does. No
+ * counterpart in the reference: the yardstick the collision kernels' bandwidth is measured
+ * against. */
+int vf_stream_inplace27(void* buf, uint64_t streamBytes, int32_t workgroupsPerCU, void* stream);
 
 /* ---------------------------------------------------------------------------------
  * K17 collide + eso-twist stream
@@ -289,8 +299,9 @@ int vf_k17_adjoint(const vf_lbm_parameter* para, float* adjoint, float* velocity
 /* Adjoints of the BC kernels, run in reverse order within a step (pressure BC first, then the
  * post-collision BCs in reverse launch order, then the collision adjoint). For the post BCs
  * para->distributions holds the populations BEFORE the step (each BC node's collision is
- * recomputed from them with para's omega, limiters and force); for the pressure BC it holds the
- * populations after the step. para->isEvenTimestep is the parity the forward kernel used.
+ * recomputed from them with para's omega, limiters and force), or bc->postCollision holds the
+ * post-collision values the forward BC kernel stored (then nothing is recomputed); for the
+ * pressure BC para->distributions holds the populations after the step. para->isEvenTimestep is the parity the forward kernel used.
  * The adjoint of every written cell moves to the cells the value was computed from and the
  * written cell's adjoint is cleared. velocityAdjoint (optional, 3*numberOfBCnodes floats: Vx,
  * Vy, Vz blocks) and rhoBCAdjoint (optional, numberOfBCnodes) accumulate parameter gradients.

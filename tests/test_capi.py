@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
     lib = capi.load()
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.vf_abi_version() == 4
+    assert lib.vf_abi_version() == 5
 
 
 def test_signature_table_covers_header():
@@ -107,7 +107,9 @@ def test_stream_copy_validates_alignment():
 
 def test_stream_inplace27_validates_sizes():
     lib = capi.load()
-    assert lib.vf_stream_inplace27(None, 1024, None) == capi.VF_ERR_INVALID_ARGUMENT
-    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1000, None) == capi.VF_ERR_INVALID_ARGUMENT
-    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1 << 34, None) == capi.VF_ERR_INVALID_ARGUMENT
-    assert lib.vf_stream_inplace27(None, 0, None) == capi.VF_OK  # nothing to do
+    assert lib.vf_stream_inplace27(None, 1024, 0, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1000, 0, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1 << 34, 0, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1024, -1, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(ctypes.c_void_p(0x100), 1024, 33, None) == capi.VF_ERR_INVALID_ARGUMENT
+    assert lib.vf_stream_inplace27(None, 0, 0, None) == capi.VF_OK  # nothing to do

@@ -199,10 +199,12 @@ def _duct(q_random, seed=47):
 @pytest.mark.parametrize("fn", sorted(POST_KINDS))
 @pytest.mark.parametrize("even", [True, False])
 @pytest.mark.parametrize("box", [True, False])  # (x, y, z) addressing or neighbour arrays
-def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box):
+@pytest.mark.parametrize("saved", [False, True])  # recomputed or stored post-collision values
+def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box, saved):
     import torch
 
     from virtualfluids_amd import capi
+    from virtualfluids_amd.adjoint import _bc_struct
     from virtualfluids_amd.lattice import Lattice
 
     kind = POST_KINDS[fn]
@@ -217,8 +219,16 @@ def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box):
     lam = torch.from_numpy(lam0).cuda()
     bc = L.bcs[bc_name]
     ubar = torch.zeros(3 * bc.n, dtype=torch.float32, device="cuda")
-    print(f"{fn}Adjoint even={even} nodes={bc.n}", flush=True)
-    capi.call(fn + "Adjoint", L.pref(), ctypes.byref(bc.struct), ctypes.c_void_p(lam.data_ptr()),
+    post = None
+    if saved:
+        # the forward BC kernel stores the post-collision values it reads, on a copy of the state
+        before = L.f.clone()
+        L.collide()
+        post = torch.full((27 * bc.n,), float("nan"), dtype=torch.float32, device="cuda")
+        capi.call(fn, L.pref(), ctypes.byref(_bc_struct(bc, post)), None)
+        L.f.copy_(before)
+    print(f"{fn}Adjoint even={even} nodes={bc.n} saved={saved}", flush=True)
+    capi.call(fn + "Adjoint", L.pref(), ctypes.byref(_bc_struct(bc, post)), ctypes.c_void_p(lam.data_ptr()),
               ctypes.c_void_p(ubar.data_ptr()), None)
     got = lam.cpu().numpy().astype(np.float64)
 

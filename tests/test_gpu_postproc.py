@@ -113,7 +113,8 @@ def test_stream_copy_copies_exactly(gpu, nbytes):
 
 
 @pytest.mark.parametrize("stream_bytes", [1024, 3 * 4096, 4 << 20])
-def test_stream_inplace27_swaps_opposite_streams(gpu, stream_bytes):
+@pytest.mark.parametrize("wgs_per_cu", [0, 2])
+def test_stream_inplace27_swaps_opposite_streams(gpu, stream_bytes, wgs_per_cu):
     import torch
 
     from virtualfluids_amd import capi
@@ -123,11 +124,13 @@ def test_stream_inplace27_swaps_opposite_streams(gpu, stream_bytes):
     n = stream_bytes // 4
     a = torch.randn(27 * n + 64, dtype=torch.float32, device="cuda")
     ref = a.clone()
-    capi.call("vf_stream_inplace27", ctypes.c_void_p(a.data_ptr()), ctypes.c_uint64(stream_bytes), stream_handle())
+    capi.call("vf_stream_inplace27", ctypes.c_void_p(a.data_ptr()), ctypes.c_uint64(stream_bytes), wgs_per_cu,
+              stream_handle())
     torch.cuda.synchronize()
     want = ref[:27 * n].view(27, n)[torch.as_tensor(INV, device="cuda")].reshape(-1)
     assert torch.equal(a[:27 * n], want)
     assert torch.equal(a[27 * n:], ref[27 * n:])  # nothing beyond the 27 streams
-    capi.call("vf_stream_inplace27", ctypes.c_void_p(a.data_ptr()), ctypes.c_uint64(stream_bytes), stream_handle())
+    capi.call("vf_stream_inplace27", ctypes.c_void_p(a.data_ptr()), ctypes.c_uint64(stream_bytes), wgs_per_cu,
+              stream_handle())
     torch.cuda.synchronize()
     assert torch.equal(a, ref)

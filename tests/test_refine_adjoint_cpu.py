@@ -146,3 +146,24 @@ def test_lattice_scale_adjoint_dot_product(orc, fine_to_coarse, even_coarse, off
         ad = ac @ vc + af @ vf
         print(f"f2c={fine_to_coarse} even={even_coarse} off={offsets}: adjoint {ad:.10e} FD {fd:.10e}")
         assert abs(ad - fd) < 1e-7 * abs(fd) + 1e-8
+
+
+def test_interface_writes_disjoint_detects_overlaps():
+    """The replay of the two-level backward applies the interface writes in place only where the
+    written nodes are disjoint from what the reverse sweep reads: nested boxes are, and an FC cell
+    reading a CF-written fine node, or an FC write onto a pressure BC's source node, is caught."""
+    from virtualfluids_amd.adjoint import interface_writes_disjoint
+    from virtualfluids_amd.grid import channel
+    from virtualfluids_amd.refinement import InterfaceCells, nested_box
+
+    nb = nested_box((24, 20, 20), (6, 5, 5), (14, 13, 13))
+    assert interface_writes_disjoint(nb.coarse, nb.fine, nb.cf, nb.fc)
+    # an FC cell whose fine cell is a CF target
+    fc = InterfaceCells(np.append(nb.fc.coarse, nb.fc.coarse[:1]), np.append(nb.fc.fine, nb.cf.fine[:1]))
+    assert not interface_writes_disjoint(nb.coarse, nb.fine, nb.cf, fc)
+    # a channel: the FC coarse targets stay clear of the outlet's kN, unless one is made to hit it
+    nc = nested_box(None, (5, 4, 4), (11, 10, 10), coarse=channel((16, 12, 12), inflow=0.03))
+    assert interface_writes_disjoint(nc.coarse, nc.fine, nc.cf, nc.fc)
+    kn = int(nc.coarse.bcs["pressure"].kN[0])
+    fc = InterfaceCells(np.append(nc.fc.coarse, kn).astype(np.uint32), np.append(nc.fc.fine, nc.fc.fine[:1]))
+    assert not interface_writes_disjoint(nc.coarse, nc.fine, nc.cf, fc)

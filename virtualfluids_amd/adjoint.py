@@ -67,6 +67,15 @@ def _bc_restore(L: Lattice, snap):
         L.set_porosity(por)
 
 
+def _bc_struct(bc, post):
+    """The BC list's struct, with postCollision pointing at `post` (27 * n FP32) when given."""
+    if post is None:
+        return bc.struct
+    s = capi.QBoundaryConditions.from_buffer_copy(bc.struct)
+    s.postCollision = ctypes.c_void_p(post.data_ptr())
+    return s
+
+
 def _storage_input(t: torch.Tensor, L: Lattice, what: str) -> torch.Tensor:
     """A working copy of an initial buffer in the lattice's storage type. The buffer may hold the
     storage type itself or FP32 (for 16-bit lattices: then the outputs and the gradient stay FP32)."""
@@ -96,9 +105,19 @@ class Stepper:
         bc = self.L.bcs.get(name)
         return bc if bc is not None and bc.n > 0 else None
 
-    def forward(self, buf: torch.Tensor, even: bool, out: torch.Tensor | None, stream) -> torch.Tensor:
+    def post_lists(self):
+        """(name, node count) of the post-collision BC lists a step applies, in launch order."""
+        out = []
+        for name, fn in (("velocity", self.velocity), ("noslip", self.noslip), ("geometry", self.geometry)):
+            bc = self._bc(name)
+            if fn and bc is not None:
+                out.append((name, bc.n))
+        return out
+
+    def forward(self, buf: torch.Tensor, even: bool, out: torch.Tensor | None, stream, save=None) -> torch.Tensor:
         """One step from `buf` (parity `even`); into `out` when given, else in place. Returns
-        the buffer holding the result."""
+        the buffer holding the result. save: {BC list name: FP32 tensor of 27 * n} to receive
+        the post-collision values of that list's nodes (the adjoint's linearisation point)."""
         L, s = self.L, stream_handle(stream)
         if out is not None:
             if L.box:
@@ -120,17 +139,19 @@ class Stepper:
         for name, fn in (("velocity", self.velocity), ("noslip", self.noslip), ("geometry", self.geometry)):
             bc = self._bc(name)
             if fn and bc is not None:
-                capi.call(fn, ctypes.byref(p), ctypes.byref(bc.struct), s)
+                post = save.get(name) if save else None
+                capi.call(fn, ctypes.byref(p), ctypes.byref(_bc_struct(bc, post)), s)
         bc = self._bc("pressure")
         if self.pressure and bc is not None:
             capi.call(self.pressure, ctypes.byref(_param(L, target, not even)), ctypes.byref(bc.struct), s)
         return target
 
     def adjoint(self, before: torch.Tensor, after: torch.Tensor, lam: torch.Tensor, even: bool, ubar, rbar, padj,
-                stream):
+                stream, saved=None):
         """Reverse of one step: `before`/`after` are the states around it, `even` its parity;
         lam (FP32) is updated in place; ubar (3*n) / rbar (n) / padj (N) accumulate the adjoints
-        of the wall velocities, outlet densities and porosity."""
+        of the wall velocities, outlet densities and porosity. saved: the post-collision values
+        forward(save=...) stored for this step (the post BC adjoints then recompute nothing)."""
         L, s = self.L, stream_handle(stream)
         bc = self._bc("pressure")
         if self.pressure and bc is not None:
@@ -141,7 +162,8 @@ class Stepper:
                                ("velocity", self.velocity, ubar)):
             bc = self._bc(name)
             if fn and bc is not None:
-                capi.call(fn + "Adjoint", ctypes.byref(p), ctypes.byref(bc.struct), _vp(lam), _vp(grad), s)
+                post = saved.get(name) if saved else None
+                capi.call(fn + "Adjoint", ctypes.byref(p), ctypes.byref(_bc_struct(bc, post)), _vp(lam), _vp(grad), s)
         fn = "vf_k17_adjoint_box" if L.box else "vf_k17_adjoint"
         capi.call(fn, ctypes.byref(p), _vp(lam), None, _vp(padj), s)
 
@@ -215,20 +237,24 @@ class _K17Steps(torch.autograd.Function):
         ubar = torch.zeros(3 * vel_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_velocity else None
         rbar = torch.zeros(pr_bc.n, dtype=torch.float32, device=lam.device) if ctx.has_rho else None
         padj = torch.zeros(L.N, dtype=torch.float32, device=lam.device) if ctx.has_porosity else None
+        lists = stepper.post_lists()
         for start, even_start, ckpt in reversed(ctx.checkpoints):
             n = min(K, steps - start)
             states = [ckpt]
+            # the post-collision values of the BC nodes, step by step, for the BC adjoints
+            posts = [{name: torch.empty(27 * nb, dtype=torch.float32, device=lam.device) for name, nb in lists}
+                     for _ in range(n)]
             even = even_start
-            for _ in range(n):
-                states.append(stepper.forward(states[-1], even, torch.empty_like(ckpt), stream))
+            for j in range(n):
+                states.append(stepper.forward(states[-1], even, torch.empty_like(ckpt), stream, save=posts[j]))
                 even = not even
             for j in reversed(range(n)):
                 even_j = even_start ^ (j % 2 == 1)
                 if observe is not None and start + j >= ctx.observe_from:
                     # the observed objective of the state after this step
                     _observe_grad(observe, states[j + 1], not even_j, lam, gJ)
-                stepper.adjoint(states[j], states[j + 1], lam, even_j, ubar, rbar, padj, stream)
-            del states
+                stepper.adjoint(states[j], states[j + 1], lam, even_j, ubar, rbar, padj, stream, saved=posts[j])
+            del states, posts
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
         return lam.to(ctx.in_dtype), gv, rbar, padj, None, None, None, None, None, None, None
@@ -280,7 +306,7 @@ class _TwoLevelSteps(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, fc0, ff0, velocity, rho_bc, por_c, por_f, sc, sf, cells, steps, K, even0, observe, observe_from,
-                graphs):
+                graphs, in_place):
         Lc, Lf = sc.L, sf.L
         # in place on one buffer per level: the same arithmetic as the out-of-place replay
         C, F = _storage_input(fc0, Lc, "fc0"), _storage_input(ff0, Lf, "ff0")
@@ -314,6 +340,7 @@ class _TwoLevelSteps(torch.autograd.Function):
             i += n
         run.close()
         ctx.sc, ctx.sf, ctx.cells, ctx.steps, ctx.K = sc, sf, cells, steps, K
+        ctx.in_place = in_place
         ctx.observe, ctx.observe_from = observe, observe_from
         ctx.checkpoints = checkpoints
         ctx.flags = (velocity is not None, rho_bc is not None, por_c is not None, por_f is not None)
@@ -351,16 +378,24 @@ class _TwoLevelSteps(torch.autograd.Function):
         pfb = torch.zeros(Lf.N, device=dev) if has_pf else None
         s = stream_handle(stream)
         cf, fc = cells
+        lists_c, lists_f = sc.post_lists(), sf.post_lists()
+
+        def post_buffers(lists):
+            return {name: torch.empty(27 * nb, dtype=torch.float32, device=dev) for name, nb in lists}
+
         for start, even_start, C0, F0 in reversed(ctx.checkpoints):
             n = min(K, steps - start)
             states = []
+            # the post-collision values of every level's BC nodes, step by step, for the BC adjoints
+            posts = [(post_buffers(lists_c), post_buffers(lists_f), post_buffers(lists_f)) for _ in range(n)]
             C, F, even = C0, F0, even_start
-            for _ in range(n):
-                st = _two_level_step(sc, sf, cells, C, F, even, stream)
+            for j in range(n):
+                st = _two_level_step(sc, sf, cells, C, F, even, stream, posts[j], ctx.in_place)
                 states.append((even, C, F) + st)
                 C, F = st[-2], st[-1]
                 even = not even
             for j, (even, C0s, F0s, F1, F2, C1, C2, _, _) in reversed(list(enumerate(states))):
+                post_c, post_f1, post_f2 = posts[j]
                 if ctx.observe is not None and start + j >= ctx.observe_from:
                     # the observed objective of the state after this step
                     _observe_grad(ctx.observe, C2, not even, lc, gJ)
@@ -372,14 +407,14 @@ class _TwoLevelSteps(torch.autograd.Function):
                 if fc.n:
                     capi.call("vf_ScaleFineToCoarseCompressibleAdjoint", ctypes.byref(pc), ctypes.byref(pf),
                               ctypes.byref(fc.struct), _neigh(fc), _vp(lc), _vp(lf), s)
-                sc.adjoint(C0s, C1, lc, even, ubar, rbar, pcb, stream)
-                sf.adjoint(F1, F2, lf, False, None, None, pfb, stream)
-                sf.adjoint(F0s, F1, lf, True, None, None, pfb, stream)
-            del states
+                sc.adjoint(C0s, C1, lc, even, ubar, rbar, pcb, stream, saved=post_c)
+                sf.adjoint(F1, F2, lf, False, None, None, pfb, stream, saved=post_f2)
+                sf.adjoint(F0s, F1, lf, True, None, None, pfb, stream, saved=post_f1)
+            del states, posts
         ctx.checkpoints = None
         gv = ubar.view(3, -1) if ubar is not None else None
         return (lc.to(ctx.in_dtypes[0]), lf.to(ctx.in_dtypes[1]), gv, rbar, pcb, pfb, None, None, None, None, None,
-                None, None, None, None)
+                None, None, None, None, None)
 
 
 def _observe_grad(observe, buf, even, lam, gJ):
@@ -402,18 +437,47 @@ def _neigh(cells):
     return ctypes.byref(cells.neigh) if cells.offsets is not None else None
 
 
-def _two_level_step(sc, sf, cells, C, F, even, stream):
+def _cell_nodes(grid, base) -> np.ndarray:
+    """The 8 nodes of the cells at `base` (+x, +y, +z neighbours of the grid), flattened."""
+    b = np.asarray(base, dtype=np.int64)
+    n1, n2, n4 = grid.nbx[b], grid.nby[b], grid.nbz[b]
+    n3 = grid.nby[n1]
+    return np.concatenate([b, n1, n2, n3, n4, grid.nbz[n1], grid.nbz[n2], grid.nbz[n3]]).astype(np.int64)
+
+
+def interface_writes_disjoint(coarse_grid, fine_grid, cf, fc) -> bool:
+    """Whether a coarse step's interface writes leave every value that the reverse sweep reads
+    from the states before them unchanged, so that the replay may apply them in place: the
+    coarse nodes fine -> coarse writes are not the source nodes kN of a coarse pressure BC, and the
+    fine nodes coarse -> fine writes are neither the cell nodes fine -> coarse reads nor the kN of a
+    fine pressure BC. Each node's cell is a disjoint set of addresses, so disjoint node sets mean
+    disjoint memory. cf, fc: InterfaceCells (host arrays)."""
+    def kN(grid):
+        bc = grid.bcs.get("pressure")
+        return np.asarray(bc.kN, dtype=np.int64) if bc is not None and bc.n and bc.kN is not None else np.zeros(0, np.int64)
+
+    if np.intersect1d(np.asarray(fc.coarse, dtype=np.int64), kN(coarse_grid)).size:
+        return False
+    cf_written = _cell_nodes(fine_grid, cf.fine) if cf.n else np.zeros(0, np.int64)
+    fc_read = _cell_nodes(fine_grid, fc.fine) if fc.n else np.zeros(0, np.int64)
+    return not (np.intersect1d(cf_written, fc_read).size or np.intersect1d(cf_written, kN(fine_grid)).size)
+
+
+def _two_level_step(sc, sf, cells, C, F, even, stream, posts=None, in_place=False):
     """One coarse step out of place: returns (F1, F2, C1, C2, C_next, F_next) where F1, F2 follow
     the two fine steps, C1 the coarse step, C2 = C_next the fine -> coarse write and F_next the
-    coarse -> fine write."""
+    coarse -> fine write. posts: (coarse, first fine, second fine) save dicts of Stepper.forward.
+    in_place: the interface writes go into C1 and F2 themselves (C2 is C1, F_next is F2), which
+    interface_writes_disjoint allows; else into copies."""
     Lc, Lf = sc.L, sf.L
     cf, fc = cells
     s = stream_handle(stream)
-    F1 = sf.forward(F, True, torch.empty_like(F), stream)
-    F2 = sf.forward(F1, False, torch.empty_like(F), stream)
-    C1 = sc.forward(C, even, torch.empty_like(C), stream)
-    C2 = C1.clone()
-    F3 = F2.clone()
+    pc_, pf1, pf2 = posts if posts is not None else (None, None, None)
+    F1 = sf.forward(F, True, torch.empty_like(F), stream, save=pf1)
+    F2 = sf.forward(F1, False, torch.empty_like(F), stream, save=pf2)
+    C1 = sc.forward(C, even, torch.empty_like(C), stream, save=pc_)
+    C2 = C1 if in_place else C1.clone()
+    F3 = F2 if in_place else F2.clone()
     pc = _param(Lc, C2, not even)
     pf = _param(Lf, F2, True)
     if fc.n:
@@ -482,8 +546,10 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
     cf, fc = interface
     cells = (DeviceInterfaceCells(cf, coarse.device), DeviceInterfaceCells(fc, coarse.device))
     even0 = coarse.even if even is None else bool(even)
+    # the replay of the backward applies the interface in place where that changes nothing it reads
+    in_place = interface_writes_disjoint(coarse.grid, fine.grid, cf, fc)
     return _TwoLevelSteps.apply(fc0, ff0, velocity, rho_bc, porosity_coarse, porosity_fine, sc, sf, cells, steps,
-                                checkpoint_every, even0, observe, observe_from, graphs)
+                                checkpoint_every, even0, observe, observe_from, graphs, in_place)
 
 
 def pre_collision_index(grid, even: bool) -> np.ndarray:

@@ -36,7 +36,7 @@ _vp = ctypes.c_void_p
 # vf_box_kernel, vf_row_addressing, vf_adjoint_kernel
 BOX_KERNEL_AUTO, BOX_KERNEL_ROW, BOX_KERNEL_NODE = 0, 1, 2
 ROW_ADDRESSING_AUTO, ROW_ADDRESSING_BUFFER, ROW_ADDRESSING_GLOBAL = 0, 1, 2
-ADJOINT_KERNEL_AUTO, ADJOINT_KERNEL_ROW_WHOLE, ADJOINT_KERNEL_NODE = 0, 1, 2
+ADJOINT_KERNEL_AUTO, ADJOINT_KERNEL_ROW_WHOLE, ADJOINT_KERNEL_NODE, ADJOINT_KERNEL_ROW = 0, 1, 2, 3
 
 
 class KernelChoice(ctypes.Structure):
@@ -101,6 +101,7 @@ class QBoundaryConditions(ctypes.Structure):
         ("Vz", _vp),
         ("RhoBC", _vp),
         ("direction", ctypes.c_int32),
+        ("postCollision", _vp),
     ]
 
 
@@ -125,6 +126,9 @@ class InterpolationCellNeighbor(ctypes.Structure):
     _fields_ = [("x", _vp), ("y", _vp), ("z", _vp)]
 
 
+# VF_ABI_VERSION of include/virtualfluids_amd.h that these structs mirror
+ABI_VERSION = 5
+
 _P = ctypes.POINTER(LBMParameter)
 _Q = ctypes.POINTER(QBoundaryConditions)
 
@@ -132,7 +136,7 @@ _Q = ctypes.POINTER(QBoundaryConditions)
 SIGNATURES = {
     "vf_abi_version": [],
     "vf_stream_copy": [_vp, _vp, ctypes.c_uint64, _vp],
-    "vf_stream_inplace27": [_vp, ctypes.c_uint64, _vp],
+    "vf_stream_inplace27": [_vp, ctypes.c_uint64, ctypes.c_int32, _vp],
     "vf_k17_run": [_P, _vp],
     "vf_k17_run_on_indices": [_P, _vp, ctypes.c_uint32, ctypes.c_int32, _vp],
     "vf_k17_run_box": [_P, ctypes.c_int32, _vp],
@@ -177,6 +181,9 @@ def load():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run virtualfluids_amd/build.py (or __graft_entry__.build())")
         lib = ctypes.CDLL(LIB_PATH)
+        lib.vf_abi_version.restype = ctypes.c_int
+        if lib.vf_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{LIB_PATH} has ABI {lib.vf_abi_version()}, this binding needs {ABI_VERSION}: rebuild it")
         for name, args in SIGNATURES.items():
             fn = getattr(lib, name)
             fn.argtypes = args

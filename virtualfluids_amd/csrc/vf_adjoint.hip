@@ -383,7 +383,9 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj
     if (p.nx > 64 && p.nx % 64 != 0 && p.nx % 64 < 32 && kc.adjointKernel != VF_ADJOINT_KERNEL_ROW_WHOLE)
         rowEnd = p.nx - p.nx % 64;
     const int waves = (int)((rowEnd + 63) / 64);
-    if (rowEnd <= (uint32_t)kAdjRowMaxThreads && 8 % waves == 0 && kc.adjointKernel != VF_ADJOINT_KERNEL_NODE) {
+    const bool anyWaves = kc.adjointKernel == VF_ADJOINT_KERNEL_ROW;
+    if (rowEnd <= (uint32_t)kAdjRowMaxThreads && (8 % waves == 0 || anyWaves) &&
+        kc.adjointKernel != VF_ADJOINT_KERNEL_NODE) {
         // the buffer form needs kRowSplit slots of the adjoint (the larger element) within
         // 32-bit offsets
         const S* f = static_cast<const S*>(p.distributions);

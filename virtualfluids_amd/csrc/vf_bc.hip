@@ -23,6 +23,7 @@ struct BcArgs {
     const float* vy;
     const float* vz;
     float omega;
+    float* postOut; // 27 * nbc post-collision values stored for the adjoint, or null
 };
 
 template <class S, int KIND>
@@ -40,6 +41,11 @@ __global__ __launch_bounds__(256) void bc_post_kernel(S* __restrict__ f, Addr a,
     for (int d = 1; d < 27; ++d) qd[d] = b.q[(uint64_t)d * b.nbc + i];
 #pragma unroll
     for (int d = 0; d < 27; ++d) post[d] = read_post(f, a, n, d, even);
+    if (b.postOut) {
+        // the linearisation point of this node's adjoint (SoA: coalesced over the BC nodes)
+#pragma unroll
+        for (int d = 0; d < 27; ++d) b.postOut[(uint64_t)d * b.nbc + i] = post[d];
+    }
 
     float drho = 0.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f, cu_sq = 0.0f;
     if (KIND >= kNoSlipInterp) {
@@ -154,7 +160,7 @@ template <int KIND> static int launch_bc_post(const vf_lbm_parameter* p, const v
         return VF_ERR_INVALID_ARGUMENT;
     }
     const Addr a = make_addr(*p);
-    BcArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega};
+    BcArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega, bc->postCollision};
     const dim3 grid((bc->numberOfBCnodes + kBcBlock - 1) / kBcBlock);
     hipStream_t s = (hipStream_t)stream;
     switch (p->storage) {

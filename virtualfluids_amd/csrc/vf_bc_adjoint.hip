@@ -30,6 +30,7 @@ struct BcAdjArgs {
     float* ubar; // 3*nbc (Vx block, Vy block, Vz block) or null
     K17Rates rates;
     const float* porosity;
+    const float* post; // 27 * nbc post-collision values the forward kernel stored, or null
 };
 
 __device__ __forceinline__ float* lam_cell(float* lam, const Addr& a, const uint32_t (&n)[8], int d, bool even)
@@ -37,7 +38,9 @@ __device__ __forceinline__ float* lam_cell(float* lam, const Addr& a, const uint
     return slot_ptr(lam, a.N, d, even) + n[node_sel(d)];
 }
 
-template <class S, int KIND>
+// SAVED: the node's post-collision values come from b.post (stored by the forward kernel);
+// otherwise they are recomputed from its pre-collision values in f
+template <class S, int KIND, bool SAVED>
 __global__ __launch_bounds__(256) void bc_post_adjoint_kernel(const S* __restrict__ f, float* __restrict__ lam, Addr a,
                                                               BcAdjArgs b, int even_flag)
 {
@@ -46,15 +49,16 @@ __global__ __launch_bounds__(256) void bc_post_adjoint_kernel(const S* __restric
     const bool even = even_flag != 0;
     uint32_t n[8];
     cell_nodes(a, (uint32_t)b.k[i], n);
-    // the node's post-collision values, recomputed from its pre-collision values
     float post[27], q[27], G[27], fb[27], Ub[3];
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
-        post[d] = read_pre(f, a, n, d, even);
+        post[d] = SAVED ? b.post[(uint64_t)d * b.nbc + i] : read_pre(f, a, n, d, even);
         q[d] = d == 0 ? -1.0f : b.q[(uint64_t)d * b.nbc + i];
     }
-    MacroOut mo;
-    k17_collide(post, b.rates, mo, b.porosity ? 1.0f - 0.5f * b.porosity[b.k[i]] : 1.0f);
+    if constexpr (!SAVED) {
+        MacroOut mo;
+        k17_collide(post, b.rates, mo, b.porosity ? 1.0f - 0.5f * b.porosity[b.k[i]] : 1.0f);
+    }
     // adjoint of the written cells (incoming direction inv(d), next parity), then cleared; the
     // 27 candidate cells are loaded unconditionally (every one is a valid cell) so that the
     // loads overlap instead of waiting one by one behind the link tests
@@ -138,8 +142,12 @@ template <int KIND> struct PostAdjL {
         static void run(const vf_lbm_parameter& p, const BcAdjArgs& b, float* lam, hipStream_t s)
         {
             const dim3 grid((b.nbc + kBcBlock - 1) / kBcBlock);
-            hipLaunchKernelGGL((bc_post_adjoint_kernel<S, KIND>), grid, dim3(kBcBlock), 0, s, (const S*)p.distributions, lam,
-                               make_addr(p), b, p.isEvenTimestep);
+            if (b.post)
+                hipLaunchKernelGGL((bc_post_adjoint_kernel<S, KIND, true>), grid, dim3(kBcBlock), 0, s,
+                                   (const S*)p.distributions, lam, make_addr(p), b, p.isEvenTimestep);
+            else
+                hipLaunchKernelGGL((bc_post_adjoint_kernel<S, KIND, false>), grid, dim3(kBcBlock), 0, s,
+                                   (const S*)p.distributions, lam, make_addr(p), b, p.isEvenTimestep);
         }
     };
 };
@@ -187,7 +195,7 @@ int launch_post_adjoint(const vf_lbm_parameter* p, const vf_q_boundary_condition
         return VF_ERR_INVALID_ARGUMENT;
     }
     const BcAdjArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega, velocity ? ubar : nullptr,
-                      make_k17_rates(*p), p->porosity};
+                      make_k17_rates(*p), p->porosity, bc->postCollision};
     rc = by_storage<PostAdjL<KIND>::template L>(p->storage, *p, b, lam, (hipStream_t)stream);
     return rc ? rc : check_launch(name);
 }

@@ -458,25 +458,6 @@ template <class S, int V, bool BUF> static int row_wgs_per_cu(uint64_t workingSe
     return (Codec<S>::storage == VF_STORAGE_FP32 && V == 1 && BUF && streams) ? 2 : 0;
 }
 
-// dynamic LDS bytes that leave room for at most k workgroups of the row kernel on one CU
-static size_t row_lds_reserve(int k)
-{
-    if (k <= 0) return 0;
-    static std::atomic<int> cached[64]; // LDS bytes per CU of each device, 0 until queried
-    int dev = 0, lds = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (dev >= 0 && dev < 64) lds = cached[dev].load(std::memory_order_relaxed);
-    if (lds <= 0) {
-        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || lds <= 0)
-            return 0;
-        if (dev >= 0 && dev < 64) cached[dev].store(lds, std::memory_order_relaxed);
-    }
-    // whole KiB below the share of one workgroup, less 1 KiB for the kernel's own LDS
-    // words and the allocation granularity
-    const long bytes = (long)(lds / k / 1024) * 1024 - 1024;
-    return bytes > 0 ? (size_t)bytes : 0;
-}
-
 // out: null for the in-place step, else the buffer the out-of-place step writes
 template <class S, int V, bool EVEN, bool BUF>
 static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool macro, S* f, S* out, uint64_t ws,
@@ -484,7 +465,7 @@ static void launch_row_b(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
 {
     dim3 grid(a.ny, a.nz);
     dim3 block(((a.nx / V + 63) / 64) * 64); // whole waves; lanes beyond the row idle
-    const size_t lds = row_lds_reserve(row_wgs_per_cu<S, V, BUF>(ws, kc));
+    const size_t lds = lds_reserve_for(row_wgs_per_cu<S, V, BUF>(ws, kc));
     if (out) {
         if (mask) hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, true, false, BUF, true>), grid, block, lds, s, f, out, a, r);
         else hipLaunchKernelGGL((k17_row_kernel<S, V, EVEN, false, false, BUF, true>), grid, block, lds, s, f, out, a, r);

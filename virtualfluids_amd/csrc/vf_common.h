@@ -107,4 +107,8 @@ template <> struct Codec<__hip_bfloat16> {
 void set_last_error(const char* msg);
 int check_launch(const char* what);
 
+// Dynamic LDS bytes a launch reserves (and never touches) so that at most k of its workgroups
+// are resident on one CU of the current device (0 for k <= 0: no cap).
+size_t lds_reserve_for(int k);
+
 } // namespace vfa
