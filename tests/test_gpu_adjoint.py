@@ -35,14 +35,16 @@ def _grid(kind):
     return periodic_box(130, 4, 3)
 
 
-# box: the row kernel (buffer addressing); box_global: its global-address form; box_scalar:
-# the one-node-per-thread kernel; indexed: fluid-node list and neighbour arrays
-# (pinned through vf_lbm_parameter.kernelChoice)
-ADJ_CHOICE = {"box": {}, "box_global": {"rowAddressing": 2}, "box_scalar": {"adjointKernel": 2}, "indexed": {}}
+# box: the automatic choice (the one-node-per-thread kernel for these widths); box_row: the row
+# kernel (buffer addressing) over the whole waves of each row plus the scalar kernel on split-off
+# columns; box_global: the same with 64-bit addresses; box_scalar: the one-node-per-thread kernel;
+# indexed: fluid-node list and neighbour arrays (pinned through vf_lbm_parameter.kernelChoice)
+ADJ_CHOICE = {"box": {}, "box_global": {"rowAddressing": 2, "adjointKernel": 3}, "box_scalar": {"adjointKernel": 2},
+              "box_row": {"adjointKernel": 3}, "indexed": {}}
 
 
 @pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
-@pytest.mark.parametrize("path", ["box", "box_global", "box_scalar", "indexed"])
+@pytest.mark.parametrize("path", ["box", "box_global", "box_scalar", "box_row", "indexed"])
 @pytest.mark.parametrize("even", [True, False])
 @pytest.mark.parametrize("kind", ["periodic", "walled", "long_periodic", "long_walled", "split_walled"])
 def test_adjoint_kernel_matches_reference_adjoint(gpu, orc, storage, path, even, kind):

@@ -54,6 +54,7 @@ struct BoxArgs {
     float* vy;
     float* vz;
     const float* porosity; // Brinkman porosity per node, or null
+    PlaneDiv plane;        // plane-flat numbering of the one-node-per-thread kernels
 };
 
 // Brinkman drag a = -porosity u through the half-step velocity shift: u (1 - porosity/2)
@@ -62,7 +63,7 @@ __device__ __forceinline__ float brinkman_scale(const float* porosity, uint32_t 
     return porosity ? 1.0f - 0.5f * porosity[k] : 1.0f;
 }
 
-constexpr int kBoxBlock = 256;
+constexpr int kBoxBlock = kPlaneBlock; // one-node-per-thread kernels: plane-flat workgroups
 
 // Bounce-back lists applied by the collision itself (vf_post_bounce_back): for a node with an
 // entry, list l bounces back the links of entryLinks[3 e + l] (velocity lists subtract
@@ -114,10 +115,9 @@ __device__ __forceinline__ void bounce_back_epilogue(S* __restrict__ f, uint64_t
 template <class S, bool EVEN, bool MASK, bool MACRO, bool BB = false>
 __global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, BoxArgs a, K17Rates r, FusedBB bb)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int z = blockIdx.z;
-    if (x >= a.nx) return;
+    int x, y;
+    if (!plane_xy(a.plane, x, y)) return;
+    const int z = blockIdx.y;
     const int xp = (x + 1 == a.nx) ? 0 : x + 1;
     const int yp = (y + 1 == a.ny) ? 0 : y + 1;
     const int zp = (z + 1 == a.nz) ? 0 : z + 1;
@@ -149,10 +149,9 @@ template <class S, bool EVEN, bool MASK>
 __global__ __launch_bounds__(kBoxBlock) void k17_box_into_kernel(const S* __restrict__ in, S* __restrict__ out,
                                                                  BoxArgs a, K17Rates r)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
-    const int z = blockIdx.z;
-    if (x >= a.nx) return;
+    int x, y;
+    if (!plane_xy(a.plane, x, y)) return;
+    const int z = blockIdx.y;
     const int xp = (x + 1 == a.nx) ? 0 : x + 1;
     const int yp = (y + 1 == a.ny) ? 0 : y + 1;
     const int zp = (z + 1 == a.nz) ? 0 : z + 1;
@@ -495,11 +494,12 @@ static void launch_row_t(const BoxArgs& a, const K17Rates& r, bool mask, bool ma
 // Nodes per lane of the row kernel the dense-box collision takes for this box, or 0 when it
 // takes the one-node-per-thread kernel: a row of nx nodes is nx/V lanes, at most one workgroup.
 // A working set that stays in the 256 MB infinity cache is latency-bound: the node kernel (no
-// shuffles or barriers, every lane busy whatever nx, more resident waves) serves it best.
-// Measured on MI355X, one lattice stepped alone: 128^3 FP32 67 against 86 us per step, FP16 36
-// against 44; 194 x 98 x 98 FP32 68 against 109; 130^3 FP32 85 against 134; the 66^3 cavity 17
-// against 18. Streaming lattices are the row kernel's (256^3 FP32 620 against 682-736 us). A
-// pinned row kernel or row width keeps the row kernel there.
+// shuffles or barriers, every lane busy, more resident waves) serves it best. Beyond the cache
+// the row kernel leads where its rows are whole waves (256^3 FP32 620 against 682-736 us per
+// step); rows of other lengths leave lanes idle and start inside cache lines, and there the
+// node kernel, whose workgroups take consecutive nodes of a plane, leads (258 x 130 x 130 FP16
+// 87 against 153 us, FP32 203 against 264 us; measured on MI355X, one lattice stepped alone).
+// A pinned row kernel or row width keeps the row kernel.
 template <class S> static int box_row_width(const vf_lbm_parameter& p)
 {
     const vf_kernel_choice& kc = p.kernelChoice;
@@ -510,7 +510,8 @@ template <class S> static int box_row_width(const vf_lbm_parameter& p)
     if (V != 1 && V != 2 && V != 4) return 0;
     const uint64_t ws = p.workingSetBytes ? p.workingSetBytes : 27 * p.numberOfNodes * sizeof(S);
     const bool pinned = kc.boxKernel == VF_BOX_KERNEL_ROW || kc.rowWidth != 0;
-    return (!pinned && ws <= kCacheResidentBytes) ? 0 : V;
+    if (pinned) return V;
+    return (ws <= kCacheResidentBytes || (p.nx / V) % 64 != 0) ? 0 : V;
 }
 
 // The row kernel when box_row_width selects it. Returns false when it does not apply.
@@ -536,13 +537,12 @@ template <class S, bool EVEN>
 static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
 {
     BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, p.rho, p.velocityX, p.velocityY, p.velocityZ,
-              p.porosity};
+              p.porosity, make_plane_div(p.nx, p.ny)};
     const K17Rates r = make_k17_rates(p);
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
     if (try_launch_row<S, EVEN>(p, a, r, macro, nullptr, s)) return;
-    const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
-    dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
+    const dim3 grid(plane_blocks(p.nx, p.ny), p.nz), bx(kBoxBlock);
     const FusedBB none{};
     if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r, none);
     else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r, none);
@@ -557,12 +557,11 @@ static bool launch_box_bb_t(const vf_lbm_parameter& p, bool macro, const FusedBB
 {
     if (box_row_width<S>(p) != 0) return false;
     BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, p.rho, p.velocityX, p.velocityY, p.velocityZ,
-              p.porosity};
+              p.porosity, make_plane_div(p.nx, p.ny)};
     const K17Rates r = make_k17_rates(p);
     S* f = static_cast<S*>(p.distributions);
     const bool mask = p.fluidMask != nullptr;
-    const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
-    dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
+    const dim3 grid(plane_blocks(p.nx, p.ny), p.nz), bx(kBoxBlock);
     if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true, true>), grid, dim3(bx), 0, s, f, a, r, bb);
     else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false, true>), grid, dim3(bx), 0, s, f, a, r, bb);
     else if (macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, true, true>), grid, dim3(bx), 0, s, f, a, r, bb);
@@ -612,11 +611,11 @@ template <class S, bool EVEN> struct BoxL {
 template <class S, bool EVEN> struct BoxIntoL {
     static void run(const vf_lbm_parameter& p, void* out, hipStream_t s)
     {
-        BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, nullptr, nullptr, nullptr, nullptr, p.porosity};
+        BoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, nullptr, nullptr, nullptr, nullptr, p.porosity,
+                  make_plane_div(p.nx, p.ny)};
         const K17Rates r = make_k17_rates(p);
         if (try_launch_row<S, EVEN>(p, a, r, false, static_cast<S*>(out), s)) return;
-        const int bx = p.nx >= kBoxBlock ? kBoxBlock : ((p.nx + 63) / 64) * 64;
-        const dim3 grid((p.nx + bx - 1) / bx, p.ny, p.nz);
+        const dim3 grid(plane_blocks(p.nx, p.ny), p.nz), bx(kBoxBlock);
         const S* in = static_cast<const S*>(p.distributions);
         if (p.fluidMask)
             hipLaunchKernelGGL((k17_box_into_kernel<S, EVEN, true>), grid, dim3(bx), 0, s, in, (S*)out, a, r);

@@ -7,6 +7,8 @@
 
 namespace vfa {
 
+#define VF_PLANE_HD __host__ __device__ __forceinline__
+
 struct Addr {
     uint64_t N;
     const uint32_t* nbx; // +x/+y/+z neighbour arrays (sparse grids)
@@ -42,6 +44,38 @@ __device__ __forceinline__ void cell_nodes(const Addr& a, uint32_t k, uint32_t (
         n[0] = r00 + x; n[1] = r00 + xp; n[2] = r10 + x; n[3] = r10 + xp;
         n[4] = r01 + x; n[5] = r01 + xp; n[6] = r11 + x; n[7] = r11 + xp;
     }
+}
+
+// Plane-flat numbering of the one-node-per-thread box kernels: a launch of (plane_blocks, nz)
+// workgroups of kPlaneBlock threads gives thread t of plane z the node t of the plane's nx * ny
+// nodes, so every lane has a node whatever nx is. y = t / nx comes from a double reciprocal:
+// v = (t + 1/2) / nx lies at least 1/(2 nx) from an integer, and the two roundings (of 1/nx and
+// of the product) err by at most v 2^-52 < 1/(2 nx) for every t < 2^51, so the truncation is
+// exact.
+constexpr int kPlaneBlock = 256;
+struct PlaneDiv {
+    uint32_t nx;
+    uint32_t planeNodes; // nx * ny
+    double invNx;        // 1 / nx
+};
+inline PlaneDiv make_plane_div(int nx, int ny)
+{
+    return PlaneDiv{(uint32_t)nx, (uint32_t)nx * (uint32_t)ny, 1.0 / (double)nx};
+}
+inline unsigned plane_blocks(int nx, int ny)
+{
+    return (unsigned)(((uint64_t)nx * (uint64_t)ny + kPlaneBlock - 1) / kPlaneBlock);
+}
+VF_PLANE_HD uint32_t plane_row(uint32_t t, const PlaneDiv& p) { return (uint32_t)(((double)t + 0.5) * p.invNx); }
+// (x, y) of this thread's node in its plane (blockIdx.y); false for threads beyond the plane
+__device__ __forceinline__ bool plane_xy(const PlaneDiv& p, int& x, int& y)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.planeNodes) return false;
+    const uint32_t r = plane_row(t, p);
+    y = (int)r;
+    x = (int)(t - r * p.nx);
+    return true;
 }
 
 template <class S> __device__ __forceinline__ S* slot_ptr(S* f, uint64_t N, int d, bool even)

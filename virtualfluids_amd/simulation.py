@@ -91,39 +91,34 @@ class BoundaryConditionFactory:
         return self._resolve(table, self.geometry)
 
 
-class FusedBounceBack:
-    """The bounce-back BC lists of a dense-box lattice in the form vf_k17_run_box_bounce_back
-    applies them inside the collision launch: a per-node entry, and per entry and list the node's
-    index and its bounce-back links. lists: (name, velocity) pairs in launch order, velocity True
-    for VelocityBounceBack and False for NoSlipBounceBack. Raises ValueError where the lists do
-    not allow it: a link that leads to a node the collision updates (the epilogue would race
-    with that node's own writes), or a lattice without a fluid mask."""
+class BCEntryTables:
+    """Per-node tables of post-collision BC lists for kernels that apply the lists themselves
+    (the fused bounce-back collision, the fused BC adjoint): per node its entry or -1, and per entry
+    and list the node's index in the list (or -1) and the bitmask of its links d with
+    0 <= q_d <= 1. names: the lists in launch order (at most three). Raises ValueError where a link
+    leads to a node the collision updates (a fused kernel would race with that node's own
+    accesses), or on a lattice without a fluid mask."""
 
-    def __init__(self, lattice: Lattice, lists):
-        import ctypes
-
+    def __init__(self, lattice: Lattice, names, what: str = "fused BC"):
         import numpy as np
         import torch
 
-        from . import capi
         from .grid import CX, CY, CZ
 
         if not lattice.box or lattice.mask is None:
-            raise ValueError("fused bounce-back needs a dense box with solid nodes")
-        if len(lists) > 3:
-            raise ValueError("at most three bounce-back lists")
+            raise ValueError(f"{what} needs a dense box with solid nodes")
+        if len(names) > 3:
+            raise ValueError(f"{what}: at most three lists")
         g = lattice.grid
         fluid = g.fluid_mask
-        used = [(name, bool(vel)) for name, vel in lists
-                if lattice.bcs.get(name) is not None and lattice.bcs[name].n > 0]
-        all_k = np.unique(np.concatenate([g.bcs[name].k.astype(np.int64) for name, _ in used])) if used else \
+        all_k = np.unique(np.concatenate([g.bcs[name].k.astype(np.int64) for name in names])) if names else \
             np.zeros(0, np.int64)
         entry = np.full(g.N, -1, dtype=np.int32)
         entry[all_k] = np.arange(all_k.size, dtype=np.int32)
         index = np.full((all_k.size, 3), -1, dtype=np.int32)
         links = np.zeros((all_k.size, 3), dtype=np.uint32)
         x, y, z = g.coords()
-        for l, (name, _) in enumerate(used):
+        for l, name in enumerate(names):
             bc = g.bcs[name]
             k = bc.k.astype(np.int64)
             q = np.asarray(bc.q27).reshape(27, bc.n)
@@ -136,20 +131,44 @@ class FusedBounceBack:
                 kk = k[act]
                 m = g.index((x[kk] + CX[d]) % g.nx, (y[kk] + CY[d]) % g.ny, (z[kk] + CZ[d]) % g.nz)
                 if np.any(fluid[m]):
-                    raise ValueError(f"a bounce-back link of list '{name}' leads to a fluid node")
+                    raise ValueError(f"a {what} link of list '{name}' leads to a fluid node")
                 links[e[act], l] |= np.uint32(1 << d)
+        self.nodes = all_k
+        self.count = int(all_k.size)
         dev = lattice.device
         self.entry = torch.from_numpy(entry).to(dev)
         self.index = torch.from_numpy(index.reshape(-1)).to(dev)
         self.links = torch.from_numpy(links.reshape(-1).view(np.int32)).to(dev)
+
+    def pointers(self):
+        import ctypes
+
+        return (ctypes.c_void_p(self.entry.data_ptr()), ctypes.c_void_p(self.index.data_ptr()),
+                ctypes.c_void_p(self.links.data_ptr()))
+
+
+class FusedBounceBack:
+    """The bounce-back BC lists of a dense-box lattice in the form vf_k17_run_box_bounce_back
+    applies them inside the collision launch: a per-node entry, and per entry and list the node's
+    index and its bounce-back links. lists: (name, velocity) pairs in launch order, velocity True
+    for VelocityBounceBack and False for NoSlipBounceBack. Raises ValueError where the lists do
+    not allow it: a link that leads to a node the collision updates (the epilogue would race
+    with that node's own writes), or a lattice without a fluid mask."""
+
+    def __init__(self, lattice: Lattice, lists):
+        import ctypes
+
+        from . import capi
+
+        used = [(name, bool(vel)) for name, vel in lists
+                if lattice.bcs.get(name) is not None and lattice.bcs[name].n > 0]
+        self.tables = BCEntryTables(lattice, [name for name, _ in used], "bounce-back")
         st = capi.PostBounceBack()
         for l, (name, vel) in enumerate(used):
             st.lists[l] = ctypes.pointer(lattice.bcs[name].struct)
             st.velocity[l] = 1 if vel else 0
-        st.nodeEntry = ctypes.c_void_p(self.entry.data_ptr())
-        st.entryIndex = ctypes.c_void_p(self.index.data_ptr())
-        st.entryLinks = ctypes.c_void_p(self.links.data_ptr())
-        st.numberOfEntries = int(all_k.size)
+        st.nodeEntry, st.entryIndex, st.entryLinks = self.tables.pointers()
+        st.numberOfEntries = self.tables.count
         self.struct = st
         self.lattice = lattice
 
