@@ -293,42 +293,6 @@ int vf_PressureNonEquilibriumCompressible(const vf_lbm_parameter* para, const vf
 int vf_k17_adjoint_box(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint,
                        float* porosityAdjoint, void* stream);
 
-/* Kinds of post-collision BC lists, as the vf_<BC>Adjoint entry points below name them. */
-enum vf_post_bc_kind {
-    VF_POST_BC_NOSLIP_BOUNCE_BACK = 0,
-    VF_POST_BC_VELOCITY_BOUNCE_BACK = 1,
-    VF_POST_BC_NOSLIP_INTERPOLATED = 2,
-    VF_POST_BC_VELOCITY_INTERPOLATED = 3,
-    VF_POST_BC_VELOCITY_WITH_PRESSURE_INTERPOLATED = 4
-};
-
-/* Post-collision BC lists whose adjoints vf_k17_adjoint_box_bc applies in the same launch as the
- * collision adjoint. lists[l]: the lists in the order their forward kernels run (NULL where
- * unused), each with postCollision set (the values its forward kernel stored for this step);
- * kind[l]: enum vf_post_bc_kind; velocityAdjoint[l]: optional, 3*numberOfBCnodes floats (Vx, Vy, Vz
- * blocks) of a velocity list, ADDED to. nodeEntry / entryIndex / entryLinks as in
- * vf_post_bounce_back (per node -1 or its entry; per entry and list the node's index in the list
- * or -1, and the bitmask of its links d with 0 <= q_d <= 1). */
-typedef struct vf_post_bc_adjoint {
-    const vf_q_boundary_conditions* lists[3]; /* host pointers */
-    int32_t kind[3];
-    float* velocityAdjoint[3];
-    const int32_t* nodeEntry;
-    const int32_t* entryIndex;
-    const uint32_t* entryLinks;
-    uint32_t numberOfEntries;
-} vf_post_bc_adjoint;
-
-/* The adjoints of bc's lists in reverse launch order followed by vf_k17_adjoint_box, in one launch
- * (the same results up to FP32 rounding order): each BC node adds its lists' adjoints to the
- * adjoint of its post-collision values before the collision's own reverse, and the cells its
- * links wrote are read and cleared there, so the BC adjoint launches and their passes over the
- * adjoint buffer disappear. Needs para->fluidMask; every listed node must collide and every link
- * must lead to a node that does not (as at walls and obstacles of solid nodes). No counterpart in
- * the reference. */
-int vf_k17_adjoint_box_bc(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint,
-                          float* porosityAdjoint, const vf_post_bc_adjoint* bc, void* stream);
-
 /* Reverse of vf_k17_run (every node of fluidNodeIndices, neighbour arrays). */
 int vf_k17_adjoint(const vf_lbm_parameter* para, float* adjoint, float* velocityShiftAdjoint, float* porosityAdjoint,
                    void* stream);

@@ -68,29 +68,6 @@ extern "C" void host_bc_post_vjp(int kind, const float* post, const float* q, fl
     std::memcpy(Ub, ub, sizeof(ub));
 }
 
-// the streaming form: g (27) in/out, G indexed by link d (the adjoint written for link d)
-extern "C" void host_bc_post_vjp_add(int kind, const float* post, const float* q, float omega, const float* G,
-                                     float* g, float* Ub)
-{
-    uint32_t links = 0;
-    for (int d = 1; d < 27; ++d)
-        if (q[d] >= 0.0f && q[d] <= 1.0f) links |= 1u << d;
-    float gg[27], ub[3];
-    std::memcpy(gg, g, sizeof(gg));
-    auto P = [&](int e) { return post[e]; };
-    auto Q = [&](int d) { return q[d]; };
-    auto W = [&](int d) { return G[inv(d)]; };
-    switch (kind) {
-    case kNoSlipBB: bc_post_vjp_add<kNoSlipBB>(P, Q, W, links, omega, gg, ub); break;
-    case kVelocityBB: bc_post_vjp_add<kVelocityBB>(P, Q, W, links, omega, gg, ub); break;
-    case kNoSlipInterp: bc_post_vjp_add<kNoSlipInterp>(P, Q, W, links, omega, gg, ub); break;
-    case kVelocityInterp: bc_post_vjp_add<kVelocityInterp>(P, Q, W, links, omega, gg, ub); break;
-    default: bc_post_vjp_add<kVelocityPressureInterp>(P, Q, W, links, omega, gg, ub); break;
-    }
-    std::memcpy(g, gg, sizeof(gg));
-    std::memcpy(Ub, ub, sizeof(ub));
-}
-
 extern "C" void host_pressure_vjp(const float* fn, float rhoBC, const float* G, float* fnb, float* rb)
 {
     float f[27], g[27], out[27];

@@ -84,59 +84,13 @@ def _storage_input(t: torch.Tensor, L: Lattice, what: str) -> torch.Tensor:
     return t.detach().to(L.f.dtype, copy=True)
 
 
-# the post-collision BC kernels the fused BC adjoint knows (enum vf_post_bc_kind)
-_POST_BC_KIND = {"vf_NoSlipBounceBack": 0, "vf_VelocityBounceBack": 1, "vf_NoSlipInterpolatedCompressible": 2,
-                 "vf_VelocityInterpolatedCompressible": 3, "vf_VelocityWithPressureInterpolatedCompressible": 4}
-
-
-class FusedBCAdjoint:
-    """The post-collision BC lists of a step in the form vf_k17_adjoint_box_bc applies their
-    adjoints: inside the collision adjoint's launch, from the post-collision values the replay
-    stored, instead of one launch per list. lists: (name, C function) in launch order. Raises
-    ValueError where that does not apply (no fluid mask, a link into a node that collides, a BC
-    node that does not collide, an unknown BC function)."""
-
-    def __init__(self, L: Lattice, lists):
-        from .simulation import BCEntryTables
-
-        for _, fn in lists:
-            if fn not in _POST_BC_KIND:
-                raise ValueError(f"no fused adjoint for {fn}")
-        self.L = L
-        self.lists = list(lists)
-        self.tables = BCEntryTables(L, [name for name, _ in lists], "fused BC adjoint")
-        if self.tables.count and not np.all(L.grid.fluid_mask[self.tables.nodes]):
-            raise ValueError("fused BC adjoint: every BC node must collide")
-
-    def run(self, p, lam, shift, padj, saved, ubar, stream):
-        """vf_k17_adjoint_box_bc with level parameters p, adjoint lam, the stored post-collision
-        values `saved` ({list name: tensor}) and the velocity list's wall-velocity adjoint ubar."""
-        st = capi.PostBCAdjoint()
-        keep = []
-        for l, (name, fn) in enumerate(self.lists):
-            q = _bc_struct(self.L.bcs[name], saved[name])
-            keep.append(q)
-            st.lists[l] = ctypes.pointer(q)
-            st.kind[l] = _POST_BC_KIND[fn]
-            st.velocityAdjoint[l] = _vp(ubar) if name == "velocity" else None
-        st.nodeEntry, st.entryIndex, st.entryLinks = self.tables.pointers()
-        st.numberOfEntries = self.tables.count
-        capi.call("vf_k17_adjoint_box_bc", ctypes.byref(p), _vp(lam), _vp(shift), _vp(padj), ctypes.byref(st),
-                  stream_handle(stream))
-
-
 class Stepper:
     """One time step of a Lattice on any buffer, and its adjoint, with the BC functions a
-    BoundaryConditionFactory selects (the same selection and checks as Simulation). On a dense box
-    with solid walls the post-collision BC adjoints run inside the collision adjoint's launch
-    (FusedBCAdjoint) whenever the replay stored the post-collision values; fuse_adjoint=False keeps
-    one launch per list."""
+    BoundaryConditionFactory selects (the same selection and checks as Simulation)."""
 
-    def __init__(self, L: Lattice, bc_factory: BoundaryConditionFactory | None = None, fuse_adjoint: bool = True):
+    def __init__(self, L: Lattice, bc_factory: BoundaryConditionFactory | None = None):
         f = bc_factory or BoundaryConditionFactory()
         self.L = L
-        self.fuse_adjoint = fuse_adjoint
-        self._fused = None  # FusedBCAdjoint, built at the first adjoint (False: does not apply)
         self.velocity = f.get_velocity_boundary_condition_post()
         self.noslip = f.get_no_slip_boundary_condition_post()
         self.pressure = f.get_pressure_boundary_condition_pre()
@@ -150,21 +104,6 @@ class Stepper:
     def _bc(self, name):
         bc = self.L.bcs.get(name)
         return bc if bc is not None and bc.n > 0 else None
-
-    def _fused_adjoint(self):
-        """The FusedBCAdjoint of this step's post BC lists, or None where it does not apply."""
-        if self._fused is None:
-            self._fused = False
-            lists = []
-            for name, fn in (("velocity", self.velocity), ("noslip", self.noslip), ("geometry", self.geometry)):
-                if fn and self._bc(name) is not None:
-                    lists.append((name, fn))
-            if self.fuse_adjoint and self.L.box and lists:
-                try:
-                    self._fused = FusedBCAdjoint(self.L, lists)
-                except ValueError:
-                    self._fused = False
-        return self._fused or None
 
     def post_lists(self):
         """(name, node count) of the post-collision BC lists a step applies, in launch order."""
@@ -219,10 +158,6 @@ class Stepper:
             capi.call(self.pressure + "Adjoint", ctypes.byref(_param(L, after, not even)), ctypes.byref(bc.struct),
                       _vp(lam), _vp(rbar), s)
         p = _param(L, before, even)
-        fused = self._fused_adjoint() if saved else None
-        if fused is not None:
-            fused.run(p, lam, None, padj, saved, ubar, stream)
-            return
         for name, fn, grad in (("geometry", self.geometry, None), ("noslip", self.noslip, None),
                                ("velocity", self.velocity, ubar)):
             bc = self._bc(name)

@@ -118,20 +118,6 @@ class PostBounceBack(ctypes.Structure):
     ]
 
 
-class PostBCAdjoint(ctypes.Structure):
-    """vf_post_bc_adjoint: post-collision BC lists whose adjoints the collision adjoint applies."""
-
-    _fields_ = [
-        ("lists", ctypes.POINTER(QBoundaryConditions) * 3),
-        ("kind", ctypes.c_int32 * 3),
-        ("velocityAdjoint", _vp * 3),
-        ("nodeEntry", _vp),
-        ("entryIndex", _vp),
-        ("entryLinks", _vp),
-        ("numberOfEntries", ctypes.c_uint32),
-    ]
-
-
 class InterpolationCells(ctypes.Structure):
     _fields_ = [("fineCellIndices", _vp), ("coarseCellIndices", _vp), ("numberOfCells", ctypes.c_uint32)]
 
@@ -174,7 +160,6 @@ SIGNATURES = {
     "vf_KineticEnergy": [_P, _vp, _vp, _vp],
     "vf_ScaleCoarseToFineCompressible": [_P, _P, _vp, _vp, _vp],
     "vf_k17_adjoint_box": [_P, _vp, _vp, _vp, _vp],
-    "vf_k17_adjoint_box_bc": [_P, _vp, _vp, _vp, ctypes.POINTER(PostBCAdjoint), _vp],
     "vf_k17_adjoint": [_P, _vp, _vp, _vp, _vp],
     "vf_NoSlipBounceBackAdjoint": [_P, _Q, _vp, _vp, _vp],
     "vf_NoSlipInterpolatedCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],

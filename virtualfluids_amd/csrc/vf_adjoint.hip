@@ -14,7 +14,6 @@
 #include <cstdlib>
 #include <type_traits>
 
-#include "vf_bc_adjoint.h"
 #include "vf_k17_adjoint.h"
 #include "vf_lattice.h"
 #include "vf_rowio.h"
@@ -80,71 +79,9 @@ __device__ __forceinline__ void brinkman_adjoint(const float* porosity, float* p
     if (porosity && porosityAdj) porosityAdj[k] += -0.5f * sb;
 }
 
-// Post-collision BC lists whose adjoints the collision adjoint applies itself
-// (vf_k17_adjoint_box_bc): per node an entry (or -1), per entry and list the node's index in the
-// list and its active links; per list the kind, the link distances, the post-collision values the
-// forward BC kernel stored, and the wall-velocity adjoint (or null).
-struct FusedBcAdj {
-    const int32_t* nodeEntry;
-    const int32_t* entryIndex;
-    const uint32_t* entryLinks;
-    const float* q27[3];
-    const float* post[3];
-    float* ubar[3];
-    uint32_t n[3];
-    int kind[3];
-    float omega;
-};
-
-// The BC adjoints of a node's lists in reverse launch order, added to g (the adjoint of the
-// node's post-collision values) before the collision's own reverse: each reads the adjoint of
-// the cells its links wrote (wcell(d): the cell written for link d) and clears it. Every written
-// cell belongs to a node the collision does not update, so no other thread touches it.
-template <int KIND, class WCELL>
-__device__ __forceinline__ void bc_list_adjoint(const FusedBcAdj& b, int l, uint32_t links, int32_t i, WCELL wcell,
-                                                float (&g)[27])
-{
-    const uint32_t n = b.n[l];
-    const float* post = b.post[l];
-    const float* q27 = b.q27[l];
-    float ub[3];
-    bc_post_vjp_add<KIND>([&](int e) { return post[(uint64_t)e * n + i]; },
-                          [&](int d) { return q27[(uint64_t)d * n + i]; },
-                          [&](int d) {
-                              float* w = wcell(d);
-                              const float v = *w;
-                              *w = 0.0f;
-                              return v;
-                          },
-                          links, b.omega, g, ub);
-    if (b.ubar[l]) {
-        b.ubar[l][i] += ub[0];
-        b.ubar[l][n + i] += ub[1];
-        b.ubar[l][2 * n + i] += ub[2];
-    }
-}
-
-template <class WCELL>
-__device__ __forceinline__ void bc_adjoint_prologue(const FusedBcAdj& b, int32_t e, WCELL wcell, float (&g)[27])
-{
-#pragma unroll 1
-    for (int l = 2; l >= 0; --l) {
-        const uint32_t links = b.entryLinks[3 * e + l];
-        if (!links) continue;
-        const int32_t i = b.entryIndex[3 * e + l];
-        switch (b.kind[l]) {
-        case kNoSlipBB: bc_list_adjoint<kNoSlipBB>(b, l, links, i, wcell, g); break;
-        case kVelocityBB: bc_list_adjoint<kVelocityBB>(b, l, links, i, wcell, g); break;
-        case kNoSlipInterp: bc_list_adjoint<kNoSlipInterp>(b, l, links, i, wcell, g); break;
-        case kVelocityInterp: bc_list_adjoint<kVelocityInterp>(b, l, links, i, wcell, g); break;
-        default: bc_list_adjoint<kVelocityPressureInterp>(b, l, links, i, wcell, g); break;
-        }
-    }
-}
-
-template <class S, bool EVEN, bool MASK, bool SHIFT, bool BC = false>
+template <class S, bool EVEN, bool MASK, bool SHIFT>
 __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __restrict__ f, float* __restrict__ lam,
-                                                                    AdjBoxArgs a, K17Rates r, FusedBcAdj bc)
+                                                                    AdjBoxArgs a, K17Rates r)
 {
     int x, y, z;
     if (!adj_coords(a, x, y, z)) return;
@@ -159,12 +96,6 @@ __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __r
     if (MASK && a.mask[n[0]] == 0) return;
     float v[27], g[27], fb[27], ub[3], sb = 0.0f;
     adj_gather<S, EVEN>(f, lam, a.N, n, v, g);
-    if constexpr (BC) {
-        const int32_t e = bc.nodeEntry[n[0]];
-        if (e >= 0)
-            bc_adjoint_prologue(bc, e, [&](int d) { return lam + (uint64_t)slot<EVEN>(d) * a.N + n[node_sel(inv(d))]; },
-                                g);
-    }
     const float vs = a.porosity ? 1.0f - 0.5f * a.porosity[n[0]] : 1.0f;
     k17_vjp(v, g, r, fb, SHIFT ? ub : nullptr, vs, a.porosity ? &sb : nullptr);
     adj_scatter<EVEN>(lam, a.N, n, fb);
@@ -184,9 +115,9 @@ __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __r
 // LDS. Lanes beyond the row idle. Solid nodes write their adjoint cells back unchanged.
 constexpr int kAdjRowMaxThreads = 512;
 
-template <class S, bool EVEN, bool MASK, bool SHIFT, bool BUF, bool BC = false>
+template <class S, bool EVEN, bool MASK, bool SHIFT, bool BUF>
 __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(const S* f, float* lam, AdjBoxArgs a,
-                                                                           K17Rates r, FusedBcAdj bc)
+                                                                           K17Rates r)
 {
     using IO = RowIO<S, 1>;
     using LO = RowIO<float, 1>;
@@ -310,16 +241,6 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
     const uint32_t k = rows[0];
     bool fluid = active;
     if (MASK && active) fluid = a.mask[k] != 0;
-    if constexpr (BC) {
-        const int32_t e = active ? bc.nodeEntry[k] : -1;
-        if (e >= 0) {
-            const uint32_t xp = (x + 1 == nx) ? 0 : x + 1;
-            bc_adjoint_prologue(bc, e, [&](int d) {
-                const int sel = node_sel(inv(d));
-                return lam + (uint64_t)slot<EVEN>(d) * a.N + (rows[sel >> 1] - x + ((sel & 1) ? xp : x));
-            }, g);
-        }
-    }
     if (fluid) {
         float ub[3], sb = 0.0f;
         const float vs = a.porosity ? 1.0f - 0.5f * a.porosity[k] : 1.0f;
@@ -336,9 +257,8 @@ __global__ __launch_bounds__(kAdjRowMaxThreads) void k17_adjoint_row_kernel(cons
         for (int d = 0; d < 27; ++d) fb[d] = g[inv(d)];
     }
 
-    // A solid node's cells keep their adjoint: their stores are skipped (fused BC adjoints clear
-    // cells of solid nodes, which a write-back of the loaded values would undo).
-    // cells without an x-shift are this lane's own
+    // A solid node's cells keep their adjoint (the forward step leaves them alone): their stores
+    // are skipped. Cells without an x-shift are this lane's own
     if (active && fluid) {
 #pragma unroll
         for (int d = 0; d < 27; ++d) {
@@ -445,37 +365,29 @@ int check_adjoint_args(const vf_lbm_parameter* p, const float* adjoint)
     return VF_OK;
 }
 
-// The fused BC adjoints (bc non-null) need a fluid mask; the launchers instantiate them with it.
 template <class S, bool EVEN, bool BUF>
-void launch_row_b(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift,
-                  const FusedBcAdj* bc, hipStream_t s)
+void launch_row_b(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift, hipStream_t s)
 {
     const dim3 grid(a.ny, a.nz), block(((a.rowEnd + 63) / 64) * 64);
-    const FusedBcAdj none{};
-    if (bc && shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, true, BUF, true>), grid, block, 0, s, f, lam, a, r, *bc);
-    else if (bc) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, false, BUF, true>), grid, block, 0, s, f, lam, a, r, *bc);
-    else if (mask && shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, true, BUF>), grid, block, 0, s, f, lam, a, r, none);
-    else if (mask) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, false, BUF>), grid, block, 0, s, f, lam, a, r, none);
-    else if (shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, false, true, BUF>), grid, block, 0, s, f, lam, a, r, none);
-    else hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, false, false, BUF>), grid, block, 0, s, f, lam, a, r, none);
+    if (mask && shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, true, BUF>), grid, block, 0, s, f, lam, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, true, false, BUF>), grid, block, 0, s, f, lam, a, r);
+    else if (shift) hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, false, true, BUF>), grid, block, 0, s, f, lam, a, r);
+    else hipLaunchKernelGGL((k17_adjoint_row_kernel<S, EVEN, false, false, BUF>), grid, block, 0, s, f, lam, a, r);
 }
 
 template <class S, bool EVEN>
-void launch_scalar(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift,
-                   const FusedBcAdj* bc, dim3 grid, hipStream_t s)
+void launch_scalar(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift, dim3 grid,
+                   hipStream_t s)
 {
     const dim3 block(kAdjBlock);
-    const FusedBcAdj none{};
-    if (bc && shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, true, true>), grid, block, 0, s, f, lam, a, r, *bc);
-    else if (bc) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, false, true>), grid, block, 0, s, f, lam, a, r, *bc);
-    else if (mask && shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, true>), grid, block, 0, s, f, lam, a, r, none);
-    else if (mask) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, false>), grid, block, 0, s, f, lam, a, r, none);
-    else if (shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, true>), grid, block, 0, s, f, lam, a, r, none);
-    else hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, false>), grid, block, 0, s, f, lam, a, r, none);
+    if (mask && shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, true>), grid, block, 0, s, f, lam, a, r);
+    else if (mask) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, false>), grid, block, 0, s, f, lam, a, r);
+    else if (shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, true>), grid, block, 0, s, f, lam, a, r);
+    else hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, false>), grid, block, 0, s, f, lam, a, r);
 }
 
 template <class S, bool EVEN>
-void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj, const FusedBcAdj* bc, hipStream_t s)
+void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj, hipStream_t s)
 {
     AdjBoxArgs a{p.numberOfNodes, p.nx, p.ny, p.nz, p.fluidMask, shift, p.porosity, padj};
     a.plane = make_plane_div(p.nx, p.ny);
@@ -503,19 +415,19 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj
         AdjBoxArgs ar = a;
         ar.rowEnd = rowEnd;
         if ((uint64_t)kRowSplit * a.N * sizeof(float) <= 0xffffffffull && kc.rowAddressing != VF_ROW_ADDRESSING_GLOBAL)
-            launch_row_b<S, EVEN, true>(ar, r, f, lam, mask, shift != nullptr, bc, s);
+            launch_row_b<S, EVEN, true>(ar, r, f, lam, mask, shift != nullptr, s);
         else
-            launch_row_b<S, EVEN, false>(ar, r, f, lam, mask, shift != nullptr, bc, s);
+            launch_row_b<S, EVEN, false>(ar, r, f, lam, mask, shift != nullptr, s);
         if (rowEnd == (uint32_t)p.nx) return;
         // the tail columns, one thread per node over all rows
         AdjBoxArgs at = a;
         at.x0 = rowEnd;
         const uint64_t n = (uint64_t)(p.nx - rowEnd) * p.ny * p.nz;
-        launch_scalar<S, EVEN>(at, r, f, lam, mask, shift != nullptr, bc, dim3((unsigned)((n + kAdjBlock - 1) / kAdjBlock)), s);
+        launch_scalar<S, EVEN>(at, r, f, lam, mask, shift != nullptr, dim3((unsigned)((n + kAdjBlock - 1) / kAdjBlock)), s);
         return;
     }
     static_assert(kAdjBlock == kPlaneBlock, "the scalar adjoint kernel takes plane-flat workgroups");
-    launch_scalar<S, EVEN>(a, r, f, lam, mask, shift != nullptr, bc, dim3(plane_blocks(p.nx, p.ny), p.nz), s);
+    launch_scalar<S, EVEN>(a, r, f, lam, mask, shift != nullptr, dim3(plane_blocks(p.nx, p.ny), p.nz), s);
 }
 
 template <class S, bool EVEN>
@@ -559,70 +471,9 @@ extern "C" int vf_k17_adjoint_box(const vf_lbm_parameter* p, float* adjoint, flo
     hipStream_t s = (hipStream_t)stream;
     dispatch(*p, [&](auto tag, auto even) {
         using S = std::remove_pointer_t<decltype(tag)>;
-        launch_box<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, porosityAdjoint, nullptr, s);
+        launch_box<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, porosityAdjoint, s);
     });
     return check_launch("k17 adjoint box kernel execution failed");
-}
-
-static_assert(VF_POST_BC_NOSLIP_BOUNCE_BACK == kNoSlipBB && VF_POST_BC_VELOCITY_BOUNCE_BACK == kVelocityBB &&
-                  VF_POST_BC_NOSLIP_INTERPOLATED == kNoSlipInterp && VF_POST_BC_VELOCITY_INTERPOLATED == kVelocityInterp &&
-                  VF_POST_BC_VELOCITY_WITH_PRESSURE_INTERPOLATED == kVelocityPressureInterp,
-              "the C ABI's BC kinds are the kernels' BcKind values");
-
-extern "C" int vf_k17_adjoint_box_bc(const vf_lbm_parameter* p, float* adjoint, float* velocityShiftAdjoint,
-                                     float* porosityAdjoint, const vf_post_bc_adjoint* bc, void* stream)
-{
-    if (!bc) {
-        set_last_error("adjoint box bc: null BC lists");
-        return VF_ERR_INVALID_ARGUMENT;
-    }
-    if (bc->numberOfEntries == 0) return vf_k17_adjoint_box(p, adjoint, velocityShiftAdjoint, porosityAdjoint, stream);
-    const int st = check_adjoint_args(p, adjoint);
-    if (st != VF_OK) return st;
-    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 || (uint64_t)p->nx * p->ny * p->nz != p->numberOfNodes ||
-        p->ny > 65535 || p->nz > 65535) {
-        set_last_error("adjoint box: nx*ny*nz must equal numberOfNodes (ny, nz <= 65535)");
-        return VF_ERR_INVALID_ARGUMENT;
-    }
-    if (!p->fluidMask || !bc->nodeEntry || !bc->entryIndex || !bc->entryLinks) {
-        set_last_error("adjoint box bc: needs fluidMask, nodeEntry, entryIndex and entryLinks");
-        return VF_ERR_INVALID_ARGUMENT;
-    }
-    FusedBcAdj fb{};
-    fb.nodeEntry = bc->nodeEntry;
-    fb.entryIndex = bc->entryIndex;
-    fb.entryLinks = bc->entryLinks;
-    fb.omega = p->omega;
-    for (int l = 0; l < 3; ++l) {
-        const vf_q_boundary_conditions* q = bc->lists[l];
-        if (!q || q->numberOfBCnodes == 0) continue;
-        if (bc->kind[l] < VF_POST_BC_NOSLIP_BOUNCE_BACK || bc->kind[l] > VF_POST_BC_VELOCITY_WITH_PRESSURE_INTERPOLATED) {
-            set_last_error("adjoint box bc: unknown BC kind");
-            return VF_ERR_INVALID_ARGUMENT;
-        }
-        if (!q->q27 || !q->postCollision) {
-            set_last_error("adjoint box bc: every list needs q27 and postCollision");
-            return VF_ERR_INVALID_ARGUMENT;
-        }
-        if (bc->kind[l] >= VF_POST_BC_NOSLIP_INTERPOLATED && p->omega == 1.0f) {
-            set_last_error("adjoint interpolated BC: omega must differ from 1");
-            return VF_ERR_INVALID_ARGUMENT;
-        }
-        fb.q27[l] = q->q27;
-        fb.post[l] = q->postCollision;
-        fb.n[l] = q->numberOfBCnodes;
-        fb.kind[l] = bc->kind[l];
-        const bool velocity = bc->kind[l] == VF_POST_BC_VELOCITY_BOUNCE_BACK ||
-                              bc->kind[l] == VF_POST_BC_VELOCITY_INTERPOLATED ||
-                              bc->kind[l] == VF_POST_BC_VELOCITY_WITH_PRESSURE_INTERPOLATED;
-        fb.ubar[l] = velocity ? bc->velocityAdjoint[l] : nullptr;
-    }
-    hipStream_t s = (hipStream_t)stream;
-    dispatch(*p, [&](auto tag, auto even) {
-        using S = std::remove_pointer_t<decltype(tag)>;
-        launch_box<S, decltype(even)::value>(*p, adjoint, velocityShiftAdjoint, porosityAdjoint, &fb, s);
-    });
-    return check_launch("k17 adjoint box kernel with BC adjoints execution failed");
 }
 
 extern "C" int vf_k17_adjoint(const vf_lbm_parameter* p, float* adjoint, float* velocityShiftAdjoint,

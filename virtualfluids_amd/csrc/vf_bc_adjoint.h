@@ -80,70 +80,6 @@ VF_HD void bc_post_vjp(const float (&post)[27], const float (&q)[27], float omeg
     }
 }
 
-// bc_post_vjp in a streaming form for kernels that fuse the BC adjoint into the collision
-// adjoint: instead of returning fb it adds it to g (the adjoint of the node's post-collision
-// values), and it reads the post-collision values, link distances and written-cell adjoints on
-// demand, so that few registers stay live. post(e): post-collision value e; qd(d): link distance of
-// an active link d; G(d): adjoint of the value written for link d; links: bitmask of the active
-// links (0 <= q_d <= 1).
-template <int KIND, class POST, class QD, class GD>
-VF_HD void bc_post_vjp_add(POST post, QD qd, GD G, uint32_t links, float omega, float (&g)[27], float (&Ub)[3])
-{
-    constexpr bool interp = KIND >= kNoSlipInterp;
-    constexpr bool velocity = KIND == kVelocityBB || KIND == kVelocityInterp || KIND == kVelocityPressureInterp;
-    float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
-    if (interp) {
-#pragma unroll
-        for (int d = 0; d < 27; ++d) {
-            const float p = post(d);
-            drho += p;
-            if (cx(d)) jx += float(cx(d)) * p;
-            if (cy(d)) jy += float(cy(d)) * p;
-            if (cz(d)) jz += float(cz(d)) * p;
-        }
-    }
-    const float ir = 1.0f / (1.0f + drho);
-    const float j2 = jx * jx + jy * jy + jz * jz;
-    const float om1 = 1.0f / (1.0f - omega);
-    Ub[0] = Ub[1] = Ub[2] = 0.0f;
-    float drhob = 0.0f, jxb = 0.0f, jyb = 0.0f, jzb = 0.0f;
-#pragma unroll
-    for (int d = 1; d < 27; ++d) {
-        if (!(links >> d & 1u)) continue;
-        const int r = inv(d);
-        const float w = weight(d);
-        const float gd = G(d);
-        float wallFactor = 6.0f * w;
-        if (!interp) {
-            g[d] += gd;
-        } else {
-            const float q = qd(d);
-            const float A = 0.5f * (1.0f - q) / (1.0f + q), Bq = 1.0f / (1.0f + q);
-            g[d] += gd * (A + A * om1 + Bq * q);
-            g[r] += gd * (-A + A * om1 + Bq * q);
-            const float cj = float(cx(d)) * jx + float(cy(d)) * jy + float(cz(d)) * jz;
-            const float Q = 4.5f * cj * cj - 1.5f * j2;
-            const float feqb = gd * (-2.0f * A * omega * om1);
-            drhob += feqb * w * (1.0f - Q * ir * ir);
-            const float t = feqb * w * ir;
-            jxb += t * (9.0f * cj * float(cx(d)) - 3.0f * jx);
-            jyb += t * (9.0f * cj * float(cy(d)) - 3.0f * jy);
-            jzb += t * (9.0f * cj * float(cz(d)) - 3.0f * jz);
-            if (KIND == kVelocityPressureInterp) drhob -= gd * w;
-            wallFactor *= Bq;
-        }
-        if (velocity) {
-            Ub[0] -= wallFactor * float(cx(d)) * gd;
-            Ub[1] -= wallFactor * float(cy(d)) * gd;
-            Ub[2] -= wallFactor * float(cz(d)) * gd;
-        }
-    }
-    if (interp) {
-#pragma unroll
-        for (int e = 0; e < 27; ++e) g[e] += drhob + float(cx(e)) * jxb + float(cy(e)) * jyb + float(cz(e)) * jzb;
-    }
-}
-
 // v_d = fn_d + w_d (rhoBC - drho) (1 + E_d(u)),  E_d = 3 c.u + 4.5 (c.u)^2 - 1.5 |u|^2
 VF_HD void pressure_vjp(const float (&fn)[27], float rhoBC, const float (&G)[27], float (&fnb)[27], float& rhob)
 {

@@ -92,14 +92,14 @@ class BoundaryConditionFactory:
 
 
 class BCEntryTables:
-    """Per-node tables of post-collision BC lists for kernels that apply the lists themselves
-    (the fused bounce-back collision, the fused BC adjoint): per node its entry or -1, and per entry
+    """Per-node tables of post-collision BC lists for a kernel that applies the lists itself
+    (the fused bounce-back collision): per node its entry or -1, and per entry
     and list the node's index in the list (or -1) and the bitmask of its links d with
     0 <= q_d <= 1. names: the lists in launch order (at most three). Raises ValueError where a link
     leads to a node the collision updates (a fused kernel would race with that node's own
     accesses), or on a lattice without a fluid mask."""
 
-    def __init__(self, lattice: Lattice, names, what: str = "fused BC"):
+    def __init__(self, lattice: Lattice, names, what: str = "bounce-back"):
         import numpy as np
         import torch
 
