@@ -318,8 +318,12 @@ int vf_VelocityInterpolatedCompressibleAdjoint(const vf_lbm_parameter* para, con
 int vf_VelocityWithPressureInterpolatedCompressibleAdjoint(const vf_lbm_parameter* para,
                                                            const vf_q_boundary_conditions* bc, float* adjoint,
                                                            float* velocityAdjoint, void* stream);
+/* neighborsDistinct: nonzero when no two nodes of the list share a kN (as for the inward
+ * neighbours of one face); each neighbour's cells are then updated by one thread with plain
+ * read-modify-writes, else with atomic adds. */
 int vf_PressureNonEquilibriumCompressibleAdjoint(const vf_lbm_parameter* para, const vf_q_boundary_conditions* bc,
-                                                 float* adjoint, float* rhoBCAdjoint, void* stream);
+                                                 float* adjoint, float* rhoBCAdjoint, int32_t neighborsDistinct,
+                                                 void* stream);
 
 /* ---------------------------------------------------------------------------------
  * Grid refinement: the launchers GridScalingFactory hands out for

@@ -253,7 +253,8 @@ def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box, saved
 
 @pytest.mark.parametrize("even", [True, False])
 @pytest.mark.parametrize("box", [True, False])
-def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box):
+@pytest.mark.parametrize("distinct", [1, 0])  # plain read-modify-writes, or atomic adds
+def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box, distinct):
     import torch
 
     from virtualfluids_amd import capi
@@ -267,9 +268,10 @@ def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box):
     lam0 = rng.normal(0, 1, 27 * g.N).astype(np.float32)
     lam = torch.from_numpy(lam0).cuda()
     bc = L.bcs["pressure"]
+    assert bc.kN_distinct
     rbar = torch.zeros(bc.n, dtype=torch.float32, device="cuda")
     capi.call("vf_PressureNonEquilibriumCompressibleAdjoint", L.pref(), ctypes.byref(bc.struct),
-              ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(rbar.data_ptr()), None)
+              ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(rbar.data_ptr()), distinct, None)
     A = orc.Lattice(g.nbx, g.nby, g.nbz)
     A.f[:] = L.populations()
     A.even = even

@@ -64,3 +64,16 @@ def test_grid_scaling_manager_checks():
         GridScalingKernelManager(3, f, cf_cells=100)
     with pytest.raises(NotImplementedError):
         g.get_grid_scaling_fc(True)
+
+
+def test_pressure_list_records_whether_its_neighbours_are_distinct():
+    """The pressure adjoint skips atomics only where no two nodes share a neighbour kN."""
+    from virtualfluids_amd.lattice import DeviceBC
+
+    q = np.full((27, 3), -1.0)
+    one = DeviceBC(BoundaryNodes(k=np.array([5, 6, 7]), q27=q, kN=np.array([1, 2, 3]), rho_bc=np.zeros(3),
+                                 direction=1), "cpu", 10)
+    assert one.kN_distinct
+    shared = DeviceBC(BoundaryNodes(k=np.array([5, 6, 7]), q27=q, kN=np.array([1, 2, 1]), rho_bc=np.zeros(3),
+                                    direction=1), "cpu", 10)
+    assert not shared.kN_distinct

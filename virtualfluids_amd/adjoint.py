@@ -156,7 +156,7 @@ class Stepper:
         bc = self._bc("pressure")
         if self.pressure and bc is not None:
             capi.call(self.pressure + "Adjoint", ctypes.byref(_param(L, after, not even)), ctypes.byref(bc.struct),
-                      _vp(lam), _vp(rbar), s)
+                      _vp(lam), _vp(rbar), int(bc.kN_distinct), s)
         p = _param(L, before, even)
         for name, fn, grad in (("geometry", self.geometry, None), ("noslip", self.noslip, None),
                                ("velocity", self.velocity, ubar)):

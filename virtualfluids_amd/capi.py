@@ -166,7 +166,7 @@ SIGNATURES = {
     "vf_VelocityBounceBackAdjoint": [_P, _Q, _vp, _vp, _vp],
     "vf_VelocityInterpolatedCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
     "vf_VelocityWithPressureInterpolatedCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
-    "vf_PressureNonEquilibriumCompressibleAdjoint": [_P, _Q, _vp, _vp, _vp],
+    "vf_PressureNonEquilibriumCompressibleAdjoint": [_P, _Q, _vp, _vp, ctypes.c_int32, _vp],
     "vf_InitNavierStokesCompressibleNonEquilibrium": [_P, _vp, _vp],
     "vf_ScaleFineToCoarseCompressible": [_P, _P, _vp, _vp, _vp],
 }

@@ -58,6 +58,8 @@ class DeviceBC:
         if nodes.velocity is not None:
             self.vel = [torch.from_numpy(np.asarray(v, dtype=np.float32)).to(device) for v in nodes.velocity]
         self.kN = None if nodes.kN is None else torch.from_numpy(nodes.kN.astype(np.int32)).to(device)
+        # whether no two nodes share a neighbour kN (the pressure adjoint then needs no atomics)
+        self.kN_distinct = nodes.kN is None or np.unique(np.asarray(nodes.kN)).size == np.asarray(nodes.kN).size
         self.rho_bc = None if nodes.rho_bc is None else torch.from_numpy(nodes.rho_bc.astype(np.float32)).to(device)
         self.direction = nodes.direction
         self.struct = capi.QBoundaryConditions()
