@@ -396,6 +396,29 @@ int vf_ScaleCoarseToFineCompressibleAdjoint(const vf_lbm_parameter* coarse, cons
                                             const vf_interpolation_cell_neighbor* offsets, float* coarseAdjoint,
                                             float* fineAdjoint, void* stream);
 
+/* The distinct source nodes of a set of interface cells: node[u] for u < numberOfNodes, and
+ * first[u] .. first[u + 1] (numberOfNodes + 1 entries) indexing lanes, the entries 8 i + q of the
+ * cells i whose corner q is node u (every 8 i + q once; for coarse-to-fine cells the corners of
+ * the coarse cell at coarseCellIndices[i], bit 0 of q the +x, bit 1 the +y, bit 2 the +z step). */
+typedef struct vf_interpolation_sources {
+    const uint32_t* node;
+    const uint32_t* first;
+    const uint32_t* lanes;
+    uint32_t numberOfNodes;
+} vf_interpolation_sources;
+
+/* vf_ScaleCoarseToFineCompressibleAdjoint with the transposed interpolation gathered per source
+ * node (no counterpart in the reference): the cell kernel stores each corner's moment adjoint in
+ * `workspace` (9 x 8 x numberOfCells floats, overwritten), then one thread per source node sums
+ * its corners' (in the order of `lanes`) and adds the result to the node's cells, instead of
+ * an atomic add per cell corner. The same results up to summation order, deterministic from
+ * call to call. */
+int vf_ScaleCoarseToFineCompressibleAdjointGather(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                                  const vf_interpolation_cells* coarseToFine,
+                                                  const vf_interpolation_cell_neighbor* offsets,
+                                                  const vf_interpolation_sources* sources, float* workspace,
+                                                  float* coarseAdjoint, float* fineAdjoint, void* stream);
+
 /* Adjoint of vf_ScaleFineToCoarseCompressible: the adjoint of the coarse cells it wrote is read
  * and cleared, its transpose added to the fine source nodes' cells (fine parity even). */
 int vf_ScaleFineToCoarseCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,

@@ -5,6 +5,8 @@ box (random states, random adjoint buffers) against orc.scale_adjoint (forward-m
 through the oracle's interpolation, cell by cell) on the same stored states. Bar: FP32 arithmetic
 of the hand-written reverse, 3e-5 of the largest adjoint entry (the CPU build of the same
 functions stays within 4e-7 of the dual numbers per cell; the atomic additions sum in any order).
+The gather variants run vf_ScaleCoarseToFineCompressibleAdjointGather (one thread per coarse
+source node sums its cell corners' contributions) to the same bar.
 """
 import numpy as np
 import pytest
@@ -20,7 +22,7 @@ OMEGA_C, OMEGA_F = 1.7, 1.5
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 @pytest.mark.parametrize("fine_to_coarse", [False, True])
 @pytest.mark.parametrize("even_coarse", [True, False])
-@pytest.mark.parametrize("variant", ["arrays", "offsets", "box_addressing"])
+@pytest.mark.parametrize("variant", ["arrays", "offsets", "box_addressing", "gather", "gather_offsets"])
 def test_scale_adjoint_kernel_matches_reference(gpu, orc, storage, fine_to_coarse, even_coarse, variant):
     import ctypes
 
@@ -32,9 +34,12 @@ def test_scale_adjoint_kernel_matches_reference(gpu, orc, storage, fine_to_coars
 
     nb = nested_box((12, 10, 14), (2, 1, 3), (9, 8, 10))
     rng = np.random.default_rng(1400 + 4 * fine_to_coarse + 2 * even_coarse + len(variant))
+    gather = variant.startswith("gather")
+    if gather and fine_to_coarse:
+        pytest.skip("the per-source-node gather is the coarse-to-fine adjoint's")
     cells = nb.fc if fine_to_coarse else nb.cf
     offs = None
-    if variant == "offsets":
+    if variant in ("offsets", "gather_offsets"):
         offs = [rng.uniform(-0.4, 0.4, cells.n) for _ in range(3)]
         cells = InterfaceCells(cells.coarse, cells.fine, tuple(offs))
     # box lattices address by (x, y, z); indexed lattices pass the neighbour arrays
@@ -55,8 +60,15 @@ def test_scale_adjoint_kernel_matches_reference(gpu, orc, storage, fine_to_coars
         assert (L.param.neighborX is None) == box
     fn = "vf_ScaleFineToCoarseCompressibleAdjoint" if fine_to_coarse else "vf_ScaleCoarseToFineCompressibleAdjoint"
     print(f"{fn} {storage} even_coarse={even_coarse} {variant} cells={cells.n}", flush=True)
-    capi.call(fn, C.pref(), F.pref(), ctypes.byref(dc.struct), neigh, ctypes.c_void_p(lc.data_ptr()),
-              ctypes.c_void_p(lf.data_ptr()), stream_handle())
+    if gather:
+        src, nwork = dc.sources(nb.coarse)
+        work = torch.full((nwork,), float("nan"), dtype=torch.float32, device="cuda")
+        capi.call("vf_ScaleCoarseToFineCompressibleAdjointGather", C.pref(), F.pref(), ctypes.byref(dc.struct), neigh,
+                  ctypes.byref(src), ctypes.c_void_p(work.data_ptr()), ctypes.c_void_p(lc.data_ptr()),
+                  ctypes.c_void_p(lf.data_ptr()), stream_handle())
+    else:
+        capi.call(fn, C.pref(), F.pref(), ctypes.byref(dc.struct), neigh, ctypes.c_void_p(lc.data_ptr()),
+                  ctypes.c_void_p(lf.data_ptr()), stream_handle())
     torch.cuda.synchronize()
 
     oc = orc.Lattice(nb.coarse.nbx, nb.coarse.nby, nb.coarse.nbz)

@@ -77,3 +77,31 @@ def test_pressure_list_records_whether_its_neighbours_are_distinct():
     shared = DeviceBC(BoundaryNodes(k=np.array([5, 6, 7]), q27=q, kN=np.array([1, 2, 1]), rho_bc=np.zeros(3),
                                     direction=1), "cpu", 10)
     assert not shared.kN_distinct
+
+
+def test_interface_sources_list_every_cell_corner_once():
+    """vf_interpolation_sources built on the host: every lane 8 i + q appears once, under the node
+    that is corner q of coarse cell i (+x, +y, +z steps through the neighbour arrays)."""
+    import torch  # noqa: F401  (DeviceInterfaceCells keeps its arrays as tensors)
+
+    from virtualfluids_amd.refinement import DeviceInterfaceCells, nested_box
+
+    nb = nested_box((12, 10, 14), (2, 1, 3), (9, 8, 10))
+    dc = DeviceInterfaceCells(nb.cf, "cpu")
+    dc.sources(nb.coarse)
+    nodes, first, lanes = (t.numpy().view(np.uint32).astype(np.int64) for t in dc._sources[1])
+    assert lanes.size == 8 * nb.cf.n and np.array_equal(np.sort(lanes), np.arange(8 * nb.cf.n))
+    assert first[0] == 0 and first[-1] == lanes.size and np.all(np.diff(first) > 0)
+    g = nb.coarse
+    for u in range(nodes.size):
+        for l in lanes[first[u]:first[u + 1]]:
+            i, q = divmod(int(l), 8)
+            k = int(nb.cf.coarse[i])
+            if q & 1:
+                k = int(g.nbx[k])
+            if q & 2:
+                k = int(g.nby[k])
+            if q & 4:
+                k = int(g.nbz[k])
+            assert k == nodes[u]
+    assert nodes.size < 8 * nb.cf.n  # neighbouring cells share corners

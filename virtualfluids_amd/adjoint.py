@@ -378,6 +378,8 @@ class _TwoLevelSteps(torch.autograd.Function):
         pfb = torch.zeros(Lf.N, device=dev) if has_pf else None
         s = stream_handle(stream)
         cf, fc = cells
+        cf_sources, nwork = cf.sources(Lc.grid)
+        cf_work = torch.empty(max(nwork, 1), dtype=torch.float32, device=dev)
         lists_c, lists_f = sc.post_lists(), sf.post_lists()
 
         def post_buffers(lists):
@@ -402,8 +404,10 @@ class _TwoLevelSteps(torch.autograd.Function):
                 pc = _param(Lc, C2, not even)
                 pf = _param(Lf, F2, True)
                 if cf.n:
-                    capi.call("vf_ScaleCoarseToFineCompressibleAdjoint", ctypes.byref(pc), ctypes.byref(pf),
-                              ctypes.byref(cf.struct), _neigh(cf), _vp(lc), _vp(lf), s)
+                    # the transposed interpolation gathered per coarse source node (no atomics)
+                    capi.call("vf_ScaleCoarseToFineCompressibleAdjointGather", ctypes.byref(pc), ctypes.byref(pf),
+                              ctypes.byref(cf.struct), _neigh(cf), ctypes.byref(cf_sources), _vp(cf_work), _vp(lc),
+                              _vp(lf), s)
                 if fc.n:
                     capi.call("vf_ScaleFineToCoarseCompressibleAdjoint", ctypes.byref(pc), ctypes.byref(pf),
                               ctypes.byref(fc.struct), _neigh(fc), _vp(lc), _vp(lf), s)

@@ -126,6 +126,12 @@ class InterpolationCellNeighbor(ctypes.Structure):
     _fields_ = [("x", _vp), ("y", _vp), ("z", _vp)]
 
 
+class InterpolationSources(ctypes.Structure):
+    """vf_interpolation_sources: the distinct source nodes of interface cells and their lanes."""
+
+    _fields_ = [("node", _vp), ("first", _vp), ("lanes", _vp), ("numberOfNodes", ctypes.c_uint32)]
+
+
 # VF_ABI_VERSION of include/virtualfluids_amd.h that these structs mirror
 ABI_VERSION = 5
 
@@ -154,6 +160,8 @@ SIGNATURES = {
     "vf_MomentSums": [_P, _vp, _vp],
     "vf_ScaleCoarseToFineCompressibleAdjoint": [_P, _P, _vp, _vp, _vp, _vp, _vp],
     "vf_ScaleFineToCoarseCompressibleAdjoint": [_P, _P, _vp, _vp, _vp, _vp, _vp],
+    "vf_ScaleCoarseToFineCompressibleAdjointGather": [_P, _P, _vp, _vp, ctypes.POINTER(InterpolationSources), _vp,
+                                                      _vp, _vp, _vp],
     "vf_DragLiftPostD27": [_P, _Q, _vp, _vp, _vp, _vp],
     "vf_DragLiftPreD27": [_P, _Q, _vp, _vp, _vp, _vp],
     "vf_DragLiftSum": [_P, _Q, _vp, _vp],

@@ -56,12 +56,25 @@ struct AdjBoxArgs {
     PlaneDiv plane{};      // scalar kernel over every node: plane-flat numbering
 };
 
-// (x, y, z) of this thread's node in the scalar kernel; false for threads beyond the nodes
+// (x, y, z) of this thread's node in the scalar kernel; false for threads beyond the nodes.
+// XCD: the plane-flat workgroups taken in XCD block ranges (as the forward node kernel).
+template <bool XCD = false>
 __device__ __forceinline__ bool adj_coords(const AdjBoxArgs& a, int& x, int& y, int& z)
 {
     if (a.x0 == 0) {
-        z = blockIdx.y;
-        return plane_xy(a.plane, x, y);
+        uint32_t bx = blockIdx.x, bz = blockIdx.y;
+        if constexpr (XCD) {
+            const uint32_t b = xcd_block(bx + gridDim.x * bz, gridDim.x * gridDim.y);
+            bx = b % gridDim.x;
+            bz = b / gridDim.x;
+        }
+        const uint32_t t = bx * blockDim.x + threadIdx.x;
+        if (t >= a.plane.planeNodes) return false;
+        const uint32_t row = plane_row(t, a.plane);
+        x = (int)(t - row * a.plane.nx);
+        y = (int)row;
+        z = (int)bz;
+        return true;
     }
     const uint32_t tw = (uint32_t)a.nx - a.x0;
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
@@ -79,12 +92,12 @@ __device__ __forceinline__ void brinkman_adjoint(const float* porosity, float* p
     if (porosity && porosityAdj) porosityAdj[k] += -0.5f * sb;
 }
 
-template <class S, bool EVEN, bool MASK, bool SHIFT>
+template <class S, bool EVEN, bool MASK, bool SHIFT, bool XCD = false>
 __global__ __launch_bounds__(kAdjBlock) void k17_adjoint_box_kernel(const S* __restrict__ f, float* __restrict__ lam,
                                                                     AdjBoxArgs a, K17Rates r)
 {
     int x, y, z;
-    if (!adj_coords(a, x, y, z)) return;
+    if (!adj_coords<XCD>(a, x, y, z)) return;
     const int xp = (x + 1 == a.nx) ? 0 : x + 1;
     const int yp = (y + 1 == a.ny) ? 0 : y + 1;
     const int zp = (z + 1 == a.nz) ? 0 : z + 1;
@@ -377,10 +390,12 @@ void launch_row_b(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam
 
 template <class S, bool EVEN>
 void launch_scalar(const AdjBoxArgs& a, const K17Rates& r, const S* f, float* lam, bool mask, bool shift, dim3 grid,
-                   hipStream_t s)
+                   hipStream_t s, bool xcd = false)
 {
     const dim3 block(kAdjBlock);
-    if (mask && shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, true>), grid, block, 0, s, f, lam, a, r);
+    if (xcd && mask && !shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, false, true>), grid, block, 0, s, f, lam, a, r);
+    else if (xcd && !mask && !shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, false, true>), grid, block, 0, s, f, lam, a, r);
+    else if (mask && shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, true>), grid, block, 0, s, f, lam, a, r);
     else if (mask) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, true, false>), grid, block, 0, s, f, lam, a, r);
     else if (shift) hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, true>), grid, block, 0, s, f, lam, a, r);
     else hipLaunchKernelGGL((k17_adjoint_box_kernel<S, EVEN, false, false>), grid, block, 0, s, f, lam, a, r);
@@ -427,7 +442,8 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj
         return;
     }
     static_assert(kAdjBlock == kPlaneBlock, "the scalar adjoint kernel takes plane-flat workgroups");
-    launch_scalar<S, EVEN>(a, r, f, lam, mask, shift != nullptr, dim3(plane_blocks(p.nx, p.ny), p.nz), s);
+    launch_scalar<S, EVEN>(a, r, f, lam, mask, shift != nullptr, dim3(plane_blocks(p.nx, p.ny), p.nz), s,
+                           xcd_planes(p.nx, p.ny, sizeof(S), kc.rowXcdRanges));
 }
 
 template <class S, bool EVEN>

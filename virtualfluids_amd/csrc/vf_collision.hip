@@ -112,12 +112,31 @@ __device__ __forceinline__ void bounce_back_epilogue(S* __restrict__ f, uint64_t
     }
 }
 
-template <class S, bool EVEN, bool MASK, bool MACRO, bool BB = false>
+// The node of this thread in the plane-flat numbering; XCD: the (plane_blocks, nz) workgroups
+// taken in XCD block ranges, so that consecutive spans of a plane, which share the cache lines at
+// their ends, meet in one L2
+template <bool XCD> __device__ __forceinline__ bool plane_node(const PlaneDiv& p, int& x, int& y, int& z)
+{
+    uint32_t bx = blockIdx.x, bz = blockIdx.y;
+    if constexpr (XCD) {
+        const uint32_t b = xcd_block(bx + gridDim.x * bz, gridDim.x * gridDim.y);
+        bx = b % gridDim.x;
+        bz = b / gridDim.x;
+    }
+    const uint32_t t = bx * kBoxBlock + threadIdx.x;
+    if (t >= p.planeNodes) return false;
+    const uint32_t row = plane_row(t, p);
+    x = (int)(t - row * p.nx);
+    y = (int)row;
+    z = (int)bz;
+    return true;
+}
+
+template <class S, bool EVEN, bool MASK, bool MACRO, bool BB = false, bool XCD = false>
 __global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, BoxArgs a, K17Rates r, FusedBB bb)
 {
-    int x, y;
-    if (!plane_xy(a.plane, x, y)) return;
-    const int z = blockIdx.y;
+    int x, y, z;
+    if (!plane_node<XCD>(a.plane, x, y, z)) return;
     const int xp = (x + 1 == a.nx) ? 0 : x + 1;
     const int yp = (y + 1 == a.ny) ? 0 : y + 1;
     const int zp = (z + 1 == a.nz) ? 0 : z + 1;
@@ -145,13 +164,12 @@ __global__ __launch_bounds__(kBoxBlock) void k17_box_kernel(S* __restrict__ f, B
 // Out-of-place variant: reads `in`, writes the stepped state to `out` (same layout). Masked
 // nodes copy their 27 cells unchanged, so `out` is the complete state the in-place step
 // would leave. Used to replay a segment of steps while keeping every intermediate state.
-template <class S, bool EVEN, bool MASK>
+template <class S, bool EVEN, bool MASK, bool XCD = false>
 __global__ __launch_bounds__(kBoxBlock) void k17_box_into_kernel(const S* __restrict__ in, S* __restrict__ out,
                                                                  BoxArgs a, K17Rates r)
 {
-    int x, y;
-    if (!plane_xy(a.plane, x, y)) return;
-    const int z = blockIdx.y;
+    int x, y, z;
+    if (!plane_node<XCD>(a.plane, x, y, z)) return;
     const int xp = (x + 1 == a.nx) ? 0 : x + 1;
     const int yp = (y + 1 == a.ny) ? 0 : y + 1;
     const int zp = (z + 1 == a.nz) ? 0 : z + 1;
@@ -544,9 +562,12 @@ static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
     if (try_launch_row<S, EVEN>(p, a, r, macro, nullptr, s)) return;
     const dim3 grid(plane_blocks(p.nx, p.ny), p.nz), bx(kBoxBlock);
     const FusedBB none{};
+    const bool xcd = xcd_planes(p.nx, p.ny, sizeof(S), p.kernelChoice.rowXcdRanges);
     if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r, none);
+    else if (mask && xcd) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false, false, true>), grid, dim3(bx), 0, s, f, a, r, none);
     else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r, none);
     else if (macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, true>), grid, dim3(bx), 0, s, f, a, r, none);
+    else if (xcd) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, false, false, true>), grid, dim3(bx), 0, s, f, a, r, none);
     else hipLaunchKernelGGL((k17_box_kernel<S, EVEN, false, false>), grid, dim3(bx), 0, s, f, a, r, none);
 }
 
@@ -617,8 +638,13 @@ template <class S, bool EVEN> struct BoxIntoL {
         if (try_launch_row<S, EVEN>(p, a, r, false, static_cast<S*>(out), s)) return;
         const dim3 grid(plane_blocks(p.nx, p.ny), p.nz), bx(kBoxBlock);
         const S* in = static_cast<const S*>(p.distributions);
-        if (p.fluidMask)
+        const bool xcd = xcd_planes(p.nx, p.ny, sizeof(S), p.kernelChoice.rowXcdRanges);
+        if (p.fluidMask && xcd)
+            hipLaunchKernelGGL((k17_box_into_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, in, (S*)out, a, r);
+        else if (p.fluidMask)
             hipLaunchKernelGGL((k17_box_into_kernel<S, EVEN, true>), grid, dim3(bx), 0, s, in, (S*)out, a, r);
+        else if (xcd)
+            hipLaunchKernelGGL((k17_box_into_kernel<S, EVEN, false, true>), grid, dim3(bx), 0, s, in, (S*)out, a, r);
         else
             hipLaunchKernelGGL((k17_box_into_kernel<S, EVEN, false>), grid, dim3(bx), 0, s, in, (S*)out, a, r);
     }

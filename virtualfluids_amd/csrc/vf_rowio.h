@@ -48,6 +48,16 @@ inline uint32_t xcd_rows(int nx, size_t elemBytes, int choice)
     return ((size_t)nx * elemBytes) % 128 != 0;
 }
 
+// host: the same for the plane-flat one-node-per-thread kernels, whose workgroups take
+// consecutive spans of a plane: spans share their end lines with the next one where a plane's
+// bytes are not a whole number of lines (258 x 130 x 130 FP16 on MI355X: 88 -> 84 us in place,
+// 140 -> 125 us out of place; FP32 204 -> 192 and 230 -> 212 us)
+inline uint32_t xcd_planes(int nx, int ny, size_t elemBytes, int choice)
+{
+    if (choice == 1 || choice == 2) return choice == 1;
+    return ((size_t)nx * (size_t)ny * elemBytes) % 128 != 0;
+}
+
 template <int BYTES> struct RawT;
 template <> struct RawT<2> { using T = unsigned short; };
 template <> struct RawT<4> { using T = unsigned int; };

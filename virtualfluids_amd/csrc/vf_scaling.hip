@@ -192,11 +192,14 @@ __device__ __forceinline__ void source_node_adjoint(const S* f, float* lam, cons
     for (int d = 0; d < 27; ++d) atomicAdd(adj_cell(lam, a, nn, d, even), fb[d]);
 }
 
-template <class S>
+// GATHER: lane l = 8 cell + corner stores the adjoint of its source node's moments in mbOut
+// (field-major, 9 x 8 n floats) instead of adding it to the node's cells; source_gather_kernel
+// then sums the lanes of every source node and applies it once per node
+template <class S, bool GATHER = false>
 __global__ __launch_bounds__(kScaleBlock) void scale_cf_adjoint_kernel(const S* __restrict__ fc,
                                                                        float* __restrict__ lamC,
                                                                        float* __restrict__ lamF, Addr ac, Addr af,
-                                                                       ScaleArgs s, int evenC)
+                                                                       ScaleArgs s, int evenC, float* mbOut = nullptr)
 {
     const CellLane cl = cell_lane(s.n);
     SrcMom m[8];
@@ -223,7 +226,60 @@ __global__ __launch_bounds__(kScaleBlock) void scale_cf_adjoint_kernel(const S* 
     coef_group_sum(cb);  // every lane of the group takes part (invalid cells add zeros)
     if (!cl.valid) return;
     const SrcMom mb = node_of_sums(coefficients_vjp_sums(m, xo, yo, zo, cb), q);
-    source_node_adjoint(fc, lamC, ac, corner(ac, s.cellsC[cl.i], q), evenC != 0, s.omegaC, mb);
+    if constexpr (GATHER) {
+        const uint64_t L = 8ull * s.n, l = 8ull * cl.i + (uint64_t)q;
+        const float* src = &mb.drho;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) mbOut[k * L + l] = src[k];
+    } else {
+        source_node_adjoint(fc, lamC, ac, corner(ac, s.cellsC[cl.i], q), evenC != 0, s.omegaC, mb);
+    }
+}
+
+// The distinct source nodes of a set of interface cells: node[u], and first[u] .. first[u+1]
+// indexing `lanes`, the lanes 8 cell + corner that read node u.
+struct SourceArgs {
+    const uint32_t* node;
+    const uint32_t* first;
+    const uint32_t* lanes;
+    uint32_t count;
+    const float* mb; // 9 x L partial adjoints of the source moments, field-major
+    uint32_t L;      // 8 x cells
+    float omega;
+};
+
+// One thread per source node: the sum of its lanes' moment adjoints, taken back to the node's 27
+// cells once (source_node_adjoint without atomics, and one read of the node's populations
+// instead of one per cell that shares it). Lane and node indices outside the arrays are skipped.
+template <class S>
+__global__ __launch_bounds__(64) void source_gather_kernel(const S* __restrict__ f, float* __restrict__ lam, Addr a,
+                                                           SourceArgs s, int even_flag)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= s.count) return;
+    const uint32_t k = s.node[u];
+    if ((uint64_t)k >= a.N) return;
+    float acc9[9] = {};
+    for (uint32_t j = s.first[u], e = s.first[u + 1]; j < e; ++j) {
+        const uint32_t l = s.lanes[j];
+        if (l >= s.L) continue;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc9[c] += s.mb[(uint64_t)c * s.L + l];
+    }
+    SrcMom mb;
+    float* dst = &mb.drho;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) dst[c] = acc9[c];
+    const bool even = even_flag != 0;
+    float v[27], fb[27], old[27];
+    node_populations(f, a, k, even, v);
+    source_moments_vjp(v, s.omega, mb, fb);
+    uint32_t nn[8];
+    cell_nodes(a, k, nn);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) old[d] = *adj_cell(lam, a, nn, d, even);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) *adj_cell(lam, a, nn, d, even) = old[d] + fb[d];
 }
 
 template <class S>
@@ -265,10 +321,14 @@ static int check_level(const vf_lbm_parameter* p)
     return 1;
 }
 
-// lamC/lamF null: the forward kernels; otherwise their adjoints
+static_assert(sizeof(SrcMom) == 9 * sizeof(float), "the gather workspace holds SrcMom as 9 floats");
+
+// lamC/lamF null: the forward kernels; otherwise their adjoints. sources/workspace (the
+// coarse-to-fine adjoint only): per-source-node gather instead of atomic adds
 static int launch_scale(bool fine_to_coarse, const vf_lbm_parameter* c, const vf_lbm_parameter* f,
                         const vf_interpolation_cells* cells, const vf_interpolation_cell_neighbor* off, void* stream,
-                        float* lamC = nullptr, float* lamF = nullptr)
+                        float* lamC = nullptr, float* lamF = nullptr, const vf_interpolation_sources* sources = nullptr,
+                        float* workspace = nullptr)
 {
     const bool adjoint = lamC || lamF;
     if (adjoint && (!lamC || !lamF)) {
@@ -292,18 +352,32 @@ static int launch_scale(bool fine_to_coarse, const vf_lbm_parameter* c, const vf
         set_last_error("omega must lie in (0, 2) on both levels");
         return VF_ERR_INVALID_ARGUMENT;
     }
+    if (sources && (!sources->node || !sources->first || !sources->lanes || !workspace)) {
+        set_last_error("grid scaling adjoint gather needs node, first and lanes of the sources and a workspace");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
     ScaleArgs s{cells->coarseCellIndices, cells->fineCellIndices, cells->numberOfCells,
                 off ? off->x : nullptr, off ? off->y : nullptr, off ? off->z : nullptr, c->omega, f->omega};
     const Addr ac = make_addr(*c), af = make_addr(*f);
     const dim3 grid((cells->numberOfCells + kScaleCells - 1) / kScaleCells), block(kScaleBlock);
     hipStream_t st = (hipStream_t)stream;
+    const SourceArgs src{sources ? sources->node : nullptr, sources ? sources->first : nullptr,
+                         sources ? sources->lanes : nullptr, sources ? sources->numberOfNodes : 0u, workspace,
+                         8u * cells->numberOfCells, c->omega};
+    const dim3 sgrid((src.count + 63) / 64);
 #define VF_LAUNCH(S)                                                                                                 \
-    if (adjoint && fine_to_coarse)                                                                                   \
+    if (sources) {                                                                                                   \
+        hipLaunchKernelGGL((scale_cf_adjoint_kernel<S, true>), grid, block, 0, st, (const S*)c->distributions, lamC, \
+                           lamF, ac, af, s, c->isEvenTimestep, workspace);                                           \
+        if (src.count)                                                                                               \
+            hipLaunchKernelGGL(source_gather_kernel<S>, sgrid, dim3(64), 0, st, (const S*)c->distributions, lamC, ac, \
+                               src, c->isEvenTimestep);                                                              \
+    } else if (adjoint && fine_to_coarse)                                                                            \
         hipLaunchKernelGGL(scale_fc_adjoint_kernel<S>, grid, block, 0, st, (const S*)f->distributions, lamC, lamF,  \
                            ac, af, s, c->isEvenTimestep);                                                            \
     else if (adjoint)                                                                                                \
-        hipLaunchKernelGGL(scale_cf_adjoint_kernel<S>, grid, block, 0, st, (const S*)c->distributions, lamC, lamF,  \
-                           ac, af, s, c->isEvenTimestep);                                                            \
+        hipLaunchKernelGGL((scale_cf_adjoint_kernel<S, false>), grid, block, 0, st, (const S*)c->distributions, lamC, \
+                           lamF, ac, af, s, c->isEvenTimestep, nullptr);                                             \
     else if (fine_to_coarse)                                                                                         \
         hipLaunchKernelGGL(scale_fc_kernel<S>, grid, block, 0, st, (S*)c->distributions, (const S*)f->distributions, \
                            ac, af, s, c->isEvenTimestep);                                                            \
@@ -346,6 +420,24 @@ extern "C" int vf_ScaleCoarseToFineCompressibleAdjoint(const vf_lbm_parameter* c
                                                        float* coarseAdjoint, float* fineAdjoint, void* stream)
 {
     return launch_scale(false, coarse, fine, coarseToFine, offsets, stream, coarseAdjoint, fineAdjoint);
+}
+
+extern "C" int vf_ScaleCoarseToFineCompressibleAdjointGather(const vf_lbm_parameter* coarse,
+                                                             const vf_lbm_parameter* fine,
+                                                             const vf_interpolation_cells* coarseToFine,
+                                                             const vf_interpolation_cell_neighbor* offsets,
+                                                             const vf_interpolation_sources* sources, float* workspace,
+                                                             float* coarseAdjoint, float* fineAdjoint, void* stream)
+{
+    if (!sources) {
+        set_last_error("grid scaling adjoint gather needs the source nodes of the cells");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (!coarseAdjoint || !fineAdjoint) {
+        set_last_error("grid scaling adjoint needs the adjoint buffers of both levels");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    return launch_scale(false, coarse, fine, coarseToFine, offsets, stream, coarseAdjoint, fineAdjoint, sources, workspace);
 }
 
 extern "C" int vf_ScaleFineToCoarseCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,

@@ -147,6 +147,39 @@ class DeviceInterfaceCells:
         if cells.offset is not None:
             self.offsets = [torch.from_numpy(np.asarray(o, dtype=np.float32)).to(device) for o in cells.offset]
             self.neigh.x, self.neigh.y, self.neigh.z = (ctypes.c_void_p(t.data_ptr()) for t in self.offsets)
+        self.device = device
+        self.host = cells
+        self._sources = None
+
+    def sources(self, grid):
+        """vf_interpolation_sources of the coarse cells (corner q of cell i: q's bits step +x, +y,
+        +z through the grid's neighbour arrays), built once: (struct, workspace floats)."""
+        if self._sources is None:
+            base = np.asarray(self.host.coarse, dtype=np.int64)
+            corners = np.empty((self.n, 8), dtype=np.int64)
+            for q in range(8):
+                k = base.copy()
+                if q & 1:
+                    k = grid.nbx[k].astype(np.int64)
+                if q & 2:
+                    k = grid.nby[k].astype(np.int64)
+                if q & 4:
+                    k = grid.nbz[k].astype(np.int64)
+                corners[:, q] = k
+            lanes_flat = corners.reshape(-1)  # entry 8 i + q
+            order = np.argsort(lanes_flat, kind="stable")
+            nodes, first_idx = np.unique(lanes_flat[order], return_index=True)
+            first = np.append(first_idx, order.size).astype(np.uint32)
+
+            def u32(a):
+                return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)).to(self.device)
+
+            t = (u32(nodes), u32(first), u32(order))
+            st = capi.InterpolationSources()
+            st.node, st.first, st.lanes = (ctypes.c_void_p(x.data_ptr()) for x in t)
+            st.numberOfNodes = int(nodes.size)
+            self._sources = (st, t, 9 * 8 * self.n)
+        return self._sources[0], self._sources[2]
 
 
 class GridScalingFactory:
