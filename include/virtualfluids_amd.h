@@ -419,6 +419,19 @@ int vf_ScaleCoarseToFineCompressibleAdjointGather(const vf_lbm_parameter* coarse
                                                   const vf_interpolation_sources* sources, float* workspace,
                                                   float* coarseAdjoint, float* fineAdjoint, void* stream);
 
+/* vf_ScaleCoarseToFineCompressibleAdjointGather that allocates nothing and takes no workspace. No
+ * counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Its workspace is a device
+ * array inside the library, 18 MiB on every GPU it runs on, held until the process ends, which
+ * limits a call to 65536 cells (larger calls return VF_ERR_INVALID_ARGUMENT). Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU (other streams, other host threads)
+ * race, and must be ordered by the caller. Results are bit-identical to the workspace form. */
+int vf_ScaleCoarseToFineCompressibleAdjointGatherNoAlloc(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
+                                                         const vf_interpolation_cells* coarseToFine,
+                                                         const vf_interpolation_cell_neighbor* offsets,
+                                                         const vf_interpolation_sources* sources, float* coarseAdjoint,
+                                                         float* fineAdjoint, void* stream);
+
 /* Adjoint of vf_ScaleFineToCoarseCompressible: the adjoint of the coarse cells it wrote is read
  * and cleared, its transpose added to the fine source nodes' cells (fine parity even). */
 int vf_ScaleFineToCoarseCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,

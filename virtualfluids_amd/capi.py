@@ -162,6 +162,8 @@ SIGNATURES = {
     "vf_ScaleFineToCoarseCompressibleAdjoint": [_P, _P, _vp, _vp, _vp, _vp, _vp],
     "vf_ScaleCoarseToFineCompressibleAdjointGather": [_P, _P, _vp, _vp, ctypes.POINTER(InterpolationSources), _vp,
                                                       _vp, _vp, _vp],
+    "vf_ScaleCoarseToFineCompressibleAdjointGatherNoAlloc": [_P, _P, _vp, _vp, ctypes.POINTER(InterpolationSources),
+                                                             _vp, _vp, _vp],
     "vf_DragLiftPostD27": [_P, _Q, _vp, _vp, _vp, _vp],
     "vf_DragLiftPreD27": [_P, _Q, _vp, _vp, _vp, _vp],
     "vf_DragLiftSum": [_P, _Q, _vp, _vp],

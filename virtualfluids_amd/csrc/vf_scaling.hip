@@ -12,6 +12,8 @@
 // fine nodes at (+-1/4, +-1/4, +-1/4) of the coarse cell with the fine grid's even parity
 // (the fine grid has taken two steps); FC writes the one coarse node at the centre of the
 // fine cell with the coarse parity, reading the fine grid with the even parity.
+#include <mutex>
+
 #include "vf_lattice.h"
 #include "vf_scaling.h"
 
@@ -323,6 +325,38 @@ static int check_level(const vf_lbm_parameter* p)
 
 static_assert(sizeof(SrcMom) == 9 * sizeof(float), "the gather workspace holds SrcMom as 9 floats");
 
+// The library-owned workspace of the gathered coarse-to-fine adjoint for callers that cannot
+// allocate (vf_ScaleCoarseToFineCompressibleAdjointGatherNoAlloc): 9 floats per cell corner for at
+// most kNoAllocCells cells (18 MiB), one copy per GPU, held until the process ends. Host code
+// never passes this symbol itself to a kernel: it passes the device address that
+// hipGetSymbolAddress returns for the current device.
+constexpr uint32_t kNoAllocCells = 1u << 16;
+static __device__ __align__(256) float scratch[9u * 8u * kNoAllocCells];
+
+// The device address of `scratch` on the current device, looked up once per device.
+static int scratch_address(float** out)
+{
+    static std::mutex lock;
+    static float* address[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_last_error("no-alloc grid scaling adjoint: no current device (or a device index above 63)");
+        return VF_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> guard(lock);
+    if (!address[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_last_error("no-alloc grid scaling adjoint: hipGetSymbolAddress of the workspace failed");
+            return VF_ERR_HIP;
+        }
+        address[dev] = static_cast<float*>(ptr);
+    }
+    *out = address[dev];
+    return VF_OK;
+}
+
 // lamC/lamF null: the forward kernels; otherwise their adjoints. sources/workspace (the
 // coarse-to-fine adjoint only): per-source-node gather instead of atomic adds
 static int launch_scale(bool fine_to_coarse, const vf_lbm_parameter* c, const vf_lbm_parameter* f,
@@ -438,6 +472,34 @@ extern "C" int vf_ScaleCoarseToFineCompressibleAdjointGather(const vf_lbm_parame
         return VF_ERR_INVALID_ARGUMENT;
     }
     return launch_scale(false, coarse, fine, coarseToFine, offsets, stream, coarseAdjoint, fineAdjoint, sources, workspace);
+}
+
+extern "C" int vf_ScaleCoarseToFineCompressibleAdjointGatherNoAlloc(const vf_lbm_parameter* coarse,
+                                                                    const vf_lbm_parameter* fine,
+                                                                    const vf_interpolation_cells* coarseToFine,
+                                                                    const vf_interpolation_cell_neighbor* offsets,
+                                                                    const vf_interpolation_sources* sources,
+                                                                    float* coarseAdjoint, float* fineAdjoint,
+                                                                    void* stream)
+{
+    if (!sources || !coarseToFine) {
+        set_last_error("grid scaling adjoint gather needs the interface cells and their source nodes");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (coarseToFine->numberOfCells > kNoAllocCells) {
+        set_last_error("no-alloc grid scaling adjoint: at most 65536 interface cells (the library's workspace)");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    if (!coarseAdjoint || !fineAdjoint) {
+        set_last_error("grid scaling adjoint needs the adjoint buffers of both levels");
+        return VF_ERR_INVALID_ARGUMENT;
+    }
+    float* work = nullptr;
+    const int rc = scratch_address(&work);
+    if (rc != VF_OK) return rc;
+    // every lane of every cell overwrites its 9 workspace floats before the gather reads them:
+    // nothing needs clearing between calls
+    return launch_scale(false, coarse, fine, coarseToFine, offsets, stream, coarseAdjoint, fineAdjoint, sources, work);
 }
 
 extern "C" int vf_ScaleFineToCoarseCompressibleAdjoint(const vf_lbm_parameter* coarse, const vf_lbm_parameter* fine,
