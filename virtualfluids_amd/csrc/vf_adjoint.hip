@@ -442,8 +442,10 @@ void launch_box(const vf_lbm_parameter& p, float* lam, float* shift, float* padj
         return;
     }
     static_assert(kAdjBlock == kPlaneBlock, "the scalar adjoint kernel takes plane-flat workgroups");
+    // XCD block ranges only where pinned: the adjoint's three streams per direction ran slower
+    // with them (sphere channel 125 -> 151 us, 258 x 130 x 130 FP16 227 -> 284 us)
     launch_scalar<S, EVEN>(a, r, f, lam, mask, shift != nullptr, dim3(plane_blocks(p.nx, p.ny), p.nz), s,
-                           xcd_planes(p.nx, p.ny, sizeof(S), kc.rowXcdRanges));
+                           kc.rowXcdRanges == 1);
 }
 
 template <class S, bool EVEN>

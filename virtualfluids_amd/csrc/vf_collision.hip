@@ -562,7 +562,9 @@ static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
     if (try_launch_row<S, EVEN>(p, a, r, macro, nullptr, s)) return;
     const dim3 grid(plane_blocks(p.nx, p.ny), p.nz), bx(kBoxBlock);
     const FusedBB none{};
-    const bool xcd = xcd_planes(p.nx, p.ny, sizeof(S), p.kernelChoice.rowXcdRanges);
+    // XCD block ranges only where pinned: in place they gain nothing inside a multi-level step
+    // (258 x 130 x 130 FP16 107 against 104-109 us) and cost the sphere channel 59 -> 61 us
+    const bool xcd = p.kernelChoice.rowXcdRanges == 1;
     if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r, none);
     else if (mask && xcd) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false, false, true>), grid, dim3(bx), 0, s, f, a, r, none);
     else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r, none);

@@ -48,10 +48,10 @@ inline uint32_t xcd_rows(int nx, size_t elemBytes, int choice)
     return ((size_t)nx * elemBytes) % 128 != 0;
 }
 
-// host: the same for the plane-flat one-node-per-thread kernels, whose workgroups take
-// consecutive spans of a plane: spans share their end lines with the next one where a plane's
-// bytes are not a whole number of lines (258 x 130 x 130 FP16 on MI355X: 88 -> 84 us in place,
-// 140 -> 125 us out of place; FP32 204 -> 192 and 230 -> 212 us)
+// host: the same for the plane-flat out-of-place kernel, whose workgroups take consecutive spans
+// of a plane: spans share their end lines with the next one where a plane's bytes are not a
+// whole number of lines (on MI355X, 258 x 130 x 130 FP16 out of place 140 -> 125 us alone and
+// 121-130 -> 118 us inside the two-level step; the sphere channel's 194 x 98 x 98 FP32 92 -> 87)
 inline uint32_t xcd_planes(int nx, int ny, size_t elemBytes, int choice)
 {
     if (choice == 1 || choice == 2) return choice == 1;
