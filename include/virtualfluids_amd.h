@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define VF_ABI_VERSION 5
+#define VF_ABI_VERSION 6
 
 enum vf_status {
     VF_OK = 0,
@@ -165,6 +165,13 @@ typedef struct vf_q_boundary_conditions {
      * adjoints then read the linearisation point from it instead of recomputing each BC node's
      * collision. */
     float* postCollision;
+    /* Layout hint for the adjoint kernels (no counterpart in the reference; ABI 6): nonzero when
+     * consecutive entries of k are, for the most part, neighbours in x (walls normal to y or z).
+     * The adjoints then give each BC node one thread, so that the loads of a direction are
+     * contiguous; zero gives each node a lane per direction, which keeps the scattered cells of a
+     * face normal to x in flight together. Either value gives the same adjoint up to the order of
+     * the per-node sums. */
+    int32_t nodesAlongRows;
 } vf_q_boundary_conditions;
 
 /* ICells / ICellNeigh (src/gpu/core/Calculation/Calculation.h:72-89) */

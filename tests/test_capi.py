@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
     lib = capi.load()
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.vf_abi_version() == 5
+    assert lib.vf_abi_version() == capi.ABI_VERSION == 6
 
 
 def test_signature_table_covers_header():

@@ -202,7 +202,8 @@ def _duct(q_random, seed=47):
 @pytest.mark.parametrize("even", [True, False])
 @pytest.mark.parametrize("box", [True, False])  # (x, y, z) addressing or neighbour arrays
 @pytest.mark.parametrize("saved", [False, True])  # recomputed or stored post-collision values
-def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box, saved):
+@pytest.mark.parametrize("rows", [0, 1])  # layout hint: a lane per direction, or a thread per node
+def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box, saved, rows):
     import torch
 
     from virtualfluids_amd import capi
@@ -220,6 +221,7 @@ def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box, saved
     lam0 = rng.normal(0, 1, 27 * g.N).astype(np.float32)
     lam = torch.from_numpy(lam0).cuda()
     bc = L.bcs[bc_name]
+    bc.struct.nodesAlongRows = rows
     ubar = torch.zeros(3 * bc.n, dtype=torch.float32, device="cuda")
     post = None
     if saved:
@@ -229,7 +231,7 @@ def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box, saved
         post = torch.full((27 * bc.n,), float("nan"), dtype=torch.float32, device="cuda")
         capi.call(fn, L.pref(), ctypes.byref(_bc_struct(bc, post)), None)
         L.f.copy_(before)
-    print(f"{fn}Adjoint even={even} nodes={bc.n} saved={saved}", flush=True)
+    print(f"{fn}Adjoint even={even} nodes={bc.n} saved={saved} rows={rows}", flush=True)
     capi.call(fn + "Adjoint", L.pref(), ctypes.byref(_bc_struct(bc, post)), ctypes.c_void_p(lam.data_ptr()),
               ctypes.c_void_p(ubar.data_ptr()), None)
     got = lam.cpu().numpy().astype(np.float64)
@@ -254,7 +256,8 @@ def test_post_bc_adjoint_kernel_matches_reference(gpu, orc, fn, even, box, saved
 @pytest.mark.parametrize("even", [True, False])
 @pytest.mark.parametrize("box", [True, False])
 @pytest.mark.parametrize("distinct", [1, 0])  # plain read-modify-writes, or atomic adds
-def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box, distinct):
+@pytest.mark.parametrize("rows", [0, 1])  # layout hint: a lane per direction, or a thread per node
+def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box, distinct, rows):
     import torch
 
     from virtualfluids_amd import capi
@@ -269,6 +272,7 @@ def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box, disti
     lam = torch.from_numpy(lam0).cuda()
     bc = L.bcs["pressure"]
     assert bc.kN_distinct
+    bc.struct.nodesAlongRows = rows
     rbar = torch.zeros(bc.n, dtype=torch.float32, device="cuda")
     capi.call("vf_PressureNonEquilibriumCompressibleAdjoint", L.pref(), ctypes.byref(bc.struct),
               ctypes.c_void_p(lam.data_ptr()), ctypes.c_void_p(rbar.data_ptr()), distinct, None)
@@ -280,7 +284,7 @@ def test_pressure_bc_adjoint_kernel_matches_reference(gpu, orc, even, box, disti
     pr = g.bcs["pressure"]
     orc.bc_pressure_adjoint(A, ref, pr.k, pr.kN, pr.rho_bc, pr.direction, rhobar=ref_r)
     err = np.abs(lam.cpu().numpy() - ref).max() / np.abs(ref).max()
-    print(f"pressure adjoint even={even}: rel err {err:.2e}")
+    print(f"pressure adjoint even={even} rows={rows}: rel err {err:.2e}")
     assert err < 2e-5
     assert np.abs(rbar.cpu().numpy() - ref_r).max() < 1e-4 * np.abs(ref_r).max()
 

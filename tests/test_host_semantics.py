@@ -79,6 +79,26 @@ def test_pressure_list_records_whether_its_neighbours_are_distinct():
     assert not shared.kN_distinct
 
 
+def test_bc_list_layout_hint_follows_the_nodes():
+    """nodesAlongRows (one thread per node in the adjoints) for lists along x rows, not for faces
+    normal to x; a list given twice is refused before any hint is formed."""
+    from virtualfluids_amd.lattice import DeviceBC, nodes_along_rows
+
+    x, y, z = np.meshgrid(np.arange(8), np.arange(6), np.arange(5), indexing="ij")
+    k = x + 8 * (y + 6 * z)
+    wall = np.sort(k[:, 0, :].ravel())   # y = 0: consecutive in x
+    face = np.sort(k[0, :, :].ravel())   # x = 0: a stride of 8 between neighbours
+    assert nodes_along_rows(wall) and not nodes_along_rows(face)
+    assert not nodes_along_rows(np.array([3])) and not nodes_along_rows(np.array([], dtype=np.int64))
+    q = np.full((27, wall.size), -1.0)
+    bc = DeviceBC(BoundaryNodes(k=wall, q27=q), "cpu", 240)
+    assert bc.struct.nodesAlongRows == 1
+    bc = DeviceBC(BoundaryNodes(k=face, q27=np.full((27, face.size), -1.0)), "cpu", 240)
+    assert bc.struct.nodesAlongRows == 0
+    with pytest.raises(ValueError, match="listed twice"):
+        DeviceBC(BoundaryNodes(k=np.array([4, 5, 4]), q27=np.full((27, 3), -1.0)), "cpu", 240)
+
+
 def test_interface_sources_list_every_cell_corner_once():
     """vf_interpolation_sources built on the host: every lane 8 i + q appears once, under the node
     that is corner q of coarse cell i (+x, +y, +z steps through the neighbour arrays)."""

@@ -102,6 +102,7 @@ class QBoundaryConditions(ctypes.Structure):
         ("RhoBC", _vp),
         ("direction", ctypes.c_int32),
         ("postCollision", _vp),
+        ("nodesAlongRows", ctypes.c_int32),
     ]
 
 
@@ -133,7 +134,7 @@ class InterpolationSources(ctypes.Structure):
 
 
 # VF_ABI_VERSION of include/virtualfluids_amd.h that these structs mirror
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _P = ctypes.POINTER(LBMParameter)
 _Q = ctypes.POINTER(QBoundaryConditions)

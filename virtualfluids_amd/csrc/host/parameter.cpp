@@ -171,6 +171,10 @@ void DeviceLevel::setBC(vf_q_boundary_conditions& bc, const HostBC& host)
     bc.Vz = opt(host.Vz);
     bc.RhoBC = opt(host.RhoBC);
     bc.direction = host.direction;
+    // layout hint of the adjoint kernels: consecutive nodes mostly neighbours in x
+    size_t along = 0;
+    for (size_t i = 1; i < n; ++i) along += host.k[i] == host.k[i - 1] + 1;
+    bc.nodesAlongRows = n > 1 && 2 * along >= n - 1;
 }
 
 void DeviceLevel::setCells(InterpolationCells& cells, ICellNeigh& neigh, bool& hasOffsets, const HostCells& host,

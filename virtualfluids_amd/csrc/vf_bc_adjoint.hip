@@ -7,11 +7,11 @@
 // adjoint of the node's post-collision cells, and clears W (its previous content was
 // overwritten). The pressure BC (pre-collision, after the swap) writes the BC node's cells from
 // the neighbour kN; its adjoint moves the written cells' adjoint to kN's cells. Linearisation
-// points: the pre-collision state of the step for the post BCs (each BC node's collision is
-// recomputed from it, so no post-collision copy of the lattice is kept), the state after the
-// step for the pressure BC, with the parity each kernel used. Written cells are assumed to belong to nodes that no
-// other BC node of the same list reads (links end outside the fluid; kN is not itself a
-// pressure-BC node), which is how the grid builders lay boundaries out.
+// points: for the post BCs the post-collision values the forward kernel stored (or, without
+// them, each BC node's collision recomputed from the pre-collision state of the step), the state
+// after the step for the pressure BC, with the parity each kernel used. Written cells are
+// assumed to belong to nodes that no other BC node of the same list reads (links end outside the
+// fluid; kN is not itself a pressure-BC node), which is how the grid builders lay boundaries out.
 #include "vf_bc_adjoint.h"
 #include "vf_lattice.h"
 
@@ -38,8 +38,9 @@ __device__ __forceinline__ float* lam_cell(float* lam, const Addr& a, const uint
     return slot_ptr(lam, a.N, d, even) + n[node_sel(d)];
 }
 
-// SAVED: the node's post-collision values come from b.post (stored by the forward kernel);
-// otherwise they are recomputed from its pre-collision values in f
+// One thread per node. SAVED: the node's post-collision values come from b.post (stored by the
+// forward kernel; lists along x rows, whose loads of a direction are contiguous); otherwise they
+// are recomputed from its pre-collision values in f
 template <class S, int KIND, bool SAVED>
 __global__ __launch_bounds__(256) void bc_post_adjoint_kernel(const S* __restrict__ f, float* __restrict__ lam, Addr a,
                                                               BcAdjArgs b, int even_flag)
@@ -87,6 +88,155 @@ __global__ __launch_bounds__(256) void bc_post_adjoint_kernel(const S* __restric
     }
 }
 
+// ---- lane-per-direction forms ---------------------------------------------------------------
+// 32 lanes per BC node (lane d < 27 takes direction d; lanes 27..31 hold zeros but take part in
+// every shuffle), 8 nodes per 256-thread workgroup, for lists whose consecutive nodes are not
+// neighbours in x (faces normal to x: every node and direction is a cache line of its own). One
+// thread per node leaves such a face of ten thousand nodes at a few waves per CU, each waiting on
+// 27-deep chains of scattered loads; here the scattered loads of a node are one load per lane
+// and 32 times as many waves are in flight. Lists along x rows (walls normal to y or z) keep one
+// thread per node, whose loads of a direction are contiguous. The node sums (density, momentum,
+// and the adjoint sums that reach all 27 directions) are butterflies over the lane group.
+constexpr int kLaneGroup = 32;
+constexpr int kLaneBlock = 256;
+constexpr int kLaneNodes = kLaneBlock / kLaneGroup;
+
+constexpr uint32_t dir_bits(int axis, int sign)
+{
+    uint32_t m = 0;
+    for (int d = 0; d < 27; ++d)
+        if ((axis == 0 ? cx(d) : (axis == 1 ? cy(d) : cz(d))) == sign) m |= 1u << d;
+    return m;
+}
+
+struct LaneDir {
+    int d;       // direction of this lane, valid for d < 27
+    bool valid;
+    int cx, cy, cz;
+    int r;       // inv(d)
+    float w;     // weight(d), 0 for the idle lanes
+};
+
+__device__ __forceinline__ LaneDir lane_dir()
+{
+    LaneDir L;
+    L.d = (int)(threadIdx.x & (kLaneGroup - 1));
+    L.valid = L.d < 27;
+    const uint32_t bit = L.valid ? 1u << L.d : 0u;
+    L.cx = int((dir_bits(0, 1) & bit) != 0) - int((dir_bits(0, -1) & bit) != 0);
+    L.cy = int((dir_bits(1, 1) & bit) != 0) - int((dir_bits(1, -1) & bit) != 0);
+    L.cz = int((dir_bits(2, 1) & bit) != 0) - int((dir_bits(2, -1) & bit) != 0);
+    L.r = L.d == 0 ? 0 : (L.d <= 18 ? ((L.d & 1) ? L.d + 1 : L.d - 1) : 45 - L.d);
+    L.w = !L.valid ? 0.0f : (L.d == 0 ? 8.0f / 27.0f : (L.d <= 6 ? 2.0f / 27.0f : (L.d <= 18 ? 1.0f / 54.0f : 1.0f / 216.0f)));
+    return L;
+}
+
+__device__ __forceinline__ float group_sum(float v)
+{
+#pragma unroll
+    for (int m = kLaneGroup / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, kLaneGroup);
+    return v;
+}
+
+// node k moved by +1 along the axes whose flag is set (the cell node that holds a direction)
+__device__ __forceinline__ uint32_t moved_node(const Addr& a, uint32_t k, bool sx, bool sy, bool sz)
+{
+    if (a.nbx) {
+        if (sx) k = a.nbx[k];
+        if (sy) k = a.nby[k];
+        if (sz) k = a.nbz[k];
+        return k;
+    }
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny, nz = (uint32_t)a.nz;
+    uint32_t x = k % nx, t = k / nx, y = t % ny, z = t / ny;
+    if (sx) x = x + 1 == nx ? 0 : x + 1;
+    if (sy) y = y + 1 == ny ? 0 : y + 1;
+    if (sz) z = z + 1 == nz ? 0 : z + 1;
+    return x + nx * (y + ny * z);
+}
+
+// The post BC adjoint from stored post-collision values, a lane per direction: the arithmetic of
+// bc_post_vjp, the node sums taken over the lane group. Lane d reads the written cell of link d
+// (the node's incoming inv(d) of the next step) and the cell of the node's post-collision value
+// d; both lie at the cell node moved by +1 where c_d > 0, in slots d and inv(d).
+template <int KIND>
+__global__ __launch_bounds__(kLaneBlock) void bc_post_adjoint_lanes_kernel(float* __restrict__ lam, Addr a, BcAdjArgs b,
+                                                                           int even_flag)
+{
+    constexpr bool interp = KIND >= kNoSlipInterp;
+    constexpr bool velocity = KIND == kVelocityBB || KIND == kVelocityInterp || KIND == kVelocityPressureInterp;
+    const uint32_t i = blockIdx.x * kLaneNodes + (threadIdx.x / kLaneGroup);
+    const bool node = i < b.nbc; // every lane of the group stays for the shuffles
+    const LaneDir L = lane_dir();
+    const bool on = node && L.valid;
+    const bool even = even_flag != 0;
+    float post = 0.0f, qd = -1.0f;
+    float* wcell = nullptr;
+    float* pcell = nullptr;
+    if (on) {
+        post = b.post[(uint64_t)L.d * b.nbc + i];
+        if (L.d != 0) qd = b.q[(uint64_t)L.d * b.nbc + i];
+        const uint32_t m = moved_node(a, (uint32_t)b.k[i], L.cx > 0, L.cy > 0, L.cz > 0);
+        wcell = lam + (uint64_t)(even ? L.d : L.r) * a.N + m;
+        pcell = lam + (uint64_t)(even ? L.r : L.d) * a.N + m;
+    }
+    const bool active = on && qd >= 0.0f && qd <= 1.0f;
+    float g = 0.0f;
+    if (active) {
+        g = *wcell; // the adjoint of the written value; its old content never reached the output
+        *wcell = 0.0f;
+    }
+    float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    if (interp) {
+        drho = group_sum(post);
+        jx = group_sum(float(L.cx) * post);
+        jy = group_sum(float(L.cy) * post);
+        jz = group_sum(float(L.cz) * post);
+    }
+    float own = 0.0f, other = 0.0f, drhob = 0.0f, jxb = 0.0f, jyb = 0.0f, jzb = 0.0f;
+    float wallFactor = 6.0f * L.w; // d value / d (c.U) = -wallFactor
+    if (active) {
+        if (!interp) {
+            own = g;
+        } else {
+            const float ir = 1.0f / (1.0f + drho);
+            const float j2 = jx * jx + jy * jy + jz * jz;
+            const float om1 = 1.0f / (1.0f - b.omega);
+            const float A = 0.5f * (1.0f - qd) / (1.0f + qd), Bq = 1.0f / (1.0f + qd);
+            own = g * (A + A * om1 + Bq * qd);
+            other = g * (-A + A * om1 + Bq * qd); // adjoint of the post-collision value inv(d)
+            const float cj = float(L.cx) * jx + float(L.cy) * jy + float(L.cz) * jz;
+            const float Q = 4.5f * cj * cj - 1.5f * j2;
+            const float feqb = g * (-2.0f * A * b.omega * om1);
+            drhob = feqb * L.w * (1.0f - Q * ir * ir);
+            const float t = feqb * L.w * ir;
+            jxb = t * (9.0f * cj * float(L.cx) - 3.0f * jx);
+            jyb = t * (9.0f * cj * float(L.cy) - 3.0f * jy);
+            jzb = t * (9.0f * cj * float(L.cz) - 3.0f * jz);
+            if (KIND == kVelocityPressureInterp) drhob -= g * L.w;
+            wallFactor *= Bq;
+        }
+    }
+    float fb = own;
+    if (interp) {
+        fb += __shfl(other, L.valid ? L.r : L.d, kLaneGroup);
+        fb += group_sum(drhob) + float(L.cx) * group_sum(jxb) + float(L.cy) * group_sum(jyb) +
+              float(L.cz) * group_sum(jzb);
+    }
+    // the adjoint of the node's post-collision value d moves to its cell (bounce-back kinds only
+    // touch their active links)
+    if (on && (interp || active)) *pcell += fb;
+    if (velocity && b.ubar) {
+        const float gw = active ? -wallFactor * g : 0.0f;
+        const float Ux = group_sum(gw * float(L.cx)), Uy = group_sum(gw * float(L.cy)), Uz = group_sum(gw * float(L.cz));
+        if (node && L.d == 0) {
+            b.ubar[i] += Ux;
+            b.ubar[b.nbc + i] += Uy;
+            b.ubar[2 * b.nbc + i] += Uz;
+        }
+    }
+}
+
 struct PressureAdjArgs {
     const int32_t* k;
     const int32_t* kN;
@@ -99,8 +249,8 @@ struct PressureAdjArgs {
 // DISTINCT: no two BC nodes share kN, so each neighbour cell has one writer (plain read-modify-
 // write; atomics cost 35 against 20 us on a 96 x 96 face of FP32 cells on MI355X)
 template <class S, bool DISTINCT>
-__global__ __launch_bounds__(256) void pressure_adjoint_kernel(const S* __restrict__ f, float* __restrict__ lam, Addr a,
-                                                               PressureAdjArgs b, int even_flag)
+__global__ __launch_bounds__(kBcBlock) void pressure_adjoint_kernel(const S* __restrict__ f, float* __restrict__ lam,
+                                                                    Addr a, PressureAdjArgs b, int even_flag)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= b.nbc) return;
@@ -138,6 +288,57 @@ __global__ __launch_bounds__(256) void pressure_adjoint_kernel(const S* __restri
     if (b.rhobar) b.rhobar[i] += rb;
 }
 
+// The pressure BC adjoint with a lane per direction (the arithmetic of pressure_vjp) for faces
+// normal to x: lane d reads the neighbour's pre-collision value d and, where the face writes
+// direction d, the adjoint of the BC node's cell d, which it then clears. DISTINCT as above.
+template <class S, bool DISTINCT>
+__global__ __launch_bounds__(kLaneBlock) void pressure_adjoint_lanes_kernel(const S* __restrict__ f,
+                                                                            float* __restrict__ lam, Addr a,
+                                                                            PressureAdjArgs b, int even_flag)
+{
+    const uint32_t i = blockIdx.x * kLaneNodes + (threadIdx.x / kLaneGroup);
+    const bool node = i < b.nbc;
+    const LaneDir L = lane_dir();
+    const bool on = node && L.valid;
+    const bool even = even_flag != 0;
+    const int c = b.axis == 0 ? L.cx : (b.axis == 1 ? L.cy : L.cz);
+    const bool written = on && c * b.sign <= 0;
+    float fn = 0.0f, G = 0.0f, rhoBC = 0.0f;
+    float* ncell = nullptr;
+    if (on) {
+        const uint64_t slot = (uint64_t)(even ? L.d : L.r) * a.N;
+        const bool sx = L.cx < 0, sy = L.cy < 0, sz = L.cz < 0;
+        const uint32_t kn = moved_node(a, (uint32_t)b.kN[i], sx, sy, sz);
+        fn = Codec<S>::load(f + slot + kn);
+        ncell = lam + slot + kn;
+        rhoBC = b.rhoBC[i];
+        if (written) {
+            float* gcell = lam + slot + moved_node(a, (uint32_t)b.k[i], sx, sy, sz);
+            G = *gcell;
+            *gcell = 0.0f;
+        }
+    }
+    const float drho = group_sum(fn);
+    const float jx = group_sum(float(L.cx) * fn), jy = group_sum(float(L.cy) * fn), jz = group_sum(float(L.cz) * fn);
+    const float ir = 1.0f / (drho + 1.0f);
+    const float ux = jx * ir, uy = jy * ir, uz = jz * ir;
+    const float usq = ux * ux + uy * uy + uz * uz;
+    const float cu = float(L.cx) * ux + float(L.cy) * uy + float(L.cz) * uz;
+    const float E = 3.0f * cu + 4.5f * cu * cu - 1.5f * usq;
+    const float t = G * L.w;
+    const float A1 = group_sum(t * (1.0f + E));
+    const float s = (rhoBC - drho) * ir;
+    const float Vx = s * group_sum(t * (3.0f * float(L.cx) + 9.0f * cu * float(L.cx) - 3.0f * ux));
+    const float Vy = s * group_sum(t * (3.0f * float(L.cy) + 9.0f * cu * float(L.cy) - 3.0f * uy));
+    const float Vz = s * group_sum(t * (3.0f * float(L.cz) + 9.0f * cu * float(L.cz) - 3.0f * uz));
+    if (on) {
+        const float fnb = G - A1 + Vx * (float(L.cx) - ux) + Vy * (float(L.cy) - uy) + Vz * (float(L.cz) - uz);
+        if constexpr (DISTINCT) *ncell += fnb;
+        else atomicAdd(ncell, fnb);
+        if (b.rhobar && L.d == 0) b.rhobar[i] += A1;
+    }
+}
+
 template <template <class> class K, class... A> int by_storage(int storage, A&&... args)
 {
     switch (storage) {
@@ -150,10 +351,13 @@ template <template <class> class K, class... A> int by_storage(int storage, A&&.
 
 template <int KIND> struct PostAdjL {
     template <class S> struct L {
-        static void run(const vf_lbm_parameter& p, const BcAdjArgs& b, float* lam, hipStream_t s)
+        static void run(const vf_lbm_parameter& p, const BcAdjArgs& b, bool rows, float* lam, hipStream_t s)
         {
             const dim3 grid((b.nbc + kBcBlock - 1) / kBcBlock);
-            if (b.post)
+            if (b.post && !rows)
+                hipLaunchKernelGGL((bc_post_adjoint_lanes_kernel<KIND>), dim3((b.nbc + kLaneNodes - 1) / kLaneNodes),
+                                   dim3(kLaneBlock), 0, s, lam, make_addr(p), b, p.isEvenTimestep);
+            else if (b.post)
                 hipLaunchKernelGGL((bc_post_adjoint_kernel<S, KIND, true>), grid, dim3(kBcBlock), 0, s,
                                    (const S*)p.distributions, lam, make_addr(p), b, p.isEvenTimestep);
             else
@@ -164,15 +368,26 @@ template <int KIND> struct PostAdjL {
 };
 
 template <class S> struct PressAdjL {
-    static void run(const vf_lbm_parameter& p, const PressureAdjArgs& b, float* lam, bool distinct, hipStream_t s)
+    static void run(const vf_lbm_parameter& p, const PressureAdjArgs& b, bool rows, float* lam, bool distinct, hipStream_t s)
     {
-        const dim3 grid((b.nbc + kBcBlock - 1) / kBcBlock);
+        const Addr a = make_addr(p);
+        if (rows) {
+            const dim3 grid((b.nbc + kBcBlock - 1) / kBcBlock);
+            if (distinct)
+                hipLaunchKernelGGL((pressure_adjoint_kernel<S, true>), grid, dim3(kBcBlock), 0, s, (const S*)p.distributions,
+                                   lam, a, b, p.isEvenTimestep);
+            else
+                hipLaunchKernelGGL((pressure_adjoint_kernel<S, false>), grid, dim3(kBcBlock), 0, s,
+                                   (const S*)p.distributions, lam, a, b, p.isEvenTimestep);
+            return;
+        }
+        const dim3 grid((b.nbc + kLaneNodes - 1) / kLaneNodes);
         if (distinct)
-            hipLaunchKernelGGL((pressure_adjoint_kernel<S, true>), grid, dim3(kBcBlock), 0, s, (const S*)p.distributions, lam,
-                               make_addr(p), b, p.isEvenTimestep);
+            hipLaunchKernelGGL((pressure_adjoint_lanes_kernel<S, true>), grid, dim3(kLaneBlock), 0, s,
+                               (const S*)p.distributions, lam, a, b, p.isEvenTimestep);
         else
-            hipLaunchKernelGGL((pressure_adjoint_kernel<S, false>), grid, dim3(kBcBlock), 0, s, (const S*)p.distributions,
-                               lam, make_addr(p), b, p.isEvenTimestep);
+            hipLaunchKernelGGL((pressure_adjoint_lanes_kernel<S, false>), grid, dim3(kLaneBlock), 0, s,
+                               (const S*)p.distributions, lam, a, b, p.isEvenTimestep);
     }
 };
 
@@ -211,7 +426,7 @@ int launch_post_adjoint(const vf_lbm_parameter* p, const vf_q_boundary_condition
     }
     const BcAdjArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega, velocity ? ubar : nullptr,
                       make_k17_rates(*p), p->porosity, bc->postCollision};
-    rc = by_storage<PostAdjL<KIND>::template L>(p->storage, *p, b, lam, (hipStream_t)stream);
+    rc = by_storage<PostAdjL<KIND>::template L>(p->storage, *p, b, bc->nodesAlongRows != 0, lam, (hipStream_t)stream);
     return rc ? rc : check_launch(name);
 }
 
@@ -271,6 +486,7 @@ extern "C" int vf_PressureNonEquilibriumCompressibleAdjoint(const vf_lbm_paramet
     case d00M: b.axis = 2; b.sign = -1; break;
     default: return VF_OK; // the forward kernel writes nothing for other directions
     }
-    rc = by_storage<PressAdjL>(p->storage, *p, b, adjoint, neighborsDistinct != 0, (hipStream_t)stream);
+    rc = by_storage<PressAdjL>(p->storage, *p, b, bc->nodesAlongRows != 0, adjoint, neighborsDistinct != 0,
+                               (hipStream_t)stream);
     return rc ? rc : check_launch("PressureNonEquilibriumCompressible adjoint");
 }

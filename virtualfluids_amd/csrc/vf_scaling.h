@@ -25,35 +25,51 @@ struct SrcMom {
     float drho, vx, vy, vz, kxy, kyz, kxz, kxxMyy, kxxMzz;
 };
 
-// MomentsOnSourceNode::calculate (InterpolationCoefficients.h:86-144) from 27 populations
-VF_HD SrcMom source_moments_of(const float (&f)[27], float omega)
+// The nine population sums a source node's moments come from
+struct SrcSums {
+    float drho, jx, jy, jz, pxy, pyz, pxz, pxxyy, pxxzz;
+};
+
+VF_HD SrcSums source_sums_of(const float (&f)[27])
 {
-    float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f, pxy = 0.f, pyz = 0.f, pxz = 0.f, pxxyy = 0.f, pxxzz = 0.f;
+    SrcSums s{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
         const float v = f[d];
-        drho += v;
-        if (cx(d)) jx += float(cx(d)) * v;
-        if (cy(d)) jy += float(cy(d)) * v;
-        if (cz(d)) jz += float(cz(d)) * v;
-        if (cx(d) * cy(d)) pxy += float(cx(d) * cy(d)) * v;
-        if (cy(d) * cz(d)) pyz += float(cy(d) * cz(d)) * v;
-        if (cx(d) * cz(d)) pxz += float(cx(d) * cz(d)) * v;
-        if (cx(d) * cx(d) - cy(d) * cy(d)) pxxyy += float(cx(d) * cx(d) - cy(d) * cy(d)) * v;
-        if (cx(d) * cx(d) - cz(d) * cz(d)) pxxzz += float(cx(d) * cx(d) - cz(d) * cz(d)) * v;
+        s.drho += v;
+        if (cx(d)) s.jx += float(cx(d)) * v;
+        if (cy(d)) s.jy += float(cy(d)) * v;
+        if (cz(d)) s.jz += float(cz(d)) * v;
+        if (cx(d) * cy(d)) s.pxy += float(cx(d) * cy(d)) * v;
+        if (cy(d) * cz(d)) s.pyz += float(cy(d) * cz(d)) * v;
+        if (cx(d) * cz(d)) s.pxz += float(cx(d) * cz(d)) * v;
+        if (cx(d) * cx(d) - cy(d) * cy(d)) s.pxxyy += float(cx(d) * cx(d) - cy(d) * cy(d)) * v;
+        if (cx(d) * cx(d) - cz(d) * cz(d)) s.pxxzz += float(cx(d) * cx(d) - cz(d) * cz(d)) * v;
     }
-    const float ir = 1.0f / (drho + 1.0f);
+    return s;
+}
+
+// MomentsOnSourceNode::calculate (InterpolationCoefficients.h:86-144) from the node's sums
+VF_HD SrcMom source_moments_of_sums(const SrcSums& s, float omega)
+{
+    const float ir = 1.0f / (s.drho + 1.0f);
     SrcMom m;
-    m.drho = drho;
-    m.vx = jx * ir;
-    m.vy = jy * ir;
-    m.vz = jz * ir;
-    m.kxy = -3.0f * omega * (pxy * ir - m.vx * m.vy);
-    m.kyz = -3.0f * omega * (pyz * ir - m.vy * m.vz);
-    m.kxz = -3.0f * omega * (pxz * ir - m.vx * m.vz);
-    m.kxxMyy = -1.5f * omega * (pxxyy * ir - (m.vx * m.vx - m.vy * m.vy));
-    m.kxxMzz = -1.5f * omega * (pxxzz * ir - (m.vx * m.vx - m.vz * m.vz));
+    m.drho = s.drho;
+    m.vx = s.jx * ir;
+    m.vy = s.jy * ir;
+    m.vz = s.jz * ir;
+    m.kxy = -3.0f * omega * (s.pxy * ir - m.vx * m.vy);
+    m.kyz = -3.0f * omega * (s.pyz * ir - m.vy * m.vz);
+    m.kxz = -3.0f * omega * (s.pxz * ir - m.vx * m.vz);
+    m.kxxMyy = -1.5f * omega * (s.pxxyy * ir - (m.vx * m.vx - m.vy * m.vy));
+    m.kxxMzz = -1.5f * omega * (s.pxxzz * ir - (m.vx * m.vx - m.vz * m.vz));
     return m;
+}
+
+// ... and from 27 populations
+VF_HD SrcMom source_moments_of(const float (&f)[27], float omega)
+{
+    return source_moments_of_sums(source_sums_of(f), omega);
 }
 
 struct Coef {
@@ -571,23 +587,12 @@ VF_HD void coefficients_vjp(const SrcMom (&m)[8], float xo, float yo, float zo, 
     for (int n = 0; n < 8; ++n) mb[n] = node_of_sums(Sb, n);
 }
 
-// (d source moments / d populations)^T mb
-VF_HD void source_moments_vjp(const float (&f)[27], float omega, const SrcMom& mb, float (&fb)[27])
+// (d source moments / d populations)^T mb, from the node's sums (the moments are linear in the
+// populations through them, so the sums are all the reverse needs)
+VF_HD void source_moments_vjp_sums(const SrcSums& s, float omega, const SrcMom& mb, float (&fb)[27])
 {
-    float drho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f, pxy = 0.f, pyz = 0.f, pxz = 0.f, pxxyy = 0.f, pxxzz = 0.f;
-#pragma unroll
-    for (int d = 0; d < 27; ++d) {
-        const float v = f[d];
-        drho += v;
-        jx += float(cx(d)) * v;
-        jy += float(cy(d)) * v;
-        jz += float(cz(d)) * v;
-        pxy += float(cx(d) * cy(d)) * v;
-        pyz += float(cy(d) * cz(d)) * v;
-        pxz += float(cx(d) * cz(d)) * v;
-        pxxyy += float(cx(d) * cx(d) - cy(d) * cy(d)) * v;
-        pxxzz += float(cx(d) * cx(d) - cz(d) * cz(d)) * v;
-    }
+    const float drho = s.drho, jx = s.jx, jy = s.jy, jz = s.jz, pxy = s.pxy, pyz = s.pyz, pxz = s.pxz;
+    const float pxxyy = s.pxxyy, pxxzz = s.pxxzz;
     const float ir = 1.0f / (drho + 1.0f);
     const float vx = jx * ir, vy = jy * ir, vz = jz * ir;
     const float w3 = -3.0f * omega, w15 = -1.5f * omega;
@@ -607,6 +612,11 @@ VF_HD void source_moments_vjp(const float (&f)[27], float omega, const SrcMom& m
                 float(cy(d) * cz(d)) * pyzb + float(cx(d) * cz(d)) * pxzb +
                 float(cx(d) * cx(d) - cy(d) * cy(d)) * pxxyyb + float(cx(d) * cx(d) - cz(d) * cz(d)) * pxxzzb;
     }
+}
+
+VF_HD void source_moments_vjp(const float (&f)[27], float omega, const SrcMom& mb, float (&fb)[27])
+{
+    source_moments_vjp_sums(source_sums_of(f), omega, mb, fb);
 }
 
 // ---- whole cells: src[8][27] source populations, g the adjoints of the written populations ---

@@ -29,14 +29,6 @@ __device__ __forceinline__ void node_populations(const S* f, const Addr& a, uint
     for (int d = 0; d < 27; ++d) v[d] = read_pre(f, a, n, d, even);
 }
 
-template <class S>
-__device__ __forceinline__ SrcMom source_moments(const S* f, const Addr& a, uint32_t k, bool even, float omega)
-{
-    float v[27];
-    node_populations(f, a, k, even, v);
-    return source_moments_of(v, omega);
-}
-
 __device__ __forceinline__ uint32_t corner(const Addr& a, uint32_t base, int n)
 {
     uint32_t k = base;
@@ -91,13 +83,20 @@ __device__ __forceinline__ CellLane cell_lane(uint32_t n)
     return c;
 }
 
-// the 8 source moments of this lane's cell: lane q computes corner q, LDS hands them round
+// the 8 source moments of this lane's cell: lane q computes corner q, LDS hands them round;
+// own (optional) receives corner q's population sums, all its reverse needs later
 template <class S>
 __device__ __forceinline__ void cell_source_moments(const S* f, const Addr& a, uint32_t base, const CellLane& cl,
-                                                    bool even, float omega, SrcMom (&m)[8])
+                                                    bool even, float omega, SrcMom (&m)[8], SrcSums* own = nullptr)
 {
     __shared__ SrcMom sm[kScaleBlock];
-    if (cl.valid) sm[threadIdx.x] = source_moments(f, a, corner(a, base, cl.q), even, omega);
+    if (cl.valid) {
+        float v[27];
+        node_populations(f, a, corner(a, base, cl.q), even, v);
+        const SrcSums sums = source_sums_of(v);
+        sm[threadIdx.x] = source_moments_of_sums(sums, omega);
+        if (own) *own = sums;
+    }
     __syncthreads();
     const int g0 = threadIdx.x & ~7;
 #pragma unroll
@@ -180,14 +179,13 @@ __device__ __forceinline__ void coef_group_sum(Coef& c)
     }
 }
 
-// adjoint of source node q's moments back to its 27 cells
-template <class S>
-__device__ __forceinline__ void source_node_adjoint(const S* f, float* lam, const Addr& a, uint32_t k, bool even,
-                                                    float omega, const SrcMom& mb)
+// adjoint of source node k's moments back to its 27 cells (atomic adds: cells share source
+// nodes), from the node's population sums
+__device__ __forceinline__ void source_node_adjoint(float* lam, const Addr& a, uint32_t k, bool even, float omega,
+                                                    const SrcSums& sums, const SrcMom& mb)
 {
-    float v[27], fb[27];
-    node_populations(f, a, k, even, v);
-    source_moments_vjp(v, omega, mb, fb);
+    float fb[27];
+    source_moments_vjp_sums(sums, omega, mb, fb);
     uint32_t nn[8];
     cell_nodes(a, k, nn);
 #pragma unroll
@@ -205,7 +203,8 @@ __global__ __launch_bounds__(kScaleBlock) void scale_cf_adjoint_kernel(const S* 
 {
     const CellLane cl = cell_lane(s.n);
     SrcMom m[8];
-    cell_source_moments(fc, ac, cl.valid ? s.cellsC[cl.i] : 0u, cl, evenC != 0, s.omegaC, m);
+    SrcSums own;
+    cell_source_moments(fc, ac, cl.valid ? s.cellsC[cl.i] : 0u, cl, evenC != 0, s.omegaC, m, GATHER ? nullptr : &own);
     float xo, yo, zo;
     cell_offsets(s, cl, xo, yo, zo);
     const Coef c = coefficients(m, xo, yo, zo);
@@ -234,7 +233,7 @@ __global__ __launch_bounds__(kScaleBlock) void scale_cf_adjoint_kernel(const S* 
 #pragma unroll
         for (int k = 0; k < 9; ++k) mbOut[k * L + l] = src[k];
     } else {
-        source_node_adjoint(fc, lamC, ac, corner(ac, s.cellsC[cl.i], q), evenC != 0, s.omegaC, mb);
+        source_node_adjoint(lamC, ac, corner(ac, s.cellsC[cl.i], q), evenC != 0, s.omegaC, own, mb);
     }
 }
 
@@ -292,7 +291,8 @@ __global__ __launch_bounds__(kScaleBlock) void scale_fc_adjoint_kernel(const S* 
 {
     const CellLane cl = cell_lane(s.n);
     SrcMom m[8];
-    cell_source_moments(ff, af, cl.valid ? s.cellsF[cl.i] : 0u, cl, true, s.omegaF, m);
+    SrcSums own;
+    cell_source_moments(ff, af, cl.valid ? s.cellsF[cl.i] : 0u, cl, true, s.omegaF, m, &own);
     // every lane reads the written coarse node's adjoint; after the barrier lane q clears its
     // slice of the 27 cells
     uint32_t nc[8];
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kScaleBlock) void scale_fc_adjoint_kernel(const S* 
     Coef cb = coef_zero();
     fc_node_moments_vjp(c, s.omegaC, rebuild_vjp(fc_node_moments(c, s.omegaC), g), cb);
     const SrcMom mb = node_of_sums(coefficients_vjp_sums(m, xo, yo, zo, cb), cl.q);
-    source_node_adjoint(ff, lamF, af, corner(af, s.cellsF[cl.i], cl.q), true, s.omegaF, mb);
+    source_node_adjoint(lamF, af, corner(af, s.cellsF[cl.i], cl.q), true, s.omegaF, own, mb);
 }
 
 static int check_level(const vf_lbm_parameter* p)

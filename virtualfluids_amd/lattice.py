@@ -40,6 +40,13 @@ def stream_handle(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
+def nodes_along_rows(k) -> bool:
+    """Whether consecutive BC nodes are, for the most part, neighbours in x (a wall normal to y or
+    z): the layout hint vf_q_boundary_conditions.nodesAlongRows of the adjoint kernels."""
+    k = np.asarray(k, dtype=np.int64)
+    return k.size > 1 and 2 * int(np.count_nonzero(np.diff(k) == 1)) >= k.size - 1
+
+
 class DeviceBC:
     """A boundary-condition node list resident on the device (QforBoundaryConditions)."""
 
@@ -72,6 +79,7 @@ class DeviceBC:
             s.Vx, s.Vy, s.Vz = (_ptr(v) for v in self.vel)
         s.RhoBC = _ptr(self.rho_bc)
         s.direction = self.direction
+        s.nodesAlongRows = int(nodes_along_rows(k))
 
 
 class Lattice:
