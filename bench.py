@@ -96,14 +96,16 @@ def time_gradient(L, steps, checkpoint_every):
     return dt
 
 
-def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, omega=1.8):
+def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, omega=1.8, keep_fraction=0.6):
     """Channel flow past a sphere at full size (192 x 96 x 96 fluid nodes, sphere of radius 12
     at (48, 48, 48), velocity inflow 0.04, pressure outlet, no-slip walls, interpolated
     bounce-back on the sphere) with a seeded Gaussian random porosity field in [0, 0.1]
     (correlation length 8). Objective: the x drag by momentum exchange averaged over the last
     `average_last` of `steps` steps; its gradient with respect to the porosity field through
     k17_steps, checkpointing every `checkpoint_every` steps. Times a forward-only run of the
-    same objective and the forward + backward run."""
+    same objective and two forward + backward runs: every segment replayed in the backward, and
+    with keep_bytes = keep_fraction of the free device memory, where the trailing segments that
+    fit keep their states from the forward (the same gradient bit for bit)."""
     import numpy as np
     import torch
 
@@ -127,9 +129,9 @@ def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, 
     # x drag by momentum exchange on the sphere's links, averaged over the window
     drag = MomentumExchangeObservation(g, "geometry", axis=0, scale=1.0 / average_last)
 
-    def objective(phi, n=steps, observe_from=spin_up):
+    def objective(phi, n=steps, observe_from=spin_up, keep=0):
         return k17_steps(L.f.detach().clone(), L, n, checkpoint_every=checkpoint_every, even=True,
-                         bc_factory=fac, porosity=phi, observe=drag, observe_from=observe_from)[1]
+                         bc_factory=fac, porosity=phi, observe=drag, observe_from=observe_from, keep_bytes=keep)[1]
 
     phi = torch.from_numpy(phi0.astype(np.float32)).cuda()
     # untimed warm-up over two checkpoint segments, forward and backward: first-use costs (code
@@ -143,20 +145,33 @@ def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, 
         J0 = float(objective(phi))
     torch.cuda.synchronize()
     t_forward = time.perf_counter() - t0
-    phi.requires_grad_(True)
-    t0 = time.perf_counter()
-    J = objective(phi)
-    J.backward()
-    torch.cuda.synchronize()
-    t_gradient = time.perf_counter() - t0
-    grad = phi.grad
+
+    def gradient(keep):
+        p = phi.detach().clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        objective(p, keep=keep).backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, p.grad
+
+    t_recompute, grad_recompute = gradient(0)
+    keep = int(keep_fraction * torch.cuda.mem_get_info()[0])
+    t_gradient, grad = gradient(keep)
+    seg_bytes = checkpoint_every * 4 * 27 * (g.N + sum(bc.n for name, bc in L.bcs.items() if name != "pressure"))
+    kept = min(-(-steps // checkpoint_every), keep // seg_bytes)
     out = {"case": "sphere channel 192x96x96 fluid, r 12, inflow 0.04, Gaussian porosity [0, 0.1] (corr. 8), "
                    f"omega {omega}, {steps} steps, checkpoint every {checkpoint_every}, "
                    f"drag averaged over the last {average_last}",
            "nodes": int(g.N), "drag": round(J0, 6), "grad_l2": float(grad.double().norm()),
            "forward_s": round(t_forward, 3), "gradient_s": round(t_gradient, 3),
-           "gradient_vs_forward": round(t_gradient / t_forward, 3)}
-    del L, phi, J, grad
+           "gradient_vs_forward": round(t_gradient / t_forward, 3),
+           "reverse_vs_forward": round((t_gradient - t_forward) / t_forward, 3),
+           "keep_GB": round(keep / 1e9, 1), "kept_segments": int(kept),
+           "segments": -(-steps // checkpoint_every),
+           "recompute": {"gradient_s": round(t_recompute, 3),
+                         "reverse_vs_forward": round((t_recompute - t_forward) / t_forward, 3)},
+           "kept_equals_recompute": bool(torch.equal(grad, grad_recompute))}
+    del L, phi, grad, grad_recompute
     torch.cuda.empty_cache()
     return out
 

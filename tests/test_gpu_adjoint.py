@@ -471,3 +471,49 @@ def test_sixteen_bit_lattice_takes_and_returns_fp32(gpu, orc, storage, tol):
     rel = abs(ad - fd) / abs(fd)
     print(f"{storage} lattice, FP32 f0: <grad, v> adjoint {ad:.8e} FD {fd:.8e} rel {rel:.2e}")
     assert rel < tol
+
+
+@pytest.mark.parametrize("keep_segments", [1, 2, 10])  # trailing segments kept (10: all of them)
+def test_kept_segments_give_the_same_gradient(gpu, orc, keep_segments):
+    """keep_bytes: the trailing segments that fit are stepped out of place in the forward and walked
+    by the backward without a replay. The duct with velocity inflow, pressure outlet, walls, a
+    porosity field and an observed kinetic energy: outputs, observed sum and every gradient equal
+    to those of the pure-recompute run bit for bit."""
+    import torch
+
+    from virtualfluids_amd.adjoint import KineticEnergyObservation, k17_steps
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory
+
+    g = _duct(True)
+    omega, steps, K = 1.7, 11, 3
+    rng = np.random.default_rng(760)
+    f0 = pack_pre(near_equilibrium(rng, g.N, u_amp=0.03, noise=2e-3), g, True).astype(np.float32)
+    F = BoundaryConditionFactory
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    L = Lattice(g, omega)
+    U = np.array(g.bcs["velocity"].velocity, dtype=np.float32)
+    rho = rng.normal(0, 1e-3, g.bcs["pressure"].n).astype(np.float32)
+    phi = rng.uniform(0, 0.05, g.N).astype(np.float32)
+    lists = sum(27 * bc.n * 4 for name, bc in L.bcs.items() if name in ("velocity", "noslip"))
+    seg = K * (27 * g.N * 4 + lists)
+
+    def run(keep):
+        x = torch.from_numpy(f0).cuda().requires_grad_(True)
+        u = torch.from_numpy(U).cuda().requires_grad_(True)
+        r = torch.from_numpy(rho).cuda().requires_grad_(True)
+        p = torch.from_numpy(phi).cuda().requires_grad_(True)
+        obs = KineticEnergyObservation(g, np.nonzero(g.geo == 19)[0])
+        out, J = k17_steps(x, L, steps, checkpoint_every=K, even=True, bc_factory=fac, velocity=u, rho_bc=r,
+                           porosity=p, observe=obs, observe_from=4, keep_bytes=keep)
+        (J + (out.double() ** 2).sum() * 1e-3).backward()
+        return [t.detach().cpu().numpy() for t in (out, J, x.grad, u.grad, r.grad, p.grad)]
+
+    print(f"kept segments: {keep_segments} of {-(-steps // K)}", flush=True)
+    ref = run(0)
+    got = run(keep_segments * seg)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)

@@ -171,7 +171,7 @@ class Stepper:
 class _K17Steps(torch.autograd.Function):
     @staticmethod
     def forward(ctx, f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs, observe,
-                observe_from):
+                observe_from, keep_bytes):
         L = stepper.L
         # in place on one buffer: the same arithmetic as the out-of-place replay of the backward
         buf = _storage_input(f0, L, "f0")
@@ -188,12 +188,39 @@ class _K17Steps(torch.autograd.Function):
         ctx.params = _bc_snapshot(L)
         run = _StepGraphs(lambda ev, st: stepper.forward(buf, ev, None, st), graphs)
         checkpoints = []
+        # the trailing segments whose states fit keep_bytes are stepped out of place and kept, so
+        # the backward walks them without a replay
+        lists = stepper.post_lists()
+        step_bytes = buf.numel() * buf.element_size() + sum(27 * nb * 4 for _, nb in lists)
+        kept_from = steps
+        while kept_from > 0:
+            start = (kept_from - 1) // checkpoint_every * checkpoint_every
+            if (steps - start) * step_bytes > keep_bytes:
+                break
+            kept_from = start
+        kept = {}
+        stream = torch.cuda.current_stream()
         even = even0
         J = torch.zeros((), dtype=torch.float64, device=f0.device)
         i = 0
         while i < steps:
             if i % checkpoint_every == 0:
                 checkpoints.append((i, even, buf.clone()))
+            if i >= kept_from:
+                # one segment out of place, every state and post-collision BC value kept
+                n = min(checkpoint_every, steps - i)
+                states, posts = [checkpoints[-1][2]], []
+                for j in range(n):
+                    posts.append({name: torch.empty(27 * nb, dtype=torch.float32, device=buf.device)
+                                  for name, nb in lists})
+                    states.append(stepper.forward(states[-1], even, torch.empty_like(buf), stream, save=posts[j]))
+                    even = not even
+                    if observe is not None and i + j >= observe_from:
+                        J += observe(states[-1], even).detach().to(torch.float64)
+                kept[i] = (states, posts)
+                buf = states[-1]
+                i += n
+                continue
             if observe is not None and i >= observe_from:
                 run(even)  # one step: the observation follows every step from here on
                 even = not even
@@ -208,11 +235,13 @@ class _K17Steps(torch.autograd.Function):
         run.close()
         ctx.stepper, ctx.steps, ctx.K = stepper, steps, checkpoint_every
         ctx.checkpoints = checkpoints
+        ctx.kept = kept
         ctx.observe, ctx.observe_from = observe, observe_from
         ctx.has_velocity, ctx.has_rho = velocity is not None, rho_bc is not None
         ctx.has_porosity = porosity is not None
         ctx.in_dtype = f0.dtype
-        return buf.to(f0.dtype), J
+        # a kept final state stays the backward's: the caller gets a copy
+        return buf.to(f0.dtype, copy=bool(kept)), J
 
     @staticmethod
     def backward(ctx, grad_out, gJ):
@@ -240,14 +269,17 @@ class _K17Steps(torch.autograd.Function):
         lists = stepper.post_lists()
         for start, even_start, ckpt in reversed(ctx.checkpoints):
             n = min(K, steps - start)
-            states = [ckpt]
-            # the post-collision values of the BC nodes, step by step, for the BC adjoints
-            posts = [{name: torch.empty(27 * nb, dtype=torch.float32, device=lam.device) for name, nb in lists}
-                     for _ in range(n)]
-            even = even_start
-            for j in range(n):
-                states.append(stepper.forward(states[-1], even, torch.empty_like(ckpt), stream, save=posts[j]))
-                even = not even
+            if start in ctx.kept:
+                states, posts = ctx.kept.pop(start)
+            else:
+                states = [ckpt]
+                # the post-collision values of the BC nodes, step by step, for the BC adjoints
+                posts = [{name: torch.empty(27 * nb, dtype=torch.float32, device=lam.device) for name, nb in lists}
+                         for _ in range(n)]
+                even = even_start
+                for j in range(n):
+                    states.append(stepper.forward(states[-1], even, torch.empty_like(ckpt), stream, save=posts[j]))
+                    even = not even
             for j in reversed(range(n)):
                 even_j = even_start ^ (j % 2 == 1)
                 if observe is not None and start + j >= ctx.observe_from:
@@ -256,14 +288,15 @@ class _K17Steps(torch.autograd.Function):
                 stepper.adjoint(states[j], states[j + 1], lam, even_j, ubar, rbar, padj, stream, saved=posts[j])
             del states, posts
         ctx.checkpoints = None
+        ctx.kept = None
         gv = ubar.view(3, -1) if ubar is not None else None
-        return lam.to(ctx.in_dtype), gv, rbar, padj, None, None, None, None, None, None, None
+        return lam.to(ctx.in_dtype), gv, rbar, padj, None, None, None, None, None, None, None, None
 
 
 def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: int = 10, even: bool | None = None,
               bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
               rho_bc: torch.Tensor | None = None, porosity: torch.Tensor | None = None,
-              graphs: bool = True, observe=None, observe_from: int = 0):
+              graphs: bool = True, observe=None, observe_from: int = 0, keep_bytes: int = 0):
     """`steps` time steps of `lattice` from buffer f0, differentiable in f0 and, when given, in
     the velocity BC's wall velocities `velocity` (3, n), the pressure BC's densities `rho_bc`
     and the Brinkman porosity per node `porosity` (N).
@@ -283,17 +316,26 @@ def k17_steps(f0: torch.Tensor, lattice: Lattice, steps: int, checkpoint_every: 
     observed sum in FP64), and the sum's gradient enters the reverse sweep step by step, so a
     time average needs no stored states beyond the checkpoints. Without observe it returns the
     buffer.
+
+    keep_bytes: device memory the call may hold, beyond the checkpoints, for the states of its
+    trailing checkpoint segments (0: none, every segment is replayed in the backward). The
+    segments that fit, counted from the last, are stepped out of place in the forward and keep
+    every state and the BC nodes' post-collision values, so the backward walks them without a
+    replay; a segment costs checkpoint_every x (buffer + 27 x 4 B per BC node) bytes. The
+    gradient is the same bit for bit.
     """
-    if steps < 0 or checkpoint_every < 1:
-        raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
+    if steps < 0 or checkpoint_every < 1 or keep_bytes < 0:
+        raise ValueError("steps must be >= 0, checkpoint_every >= 1 and keep_bytes >= 0")
     stepper = Stepper(lattice, bc_factory)
     if velocity is not None and lattice.bcs.get("velocity") is None:
         raise ValueError("the lattice has no velocity BC nodes")
     if rho_bc is not None and lattice.bcs.get("pressure") is None:
         raise ValueError("the lattice has no pressure BC nodes")
     even0 = lattice.even if even is None else bool(even)
+    if not (torch.is_grad_enabled() and any(t is not None and t.requires_grad for t in (f0, velocity, rho_bc, porosity))):
+        keep_bytes = 0  # no backward will run: nothing to keep
     out, J = _K17Steps.apply(f0, velocity, rho_bc, porosity, stepper, steps, checkpoint_every, even0, graphs,
-                             observe, observe_from)
+                             observe, observe_from, int(keep_bytes))
     return out if observe is None else (out, J)
 
 
