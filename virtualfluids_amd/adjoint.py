@@ -598,6 +598,229 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
                                 checkpoint_every, even0, observe, observe_from, graphs, in_place)
 
 
+# ---- any number of levels ---------------------------------------------------------------------
+def _refresh_interface(steppers, cells, level, C, F, F_out, even, stream):
+    """The interface refresh after a step of `level` (parity `even`): fine -> coarse into C from
+    the finer level's state F, then coarse -> fine from C into F_out (F itself, or a copy)."""
+    Lc, Lf = steppers[level].L, steppers[level + 1].L
+    cf, fc = cells[level]
+    s = stream_handle(stream)
+    pc = _param(Lc, C, not even)
+    if fc.n:
+        capi.call("vf_ScaleFineToCoarseCompressible", ctypes.byref(pc), ctypes.byref(_param(Lf, F, True)),
+                  ctypes.byref(fc.struct), _neigh(fc), s)
+    if cf.n:
+        capi.call("vf_ScaleCoarseToFineCompressible", ctypes.byref(pc), ctypes.byref(_param(Lf, F_out, True)),
+                  ctypes.byref(cf.struct), _neigh(cf), s)
+
+
+def _level_step_in_place(steppers, cells, bufs, level, even, stream):
+    """One step of `level` in place on bufs, as UpdateGrid27::updateGrid recurses
+    (src/gpu/core/Calculation/UpdateGrid27.cpp:54-96): the next finer level takes two steps (even
+    parity first), this level steps, then the interface to the finer level is refreshed."""
+    finer = level + 1 < len(steppers)
+    if finer:
+        _level_step_in_place(steppers, cells, bufs, level + 1, True, stream)
+        _level_step_in_place(steppers, cells, bufs, level + 1, False, stream)
+    steppers[level].forward(bufs[level], even, None, stream)
+    if finer:
+        _refresh_interface(steppers, cells, level, bufs[level], bufs[level + 1], bufs[level + 1], even, stream)
+
+
+def _level_step_record(steppers, cells, bufs, level, even, stream, post_lists):
+    """One step of `level` out of place from the states bufs (levels level, level + 1, ...):
+    the same arithmetic as _level_step_in_place, keeping what the reverse sweep reads. Returns
+    (record, states after the step) with record = (even, state before, state after the step,
+    state after the interface writes, finer state the fine -> coarse write read, post-collision
+    BC values, records of the finer level's two steps)."""
+    finer = level + 1 < len(steppers)
+    subs = []
+    fine_states = bufs[1:]
+    if finer:
+        for ev in (True, False):
+            rec, fine_states = _level_step_record(steppers, cells, fine_states, level + 1, ev, stream, post_lists)
+            subs.append(rec)
+    S = bufs[0]
+    posts = {name: torch.empty(27 * nb, dtype=torch.float32, device=S.device) for name, nb in post_lists[level]}
+    C1 = steppers[level].forward(S, even, torch.empty_like(S), stream, save=posts)
+    if not finer:
+        return (even, S, C1, C1, None, posts, subs), [C1]
+    F2 = fine_states[0]
+    C2, F3 = C1.clone(), F2.clone()
+    _refresh_interface(steppers, cells, level, C2, F2, F3, even, stream)
+    return (even, S, C1, C2, F2, posts, subs), [C2, F3] + fine_states[1:]
+
+
+def _level_adjoint(steppers, cells, sources, rec, lams, grads, level, stream):
+    """Reverse of one recorded step of `level`: the interface adjoints (coarse -> fine gathered per
+    coarse source node, then fine -> coarse), this level's step adjoint, then the finer level's
+    two steps in reverse. lams: the FP32 adjoint of every level, updated in place; grads[level]:
+    (wall velocity, outlet density, porosity) adjoint accumulators or None."""
+    even, S, C1, C2, F2, posts, subs = rec
+    if level + 1 < len(steppers):
+        Lc, Lf = steppers[level].L, steppers[level + 1].L
+        cf, fc = cells[level]
+        s = stream_handle(stream)
+        pc, pf = _param(Lc, C2, not even), _param(Lf, F2, True)
+        if cf.n:
+            src, work = sources[level]
+            capi.call("vf_ScaleCoarseToFineCompressibleAdjointGather", ctypes.byref(pc), ctypes.byref(pf),
+                      ctypes.byref(cf.struct), _neigh(cf), ctypes.byref(src), _vp(work), _vp(lams[level]),
+                      _vp(lams[level + 1]), s)
+        if fc.n:
+            capi.call("vf_ScaleFineToCoarseCompressibleAdjoint", ctypes.byref(pc), ctypes.byref(pf),
+                      ctypes.byref(fc.struct), _neigh(fc), _vp(lams[level]), _vp(lams[level + 1]), s)
+    ubar, rbar, padj = grads[level]
+    steppers[level].adjoint(S, C1, lams[level], even, ubar, rbar, padj, stream, saved=posts)
+    for sub in reversed(subs):
+        _level_adjoint(steppers, cells, sources, sub, lams, grads, level + 1, stream)
+
+
+class _MultiLevelSteps(torch.autograd.Function):
+    """Coarse steps of a level hierarchy (any number of levels), differentiable in every level's
+    initial buffer, the coarsest level's wall velocities and outlet densities and every level's
+    porosity. Inputs after `meta`: the n initial buffers, velocity, rho_bc, the n porosities."""
+
+    @staticmethod
+    def forward(ctx, meta, *args):
+        steppers, cells, steps, K, even0, observe, observe_from, graphs = meta
+        n = len(steppers)
+        f0s, velocity, rho_bc, pors = args[:n], args[n], args[n + 1], args[n + 2:]
+        lats = [s.L for s in steppers]
+        bufs = [_storage_input(f, L, f"level {i} buffer") for i, (f, L) in enumerate(zip(f0s, lats))]
+        if velocity is not None:
+            for i in range(3):
+                lats[0].bcs["velocity"].vel[i].copy_(velocity[i].detach().to(torch.float32))
+        if rho_bc is not None:
+            lats[0].bcs["pressure"].rho_bc.copy_(rho_bc.detach().to(torch.float32))
+        for L, p in zip(lats, pors):
+            if p is not None:
+                L.set_porosity(p.detach())
+        # the values this forward used: the backward replays with them
+        ctx.params = [_bc_snapshot(L) for L in lats]
+        run = _StepGraphs(lambda ev, st: _level_step_in_place(steppers, cells, bufs, 0, ev, st), graphs)
+        checkpoints = []
+        even = even0
+        J = torch.zeros((), dtype=torch.float64, device=bufs[0].device)
+        i = 0
+        while i < steps:
+            if i % K == 0:
+                checkpoints.append((i, even, [b.clone() for b in bufs]))
+            if observe is not None and i >= observe_from:
+                run(even)  # one step: the observation follows every step from here on
+                even = not even
+                J += observe(bufs[0], even).detach().to(torch.float64)
+                i += 1
+                continue
+            n_run = min(K - i % K, steps - i, (observe_from - i) if observe is not None else steps)
+            even = run.run(n_run, even)
+            i += n_run
+        run.close()
+        ctx.meta = meta
+        ctx.checkpoints = checkpoints
+        ctx.flags = (velocity is not None, rho_bc is not None, [p is not None for p in pors])
+        ctx.in_dtypes = [f.dtype for f in f0s]
+        return tuple(b.to(f.dtype) for b, f in zip(bufs, f0s)) + (J,)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        steppers = ctx.meta[0]
+        current = [_bc_snapshot(s.L) for s in steppers]
+        for s, p in zip(steppers, ctx.params):
+            _bc_restore(s.L, p)
+        try:
+            return _MultiLevelSteps._backward(ctx, grads[:-1], grads[-1])
+        finally:
+            for s, p in zip(steppers, current):
+                _bc_restore(s.L, p)
+
+    @staticmethod
+    def _backward(ctx, gbufs, gJ):
+        steppers, cells, steps, K, even0, observe, observe_from, graphs = ctx.meta
+        n = len(steppers)
+        lats = [s.L for s in steppers]
+        dev = lats[0].device
+        stream = torch.cuda.current_stream()
+        lams = [(torch.zeros(L.f.shape, dtype=torch.float32, device=dev) if g is None
+                 else g.detach().to(torch.float32).contiguous().clone()) for g, L in zip(gbufs, lats)]
+        has_u, has_r, has_p = ctx.flags
+        grads = [(torch.zeros(3 * L.bcs["velocity"].n, device=dev) if (i == 0 and has_u) else None,
+                  torch.zeros(L.bcs["pressure"].n, device=dev) if (i == 0 and has_r) else None,
+                  torch.zeros(L.N, device=dev) if has_p[i] else None) for i, L in enumerate(lats)]
+        sources = []
+        for (cf, _), L in zip(cells, lats):
+            src, nwork = cf.sources(L.grid)
+            sources.append((src, torch.empty(max(nwork, 1), dtype=torch.float32, device=dev)))
+        post_lists = [s.post_lists() for s in steppers]
+        if gJ is not None and observe is None:
+            gJ = None
+        for start, even_start, bufs0 in reversed(ctx.checkpoints):
+            records, states, even = [], bufs0, even_start
+            for _ in range(min(K, steps - start)):
+                rec, states = _level_step_record(steppers, cells, states, 0, even, stream, post_lists)
+                records.append(rec)
+                even = not even
+            for j in reversed(range(len(records))):
+                rec = records[j]
+                if gJ is not None and start + j >= observe_from:
+                    # the observed objective of the coarse state after this step
+                    _observe_grad(observe, rec[3], not rec[0], lams[0], gJ)
+                _level_adjoint(steppers, cells, sources, rec, lams, grads, 0, stream)
+            del records, states
+        ctx.checkpoints = None
+        ubar, rbar = grads[0][0], grads[0][1]
+        return ((None,) + tuple(lam.to(dt) for lam, dt in zip(lams, ctx.in_dtypes))
+                + (ubar.view(3, -1) if ubar is not None else None, rbar) + tuple(g[2] for g in grads))
+
+
+def k17_multilevel_steps(f0s, lattices, interfaces, steps: int, checkpoint_every: int = 10, even: bool | None = None,
+                         bc_factories=None, velocity: torch.Tensor | None = None, rho_bc: torch.Tensor | None = None,
+                         porosity=None, observe=None, observe_from: int = 0, graphs: bool = True):
+    """`steps` coarse steps of a hierarchy of any number of levels (level 0 coarsest) from the
+    buffers f0s, differentiable in every buffer and, when given, in the coarsest level's velocity BC
+    values (3, n) and outlet densities and in each level's porosity (porosity: one tensor or None
+    per level). interfaces: one (cf, fc) InterfaceCells pair per pair of neighbouring levels.
+
+    A coarse step follows UpdateGrid27::updateGrid (src/gpu/core/Calculation/UpdateGrid27.cpp:54-96):
+    every finer level takes two steps per step of the level above it (even parity first, so every
+    finer level starts with the even parity), then the level steps and refreshes the interface to
+    its finer level (fine -> coarse, coarse -> fine). The forward works in place and replays HIP
+    graphs of the coarse step (graphs=False: launches from the host, the same bit for bit); the
+    backward replays each checkpoint segment out of place, keeping every level's intermediate
+    states, and walks it in reverse: per step of a level, the interface adjoints, the level's step
+    adjoint, then its finer level's two steps. For two levels it computes what k17_two_level_steps
+    computes. f0s hold each level's storage type or FP32; observe / observe_from as in
+    k17_two_level_steps, on the coarsest level. Returns (list of final buffers, observed sum)."""
+    from .refinement import DeviceInterfaceCells
+
+    n = len(lattices)
+    if n < 2 or len(interfaces) != n - 1 or len(f0s) != n:
+        raise ValueError("a hierarchy needs two or more levels, one buffer per level and one interface per level pair")
+    if steps < 0 or checkpoint_every < 1:
+        raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
+    if any(L.storage != lattices[0].storage for L in lattices):
+        raise ValueError("every level must use the same storage type")
+    pors = list(porosity) if porosity is not None else [None] * n
+    if len(pors) != n:
+        raise ValueError("porosity: one tensor or None per level")
+    if velocity is not None and lattices[0].bcs.get("velocity") is None:
+        raise ValueError("the coarsest level has no velocity BC nodes")
+    if rho_bc is not None and lattices[0].bcs.get("pressure") is None:
+        raise ValueError("the coarsest level has no pressure BC nodes")
+    # a coarse step cycles through every level's populations
+    total = sum(L.state_bytes for L in lattices)
+    for L in lattices:
+        L.set_working_set(total)
+    facs = list(bc_factories) if bc_factories is not None else [None] * n
+    steppers = [Stepper(L, f) for L, f in zip(lattices, facs)]
+    cells = [(DeviceInterfaceCells(cf, L.device), DeviceInterfaceCells(fc, L.device))
+             for (cf, fc), L in zip(interfaces, lattices)]
+    even0 = lattices[0].even if even is None else bool(even)
+    meta = (steppers, cells, steps, checkpoint_every, even0, observe, observe_from, graphs)
+    out = _MultiLevelSteps.apply(meta, *f0s, velocity, rho_bc, *pors)
+    return list(out[:n]), out[n]
+
+
 def pre_collision_index(grid, even: bool) -> np.ndarray:
     """(27, N) flat indices into a 27*N buffer of every node's pre-collision populations."""
     N = grid.N
