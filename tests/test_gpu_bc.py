@@ -14,11 +14,16 @@ pytestmark = pytest.mark.gpu
 
 
 def _gpu_run(grid, omega, steps, velocity_bc, noslip_bc, pressure_bc=None, box=True, storage="fp32",
-             limiters=(0.01, 0.01, 0.01)):
+             limiters=(0.01, 0.01, 0.01), rows=None):
+    """rows: the layout hint of every BC list (None: what the host derives from the lists; 0: a
+    lane per direction for the interpolated and pressure BCs; 1: a thread per node)."""
     from virtualfluids_amd.lattice import Lattice
     from virtualfluids_amd.simulation import BoundaryConditionFactory as F, Simulation
 
     L = Lattice(grid, omega, storage=storage, box=box, limiters=limiters)
+    if rows is not None:
+        for bc in L.bcs.values():
+            bc.struct.nodesAlongRows = rows
     N = grid.N
     L.init_equilibrium(np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N))
     f = F()
@@ -52,25 +57,30 @@ def _errors(grid, gpu_fields, ref_fields, u_ref):
     ("VelocityWithPressureInterpolatedCompressible", "NoSlipInterpolatedCompressible"),
 ])
 @pytest.mark.parametrize("box", [True, False])
-def test_cavity_bcs_match_oracle(gpu, orc, velocity_bc, noslip_bc, box):
+@pytest.mark.parametrize("rows", [None, 0, 1])
+def test_cavity_bcs_match_oracle(gpu, orc, velocity_bc, noslip_bc, box, rows):
     grid = cavity(24, lid_velocity=0.05)
     omega, steps = 1.8, 200
     ref = oracle_run(orc, grid, omega, steps, velocity_bc=velocity_bc, noslip_bc=noslip_bc)
-    L = _gpu_run(grid, omega, steps, velocity_bc, noslip_bc, box=box)
+    L = _gpu_run(grid, omega, steps, velocity_bc, noslip_bc, box=box, rows=rows)
     e = _errors(grid, L.macroscopic(), ref.macroscopic(grid.geo), 0.05)
-    print(velocity_bc, noslip_bc, box, e)
+    print(velocity_bc, noslip_bc, box, rows, e)
     assert e["u_max"] < 1e-4 and e["rho_max"] < 1e-6
 
 
-def test_channel_pressure_outlet_matches_oracle(gpu, orc):
+@pytest.mark.parametrize("velocity_bc,noslip_bc", [
+    ("VelocityBounceBack", "NoSlipBounceBack"),
+    ("VelocityInterpolatedCompressible", "NoSlipInterpolatedCompressible"),
+])
+@pytest.mark.parametrize("rows", [None, 0, 1])
+def test_channel_pressure_outlet_matches_oracle(gpu, orc, velocity_bc, noslip_bc, rows):
     grid = channel((32, 8, 8), inflow=0.02)
     omega, steps = 1.8, 300
-    ref = oracle_run(orc, grid, omega, steps, velocity_bc="VelocityBounceBack", noslip_bc="NoSlipBounceBack",
-                     pressure=True)
-    L = _gpu_run(grid, omega, steps, "VelocityBounceBack", "NoSlipBounceBack",
-                 pressure_bc="PressureNonEquilibriumCompressible")
+    ref = oracle_run(orc, grid, omega, steps, velocity_bc=velocity_bc, noslip_bc=noslip_bc, pressure=True)
+    L = _gpu_run(grid, omega, steps, velocity_bc, noslip_bc, pressure_bc="PressureNonEquilibriumCompressible",
+                 rows=rows)
     e = _errors(grid, L.macroscopic(), ref.macroscopic(grid.geo), 0.02)
-    print("channel", e)
+    print("channel", velocity_bc, rows, e)
     assert e["u_max"] < 1e-4 and e["rho_max"] < 1e-6
 
 

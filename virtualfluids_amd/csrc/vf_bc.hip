@@ -1,7 +1,8 @@
 // Boundary-condition kernels of the K17 time step and their C entry points.
 //
 // Post-collision BCs run after the collision, before the parity swap
-// (src/gpu/core/Calculation/UpdateGrid27.cpp:69-77): one thread per BC node reads the
+// (src/gpu/core/Calculation/UpdateGrid27.cpp:69-77): one thread per BC node (a lane per
+// direction for interpolated lists not along x rows, vf_bc_adjoint.h) reads the
 // node's post-collision populations with the current parity and, for every link d with
 // 0 <= q_d <= 1, writes the incoming population inv(d) of the next step into the slot the
 // next collision reads (flipped parity). Formulas follow the closed forms of
@@ -94,6 +95,48 @@ __global__ __launch_bounds__(256) void bc_post_kernel(S* __restrict__ f, Addr a,
     }
 }
 
+// The interpolated post-collision BCs with a lane per direction (lists not along x rows; the
+// arithmetic of bc_post_kernel, the node sums over the lane group). Lane d reads the node's
+// post-collision value d and writes the incoming population inv(d) of its link; both cells lie at
+// the cell node moved by +1 where c_d > 0, in slots inv(d) and d (even parity; swapped for odd),
+// so no lane writes a cell another lane of the list reads (links end outside the fluid).
+template <class S, int KIND>
+__global__ __launch_bounds__(kLaneBlock) void bc_post_lanes_kernel(S* __restrict__ f, Addr a, BcArgs b, int even_flag)
+{
+    static_assert(KIND >= kNoSlipInterp, "bounce-back kinds keep one thread per node");
+    const uint32_t i = blockIdx.x * kLaneNodes + (threadIdx.x / kLaneGroup);
+    const bool node = i < b.nbc; // every lane of the group stays for the shuffles
+    const LaneDir L = lane_dir();
+    const bool on = node && L.valid;
+    const bool even = even_flag != 0;
+    float post = 0.0f, q = -1.0f;
+    S* wcell = nullptr;
+    if (on) {
+        const uint32_t m = moved_node(a, (uint32_t)b.k[i], L.cx > 0, L.cy > 0, L.cz > 0);
+        post = Codec<S>::load(f + (uint64_t)(even ? L.r : L.d) * a.N + m);
+        if (L.d != 0) q = b.q[(uint64_t)L.d * b.nbc + i];
+        wcell = f + (uint64_t)(even ? L.d : L.r) * a.N + m;
+        if (b.postOut) b.postOut[(uint64_t)L.d * b.nbc + i] = post;
+    }
+    const float drho = group_sum(post);
+    const float ir = 1.0f / (1.0f + drho);
+    const float ux = group_sum(float(L.cx) * post) * ir, uy = group_sum(float(L.cy) * post) * ir,
+                uz = group_sum(float(L.cz) * post) * ir;
+    const float cu_sq = 1.5f * (ux * ux + uy * uy + uz * uz) * (1.0f + drho);
+    const float fr = __shfl(post, L.valid ? L.r : L.d, kLaneGroup);
+    if (!(on && q >= 0.0f && q <= 1.0f)) return;
+    const float omega = b.omega, w = L.w, fd = post;
+    const float cu = float(L.cx) * ux + float(L.cy) * uy + float(L.cz) * uz;
+    const float feq = w * (drho + 4.5f * cu * cu * (1.0f + drho) - cu_sq); // getEquilibriumForBC
+    float wall = 0.0f;
+    if (KIND != kNoSlipInterp)
+        wall = 6.0f * w * (float(L.cx) * b.vx[i] + float(L.cy) * b.vy[i] + float(L.cz) * b.vz[i]);
+    float v = (1.0f - q) / (1.0f + q) * (fd - fr + (fd + fr - 2.0f * feq * omega) / (1.0f - omega)) * 0.5f +
+              (q * (fd + fr) - wall) / (1.0f + q);
+    if (KIND == kVelocityPressureInterp) v -= w * drho;
+    Codec<S>::store(wcell, v);
+}
+
 struct PressureArgs {
     const int32_t* k;
     const int32_t* kN;
@@ -138,6 +181,41 @@ __global__ __launch_bounds__(256) void pressure_neq_kernel(S* __restrict__ f, Ad
     }
 }
 
+// The pressure BC with a lane per direction (faces normal to x; the arithmetic of
+// pressure_neq_kernel): lane d reads the neighbour's pre-collision value d and, where the face
+// writes direction d, writes the BC node's cell d (kN is not itself a pressure BC node).
+template <class S>
+__global__ __launch_bounds__(kLaneBlock) void pressure_neq_lanes_kernel(S* __restrict__ f, Addr a, PressureArgs b,
+                                                                        int even_flag)
+{
+    const uint32_t i = blockIdx.x * kLaneNodes + (threadIdx.x / kLaneGroup);
+    const bool node = i < b.nbc;
+    const LaneDir L = lane_dir();
+    const bool on = node && L.valid;
+    const bool even = even_flag != 0;
+    float fn = 0.0f, rbc = 0.0f;
+    S* wcell = nullptr;
+    if (on) {
+        const uint64_t slot = (uint64_t)(even ? L.d : L.r) * a.N;
+        const bool sx = L.cx < 0, sy = L.cy < 0, sz = L.cz < 0;
+        fn = Codec<S>::load(f + slot + moved_node(a, (uint32_t)b.kN[i], sx, sy, sz));
+        rbc = b.rhoBC[i];
+        const int c = b.axis == 0 ? L.cx : (b.axis == 1 ? L.cy : L.cz);
+        if (c * b.sign <= 0) wcell = f + slot + moved_node(a, (uint32_t)b.k[i], sx, sy, sz);
+    }
+    const float drho = group_sum(fn);
+    const float ir = 1.0f / (drho + 1.0f);
+    const float ux = group_sum(float(L.cx) * fn) * ir, uy = group_sum(float(L.cy) * fn) * ir,
+                uz = group_sum(float(L.cz) * fn) * ir;
+    const float cu_sq = 1.5f * (ux * ux + uy * uy + uz * uz);
+    if (!wcell) return;
+    // non-equilibrium part of the neighbour plus the equilibrium at the prescribed density
+    const float cu = float(L.cx) * ux + float(L.cy) * uy + float(L.cz) * uz;
+    const float e = 3.0f * cu + 4.5f * cu * cu - cu_sq;
+    const float v = fn - L.w * (drho + (1.0f + drho) * e) + L.w * (rbc + (1.0f + rbc) * e);
+    Codec<S>::store(wcell, v);
+}
+
 template <int KIND> static int launch_bc_post(const vf_lbm_parameter* p, const vf_q_boundary_conditions* bc, void* stream,
                                               const char* name)
 {
@@ -163,6 +241,29 @@ template <int KIND> static int launch_bc_post(const vf_lbm_parameter* p, const v
     BcArgs b{bc->k, bc->q27, bc->numberOfBCnodes, bc->Vx, bc->Vy, bc->Vz, p->omega, bc->postCollision};
     const dim3 grid((bc->numberOfBCnodes + kBcBlock - 1) / kBcBlock);
     hipStream_t s = (hipStream_t)stream;
+    if constexpr (KIND >= kNoSlipInterp) {
+        if (!bc->nodesAlongRows) {
+            const dim3 lanes((bc->numberOfBCnodes + kLaneNodes - 1) / kLaneNodes);
+            switch (p->storage) {
+            case VF_STORAGE_FP32:
+                hipLaunchKernelGGL((bc_post_lanes_kernel<float, KIND>), lanes, dim3(kLaneBlock), 0, s,
+                                   (float*)p->distributions, a, b, p->isEvenTimestep);
+                break;
+            case VF_STORAGE_FP16:
+                hipLaunchKernelGGL((bc_post_lanes_kernel<__half, KIND>), lanes, dim3(kLaneBlock), 0, s,
+                                   (__half*)p->distributions, a, b, p->isEvenTimestep);
+                break;
+            case VF_STORAGE_BF16:
+                hipLaunchKernelGGL((bc_post_lanes_kernel<__hip_bfloat16, KIND>), lanes, dim3(kLaneBlock), 0, s,
+                                   (__hip_bfloat16*)p->distributions, a, b, p->isEvenTimestep);
+                break;
+            default:
+                set_last_error("unknown storage type");
+                return VF_ERR_INVALID_ARGUMENT;
+            }
+            return check_launch(name);
+        }
+    }
     switch (p->storage) {
     case VF_STORAGE_FP32:
         hipLaunchKernelGGL((bc_post_kernel<float, KIND>), grid, dim3(kBcBlock), 0, s, (float*)p->distributions, a, b, p->isEvenTimestep);
@@ -237,6 +338,27 @@ extern "C" int vf_PressureNonEquilibriumCompressible(const vf_lbm_parameter* p, 
     const Addr a = make_addr(*p);
     const dim3 grid((bc->numberOfBCnodes + kBcBlock - 1) / kBcBlock);
     hipStream_t s = (hipStream_t)stream;
+    if (!bc->nodesAlongRows) {
+        const dim3 lanes((bc->numberOfBCnodes + kLaneNodes - 1) / kLaneNodes);
+        switch (p->storage) {
+        case VF_STORAGE_FP32:
+            hipLaunchKernelGGL((pressure_neq_lanes_kernel<float>), lanes, dim3(kLaneBlock), 0, s,
+                               (float*)p->distributions, a, b, p->isEvenTimestep);
+            break;
+        case VF_STORAGE_FP16:
+            hipLaunchKernelGGL((pressure_neq_lanes_kernel<__half>), lanes, dim3(kLaneBlock), 0, s,
+                               (__half*)p->distributions, a, b, p->isEvenTimestep);
+            break;
+        case VF_STORAGE_BF16:
+            hipLaunchKernelGGL((pressure_neq_lanes_kernel<__hip_bfloat16>), lanes, dim3(kLaneBlock), 0, s,
+                               (__hip_bfloat16*)p->distributions, a, b, p->isEvenTimestep);
+            break;
+        default:
+            set_last_error("unknown storage type");
+            return VF_ERR_INVALID_ARGUMENT;
+        }
+        return check_launch("PressureNonEquilibriumCompressible_Device execution failed");
+    }
     switch (p->storage) {
     case VF_STORAGE_FP32:
         hipLaunchKernelGGL((pressure_neq_kernel<float>), grid, dim3(kBcBlock), 0, s, (float*)p->distributions, a, b, p->isEvenTimestep);

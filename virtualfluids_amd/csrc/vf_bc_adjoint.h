@@ -9,6 +9,7 @@
 #pragma once
 
 #include "vf_k17.h"
+#include "vf_lattice.h"
 
 namespace vfa {
 
@@ -114,6 +115,74 @@ VF_HD void pressure_vjp(const float (&fn)[27], float rhoBC, const float (&G)[27]
     for (int e = 0; e < 27; ++e)
         fnb[e] = G[e] - A1 + Vx * (float(cx(e)) - ux) + Vy * (float(cy(e)) - uy) + Vz * (float(cz(e)) - uz);
     rhob = A1;
+}
+
+// ---- lane-per-direction forms of the BC kernels and their adjoints ----------------------------
+// 32 lanes per BC node (lane d < 27 takes direction d; lanes 27..31 hold zeros but take part in
+// every shuffle), 8 nodes per 256-thread workgroup, for lists whose consecutive nodes are not
+// neighbours in x (faces normal to x: every node and direction is a cache line of its own). One
+// thread per node leaves such a face of ten thousand nodes at a few waves per CU, each waiting on
+// 27-deep chains of scattered loads; here the scattered loads of a node are one load per lane
+// and 32 times as many waves are in flight. Lists along x rows (walls normal to y or z) keep one
+// thread per node, whose loads of a direction are contiguous (vf_q_boundary_conditions::
+// nodesAlongRows). The node sums (density, momentum, and the adjoint sums that reach all 27
+// directions) are butterflies over the lane group.
+constexpr int kLaneGroup = 32;
+constexpr int kLaneBlock = 256;
+constexpr int kLaneNodes = kLaneBlock / kLaneGroup;
+
+constexpr uint32_t dir_bits(int axis, int sign)
+{
+    uint32_t m = 0;
+    for (int d = 0; d < 27; ++d)
+        if ((axis == 0 ? cx(d) : (axis == 1 ? cy(d) : cz(d))) == sign) m |= 1u << d;
+    return m;
+}
+
+struct LaneDir {
+    int d;       // direction of this lane, valid for d < 27
+    bool valid;
+    int cx, cy, cz;
+    int r;       // inv(d)
+    float w;     // weight(d), 0 for the idle lanes
+};
+
+__device__ __forceinline__ LaneDir lane_dir()
+{
+    LaneDir L;
+    L.d = (int)(threadIdx.x & (kLaneGroup - 1));
+    L.valid = L.d < 27;
+    const uint32_t bit = L.valid ? 1u << L.d : 0u;
+    L.cx = int((dir_bits(0, 1) & bit) != 0) - int((dir_bits(0, -1) & bit) != 0);
+    L.cy = int((dir_bits(1, 1) & bit) != 0) - int((dir_bits(1, -1) & bit) != 0);
+    L.cz = int((dir_bits(2, 1) & bit) != 0) - int((dir_bits(2, -1) & bit) != 0);
+    L.r = L.d == 0 ? 0 : (L.d <= 18 ? ((L.d & 1) ? L.d + 1 : L.d - 1) : 45 - L.d);
+    L.w = !L.valid ? 0.0f : (L.d == 0 ? 8.0f / 27.0f : (L.d <= 6 ? 2.0f / 27.0f : (L.d <= 18 ? 1.0f / 54.0f : 1.0f / 216.0f)));
+    return L;
+}
+
+__device__ __forceinline__ float group_sum(float v)
+{
+#pragma unroll
+    for (int m = kLaneGroup / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, kLaneGroup);
+    return v;
+}
+
+// node k moved by +1 along the axes whose flag is set (the cell node that holds a direction)
+__device__ __forceinline__ uint32_t moved_node(const Addr& a, uint32_t k, bool sx, bool sy, bool sz)
+{
+    if (a.nbx) {
+        if (sx) k = a.nbx[k];
+        if (sy) k = a.nby[k];
+        if (sz) k = a.nbz[k];
+        return k;
+    }
+    const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny, nz = (uint32_t)a.nz;
+    uint32_t x = k % nx, t = k / nx, y = t % ny, z = t / ny;
+    if (sx) x = x + 1 == nx ? 0 : x + 1;
+    if (sy) y = y + 1 == ny ? 0 : y + 1;
+    if (sz) z = z + 1 == nz ? 0 : z + 1;
+    return x + nx * (y + ny * z);
 }
 
 } // namespace vfa
