@@ -72,9 +72,10 @@ def _ke_grad(f):
     return ke, g
 
 
-def run(orc, grids, interfaces, plane, f0s, vx, porosity, steps, avg, record=False, limiters=(0.01, 0.01, 0.01)):
-    """The FP64 forward; returns J (and, with record, what the reverse reads). limiters: the K17
-    quadric limiters of every level."""
+def run(orc, grids, interfaces, plane, f0s, vx, porosity, steps, avg, record=False, limiters=(0.01, 0.01, 0.01),
+        final_states=False):
+    """The FP64 forward; returns J (and, with record, what the reverse reads; with final_states,
+    every level's buffer after the last step). limiters: the K17 quadric limiters of every level."""
     levels = [orc.Lattice(g.nbx, g.nby, g.nbz) for g in grids]
     for P, f in zip(levels, f0s):
         P.f[:] = f
@@ -113,6 +114,8 @@ def run(orc, grids, interfaces, plane, f0s, vx, porosity, steps, avg, record=Fal
         records.append(update(0))
         if step >= steps - avg:
             J += _ke_grad(C.f[idx[C.even][:, plane]])[0] / avg
+    if final_states:
+        return J, [P.f.copy() for P in levels]
     return (J, records) if record else J
 
 

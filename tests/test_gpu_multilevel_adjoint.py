@@ -1,4 +1,8 @@
-"""Gradients through any number of refinement levels on the GPU (k17_multilevel_steps).
+"""Any number of refinement levels on the GPU: MultiLevelSimulation's steps and the gradients of
+k17_multilevel_steps.
+
+Three levels forward: every level's populations after several coarse steps of MultiLevelSimulation
+(HIP graphs and host launches) against the FP64 oracle's run of the same hierarchy.
 
 Two levels: the same outputs, observed sum and gradients as k17_two_level_steps, bit for bit.
 Three levels (multilevel_reference's case: a channel with a fine box nested in it and a finer
@@ -136,3 +140,34 @@ def test_three_level_gradients_match_the_fp64_reference_adjoint(gpu, orc, K, lim
         tol = 1e-5 if name in ("inflow u_x", "initial state level 0") else (1e-2 if limiter < 0.05 else 2e-3)
         print(f"  {name}: relative L2 error against the FP64 reference adjoint {err:.2e} (bound {tol:g})")
         assert err < tol, name
+
+
+def test_three_level_steps_match_oracle(gpu, orc):
+    """Six coarse steps (24 on the finest level) of MultiLevelSimulation on three levels with the
+    channel's interpolated BCs and pressure outlet, against multilevel_reference's FP64 run from the
+    same FP32 initial buffers: every cell of every level within 2e-6; HIP-graph replays and host
+    launches equal bit for bit."""
+    import torch
+
+    import multilevel_reference as ref
+    from virtualfluids_amd.refinement import GridScalingFactory
+    from virtualfluids_amd.simulation import MultiLevelSimulation
+
+    grids, interfaces, plane, _ = ref.three_levels()
+    f0s = ref.initial_states(grids, 1630)
+    fac = GridScalingFactory()
+    fac.set_scaling_factory(GridScalingFactory.GridScaling.ScaleCompressible)
+    out = []
+    for graphs in (True, False):
+        lats = _lattices(grids)
+        for L, f in zip(lats, f0s):
+            L.f.copy_(torch.from_numpy(f.astype(np.float32)))
+        MultiLevelSimulation(lats, interfaces, [_factory(), None, None], scaling_factory=fac).run(6, graphs=graphs)
+        out.append([L.populations() for L in lats])
+    vx = np.asarray(grids[0].bcs["velocity"].velocity[0], dtype=np.float64)
+    _, want = ref.run(orc, grids, interfaces, plane, f0s, vx, [None, None, None], 6, 1, final_states=True)
+    for lev in range(3):
+        assert np.array_equal(out[0][lev], out[1][lev]), lev
+        err = np.abs(out[0][lev] - want[lev]).max()
+        print(f"three levels, 6 coarse steps: level {lev} max |GPU - FP64 oracle| {err:.2e}")
+        assert err < 2e-6, lev
