@@ -132,6 +132,40 @@ def test_two_levels_through_cpp_update_grid(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+def test_three_levels_through_cpp_update_grid(gpu, tmp_path):
+    """Three levels (a channel with interpolated BCs and a pressure outlet, a box nested in it and
+    one nested in that): the C++ UpdateGrid27's recursion gives the Python MultiLevelSimulation's
+    fields bit for bit."""
+    import multilevel_reference as ref
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.refinement import GridScalingFactory
+    from virtualfluids_amd.simulation import BoundaryConditionFactory as F, MultiLevelSimulation
+
+    nu, steps = ref.NU, 8
+    grids, interfaces, _, _ = ref.three_levels()
+    inits = [(np.zeros(g.N), np.full(g.N, 0.03), np.zeros(g.N), np.zeros(g.N)) for g in grids]
+    write_case(str(tmp_path), list(zip(grids, inits)), nu, velocity_bc="VelocityInterpolatedCompressible",
+               noslip_bc="NoSlipInterpolatedCompressible", pressure_bc=True, interfaces=interfaces)
+    run_case(str(tmp_path), steps)
+    lats = []
+    for lev, (g, init) in enumerate(zip(grids, inits)):
+        L = Lattice(g, ref.omega(lev), level=lev)
+        L.init_equilibrium(*init)
+        lats.append(L)
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    scaling = GridScalingFactory()
+    scaling.set_scaling_factory(GridScalingFactory.GridScaling.ScaleCompressible)
+    MultiLevelSimulation(lats, interfaces, [fac, None, None], scaling_factory=scaling).run(steps)
+    for lev, (g, L) in enumerate(zip(grids, lats)):
+        fluid = g.geo == 19
+        for a, b in zip(read_output(str(tmp_path), lev), L.macroscopic()[:4]):
+            assert np.array_equal(a[fluid], b[fluid]), lev
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 def test_checkpoint_and_restart_through_cpp(gpu, tmp_path, storage):
     """Two levels: 20 coarse steps with a binary checkpoint after 10, against a second run that
