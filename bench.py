@@ -411,6 +411,26 @@ def inplace27_bandwidth(n=256, reps=20, caps=(0, 2, 3, 4)):
             "best_GBps": out[best]}
 
 
+def row_pattern_bandwidth(L, reps=20):
+    """The FP32 row kernel's own memory pattern without the collision, as a yardstick: the same
+    launch on the same lattice with a fluid mask that marks every node solid, so every node takes
+    the kernel's solid-node path and writes back the 27 values it read (the same aligned loads
+    from the four rows a cell touches, the same lane and LDS exchange of the nine x-shifted
+    directions, the same stores; plus one mask byte per node). 216 B per node / time, GB/s."""
+    import ctypes
+
+    import torch
+
+    solid = torch.zeros(L.N, dtype=torch.uint8, device=L.device)
+    saved = L.param.fluidMask
+    L.param.fluidMask = ctypes.c_void_p(solid.data_ptr())
+    try:
+        t = time_steps(L, reps, 3)
+    finally:
+        L.param.fluidMask = saved
+    return round(216.0 * L.N * reps / t / 1e9, 1)
+
+
 def cpu_baseline(n=256, target_s=8.0):
     """The CPU oracle (the C++ restatement of the reference's K17 step, OpenMP over z-slabs) on the
     headline workload, Taylor-Green 256^3 periodic, in its FP64 build (the reference's CPU
@@ -590,6 +610,9 @@ def main():
         if steady is not None:
             r["steady"] = steady
         r["mass_drift"] = float(L.moment_sums()[0])
+        if storage == "fp32":
+            # after the timed window: the same kernel without the collision on the same state
+            r["row_pattern_GBps"] = row_pattern_bandwidth(L)
         results[storage] = r
         if storage == "fp32" and not args.no_adjoint:
             ta = time_adjoint(L, args.adjoint_steps, 5)
@@ -660,6 +683,9 @@ def main():
         "blit_copy_GBps": round(blit_gbps, 1),
         "inplace27": rmw,
         "vs_inplace27": round(r["gbps"] / rmw["best_GBps"], 4) if main_key == "fp32" else None,
+        # the kernel against its own access pattern without the collision (row_pattern_bandwidth)
+        "row_pattern_GBps": r.get("row_pattern_GBps"),
+        "vs_row_pattern": round(r["gbps"] / r["row_pattern_GBps"], 4) if "row_pattern_GBps" in r else None,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
     print(json.dumps(line), flush=True)
