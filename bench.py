@@ -524,6 +524,78 @@ def storage_precision(n=64, steps=200):
     return out
 
 
+def interface_kernels(reps=200):
+    """The refinement interface of the two-level case alone (258 x 130 x 130 coarse and 128^3
+    fine FP16 lattices, the nested box of two_level_gradient): each interface kernel launched
+    `reps` times back to back on its full cell list and on prefixes of it (1/2, 1/4, 1/8, 1/16),
+    with the time per launch; and, counted on the host from the cell lists, the distinct 64-byte
+    segments the fine-to-coarse kernel's 27 gathers per lane touch (each 64-lane wave holds 8
+    cells; summed over the waves) and the distinct 128-byte lines of the launch."""
+    import numpy as np
+    import torch
+
+    from virtualfluids_amd.grid import CX, CY, CZ, channel
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.refinement import (DeviceInterfaceCells, GridScalingFactory, GridScalingKernelManager,
+                                              InterfaceCells, nested_box)
+
+    nb = nested_box(None, (65, 33, 33), (129, 97, 97), coarse=channel((256, 128, 128), inflow=0.03))
+    C = Lattice(nb.coarse, 1.8, storage="fp16", level=0)
+    F = Lattice(nb.fine, 1.7, storage="fp16", level=1)
+    for L in (C, F):
+        z = np.zeros(L.N)
+        L.init_equilibrium(z, np.full(L.N, 0.03), z, z)
+    fac = GridScalingFactory()
+    fac.set_scaling_factory(GridScalingFactory.GridScaling.ScaleCompressible)
+    man = GridScalingKernelManager(2, fac, nb.fc.n, nb.cf.n)
+    out = {"case": "refinement interface of the two-level channel, FP16, cells and us per launch",
+           "fine_to_coarse": {}, "coarse_to_fine": {}}
+    for frac in (1.0, 0.5, 0.25, 0.125, 0.0625):
+        for key, cells, run in (("fine_to_coarse", nb.fc, man.run_fine_to_coarse),
+                                ("coarse_to_fine", nb.cf, man.run_coarse_to_fine)):
+            n = max(1, int(cells.n * frac))
+            dc = DeviceInterfaceCells(InterfaceCells(cells.coarse[:n].copy(), cells.fine[:n].copy(), None), "cuda")
+            for _ in range(5):
+                run(C, F, dc)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                run(C, F, dc)
+            e1.record()
+            torch.cuda.synchronize()
+            out[key][str(n)] = round(e0.elapsed_time(e1) / reps * 1e3, 2)
+    # the fine-to-coarse kernel's gathers: lane 8 c + q reads the 27 populations of corner q of
+    # cell c's fine source cell, direction d from the cell node moved by +1 where c_d < 0
+    g = nb.fine
+
+    def moved(k, bits):
+        if bits & 1:
+            k = g.nbx[k]
+        if bits & 2:
+            k = g.nby[k]
+        if bits & 4:
+            k = g.nbz[k]
+        return k.astype(np.int64)
+
+    base = nb.fc.fine.astype(np.int64)
+    corners = np.stack([moved(base, q) for q in range(8)], 1).reshape(-1)
+    waves = (corners.size + 63) // 64
+    pad = waves * 64 - corners.size
+    segments, lines = 0, 0
+    for d in range(27):
+        sel = (1 if CX[d] < 0 else 0) | (2 if CY[d] < 0 else 0) | (4 if CZ[d] < 0 else 0)
+        addr = (d * g.N + moved(corners, sel)) * 2
+        lines += np.unique(addr // 128).size
+        a = np.concatenate([addr, np.repeat(addr[-1:], pad)]).reshape(waves, 64) // 64
+        a.sort(axis=1)
+        segments += int(waves + (np.diff(a, axis=1) != 0).sum())
+    out["fine_to_coarse_gathers"] = {"cells": int(nb.fc.n), "waves": int(waves), "segments_64B": segments,
+                                     "lines_128B": int(lines), "line_MB": round(lines * 128 / 1e6, 1),
+                                     "population_MB": round(nb.fc.n * 8 * 27 * 2 / 1e6, 2)}
+    return out
+
+
 def run_case(case, steps=None):
     """One full-size case alone (profiling runs): its record, with `steps` steps when given (the
     sphere channel averages its drag over the last tenth, the two-level channel its outlet energy
@@ -536,6 +608,8 @@ def run_case(case, steps=None):
             two_level_gradient(steps=steps, average_last=max(1, steps // 4))
         out.pop("_grad")
         return out
+    if case == "interface":
+        return interface_kernels()
     return published_cavity() if steps is None else published_cavity(steps)
 
 
@@ -555,7 +629,7 @@ def main():
     ap.add_argument("--no-cavity", action="store_true", help="skip the reference's published cavity benchmark")
     ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
                     help="time one storage type only (for profiling runs)")
-    ap.add_argument("--only-case", choices=["sphere", "two-level", "cavity"], default=None,
+    ap.add_argument("--only-case", choices=["sphere", "two-level", "cavity", "interface"], default=None,
                     help="run one full-size case alone and print its record (for profiling runs)")
     ap.add_argument("--case-steps", type=int, default=None,
                     help="steps of --only-case (default: the case's full length)")
