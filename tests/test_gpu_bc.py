@@ -156,3 +156,54 @@ def test_fused_bounce_back_matches_separate_launches(gpu, storage, graphs, row_k
         sim.run(37, graphs=graphs)
         out.append(L.populations())
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_joined_bc_launches_match_separate_launches(gpu, storage):
+    """A sphere channel whose wall and sphere lists share the no-slip launcher: the step that
+    launches them as one list (post_launch_groups) against the two launches, through Simulation
+    with HIP graphs, the Stepper's out-of-place step and its adjoint step: every population, the
+    adjoint buffer and the inflow-velocity adjoint bit-identical."""
+    import torch
+
+    from virtualfluids_amd.adjoint import Stepper
+    from virtualfluids_amd.grid import sphere_channel
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory as F, Simulation
+
+    grid = sphere_channel((30, 16, 16), (11, 8, 8), 3.6, 0.04)
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_geometry_boundary_condition(F.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(F.PressureBC.PressureNonEquilibriumCompressible)
+    rng = np.random.default_rng(29)
+    N = grid.N
+    init = (rng.normal(0, 0.01, N), 0.04 + rng.normal(0, 0.01, N), rng.normal(0, 0.01, N), rng.normal(0, 0.01, N))
+    lam0 = rng.normal(0, 1, 27 * N).astype(np.float32)
+    out = []
+    for joined in (True, False):
+        L = Lattice(grid, 1.8, storage=storage)
+        L.init_equilibrium(*init)
+        sim = Simulation(L, fac)
+        st = Stepper(L, fac)
+        if not joined:
+            separate = [(name, fn, L.bcs[name]) for name, fn in
+                        (("velocity", sim.velocity_fn), ("noslip", sim.noslip_fn), ("geometry", sim.geometry_fn))]
+            sim.groups = separate
+            st._launch_groups = [(name, fn, bc) for name, fn, bc in separate]
+        sim.run(41, graphs=True)
+        assert [name for name, _, _ in sim.groups] == (["velocity", "noslip+geometry"] if joined else
+                                                      ["velocity", "noslip", "geometry"])
+        posts = {name: torch.zeros(27 * n, device="cuda") for name, n in st.post_lists()}
+        before = L.f.clone()
+        after = st.forward(before, True, torch.empty_like(before), None, save=posts)
+        lam = torch.from_numpy(lam0).cuda()
+        ubar = torch.zeros(3 * grid.bcs["velocity"].n, device="cuda")
+        rbar = torch.zeros(grid.bcs["pressure"].n, device="cuda")
+        st.adjoint(before, after, lam, True, ubar, rbar, None, None, saved=posts)
+        torch.cuda.synchronize()
+        print(f"{storage} joined={joined}", flush=True)
+        out.append([L.populations(), after.cpu().numpy(), lam.cpu().numpy(), ubar.cpu().numpy()])
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)

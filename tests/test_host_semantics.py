@@ -99,6 +99,39 @@ def test_bc_list_layout_hint_follows_the_nodes():
         DeviceBC(BoundaryNodes(k=np.array([4, 5, 4]), q27=np.full((27, 3), -1.0)), "cpu", 240)
 
 
+def test_post_launch_groups_join_lists_with_one_launcher_and_no_common_node():
+    """post_launch_groups: the sphere channel's walls and sphere (one no-slip launcher, no common
+    node) become one launch over both lists, in list order; the inflow (wall velocities) stays on
+    its own; lists with different launchers, or sharing a node, keep their own launches."""
+    from virtualfluids_amd.grid import sphere_channel
+    from virtualfluids_amd.lattice import DeviceBC
+    from virtualfluids_amd.simulation import post_launch_groups
+
+    class Host:
+        pass
+
+    g = sphere_channel((24, 14, 14), (9, 7, 7), 3.3, 0.04)
+    L = Host()
+    L.grid, L.device, L.N = g, "cpu", g.N
+    L.bcs = {name: DeviceBC(b, "cpu", g.N) for name, b in g.bcs.items()}
+    ns, vel = "vf_NoSlipInterpolatedCompressible", "vf_VelocityInterpolatedCompressible"
+    groups = post_launch_groups(L, [("velocity", vel), ("noslip", ns), ("geometry", ns)])
+    assert [(name, fn) for name, fn, _ in groups] == [("velocity", vel), ("noslip+geometry", ns)]
+    joined = groups[1][2]
+    wall, sphere = g.bcs["noslip"], g.bcs["geometry"]
+    assert joined.n == wall.n + sphere.n
+    assert np.array_equal(joined.k.numpy(), np.concatenate([wall.k, sphere.k]).astype(np.int32))
+    q = joined.q27.numpy().reshape(27, joined.n)
+    assert np.array_equal(q[:, :wall.n], np.asarray(wall.q27, np.float32).reshape(27, wall.n))
+    assert np.array_equal(q[:, wall.n:], np.asarray(sphere.q27, np.float32).reshape(27, sphere.n))
+    groups = post_launch_groups(L, [("noslip", ns), ("geometry", "vf_NoSlipBounceBack")])
+    assert [name for name, _, _ in groups] == ["noslip", "geometry"]
+    g.bcs["twin"] = g.bcs["noslip"]
+    L.bcs["twin"] = L.bcs["noslip"]
+    groups = post_launch_groups(L, [("noslip", ns), ("twin", ns)])
+    assert [name for name, _, _ in groups] == ["noslip", "twin"]
+
+
 def test_interface_sources_list_every_cell_corner_once():
     """vf_interpolation_sources built on the host: every lane 8 i + q appears once, under the node
     that is corner q of coarse cell i (+x, +y, +z steps through the neighbour arrays)."""

@@ -91,6 +91,7 @@ class Stepper:
     def __init__(self, L: Lattice, bc_factory: BoundaryConditionFactory | None = None):
         f = bc_factory or BoundaryConditionFactory()
         self.L = L
+        self._launch_groups = None
         self.velocity = f.get_velocity_boundary_condition_post()
         self.noslip = f.get_no_slip_boundary_condition_post()
         self.pressure = f.get_pressure_boundary_condition_pre()
@@ -105,14 +106,20 @@ class Stepper:
         bc = self.L.bcs.get(name)
         return bc if bc is not None and bc.n > 0 else None
 
+    def _groups(self):
+        """The post-collision launches of a step, (name, launcher, DeviceBC) in launch order
+        (simulation.post_launch_groups); built on first use."""
+        if self._launch_groups is None:
+            from .simulation import post_launch_groups
+
+            self._launch_groups = post_launch_groups(self.L, [("velocity", self.velocity), ("noslip", self.noslip),
+                                                              ("geometry", self.geometry)])
+        return self._launch_groups
+
     def post_lists(self):
-        """(name, node count) of the post-collision BC lists a step applies, in launch order."""
-        out = []
-        for name, fn in (("velocity", self.velocity), ("noslip", self.noslip), ("geometry", self.geometry)):
-            bc = self._bc(name)
-            if fn and bc is not None:
-                out.append((name, bc.n))
-        return out
+        """(name, node count) of the post-collision launches a step makes, in launch order (a
+        launch over two joined lists is named "a+b")."""
+        return [(name, bc.n) for name, _, bc in self._groups()]
 
     def forward(self, buf: torch.Tensor, even: bool, out: torch.Tensor | None, stream, save=None) -> torch.Tensor:
         """One step from `buf` (parity `even`); into `out` when given, else in place. Returns
@@ -136,11 +143,9 @@ class Stepper:
                 capi.call("vf_k17_run_on_indices", ctypes.byref(p), _vp(L.fluid), int(L.grid.fluid.size),
                           capi.COLLISION_DEFAULT, s)
         p = _param(L, target, even)
-        for name, fn in (("velocity", self.velocity), ("noslip", self.noslip), ("geometry", self.geometry)):
-            bc = self._bc(name)
-            if fn and bc is not None:
-                post = save.get(name) if save else None
-                capi.call(fn, ctypes.byref(p), ctypes.byref(_bc_struct(bc, post)), s)
+        for name, fn, bc in self._groups():
+            post = save.get(name) if save else None
+            capi.call(fn, ctypes.byref(p), ctypes.byref(_bc_struct(bc, post)), s)
         bc = self._bc("pressure")
         if self.pressure and bc is not None:
             capi.call(self.pressure, ctypes.byref(_param(L, target, not even)), ctypes.byref(bc.struct), s)
@@ -158,12 +163,10 @@ class Stepper:
             capi.call(self.pressure + "Adjoint", ctypes.byref(_param(L, after, not even)), ctypes.byref(bc.struct),
                       _vp(lam), _vp(rbar), int(bc.kN_distinct), s)
         p = _param(L, before, even)
-        for name, fn, grad in (("geometry", self.geometry, None), ("noslip", self.noslip, None),
-                               ("velocity", self.velocity, ubar)):
-            bc = self._bc(name)
-            if fn and bc is not None:
-                post = saved.get(name) if saved else None
-                capi.call(fn + "Adjoint", ctypes.byref(p), ctypes.byref(_bc_struct(bc, post)), _vp(lam), _vp(grad), s)
+        for name, fn, bc in reversed(self._groups()):
+            post = saved.get(name) if saved else None
+            grad = ubar if name == "velocity" else None
+            capi.call(fn + "Adjoint", ctypes.byref(p), ctypes.byref(_bc_struct(bc, post)), _vp(lam), _vp(grad), s)
         fn = "vf_k17_adjoint_box" if L.box else "vf_k17_adjoint"
         capi.call(fn, ctypes.byref(p), _vp(lam), None, _vp(padj), s)
 

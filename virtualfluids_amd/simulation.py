@@ -91,6 +91,43 @@ class BoundaryConditionFactory:
         return self._resolve(table, self.geometry)
 
 
+# launchers whose lists carry wall velocities (and, in the adjoint, their gradient)
+_VELOCITY_LAUNCHERS = ("vf_VelocityBounceBack", "vf_VelocityInterpolatedCompressible",
+                       "vf_VelocityWithPressureInterpolatedCompressible")
+
+
+def post_launch_groups(lattice: Lattice, lists):
+    """The launches of the post-collision lists [(name, launcher)], in launch order, as
+    [(name, launcher, DeviceBC)]. A list without wall velocities that follows another list with
+    the same launcher and no node in common (the channel walls, then an obstacle as the
+    geometry list) joins that list's launch, named "a+b": every node keeps its one thread and its
+    links, and a link writes a cell of the node across the wall, which no node of either list
+    reads, so every value is what the two launches leave, and the adjoint keeps one thread per
+    node as well. A short launch costs a few microseconds of latency whatever its size; the
+    reference launches the lists one by one (UpdateGrid27.cpp:69-77)."""
+    import numpy as np
+
+    from .grid import BoundaryNodes
+    from .lattice import DeviceBC
+
+    out = []
+    for name, fn in lists:
+        bc = lattice.bcs.get(name)
+        if not fn or bc is None or bc.n == 0:
+            continue
+        if out:
+            pname, pfn, pbc = out[-1]
+            host = [lattice.grid.bcs[part] for part in pname.split("+")] + [lattice.grid.bcs[name]]
+            if (fn == pfn and fn not in _VELOCITY_LAUNCHERS
+                    and np.unique(np.concatenate([h.k for h in host])).size == sum(h.n for h in host)):
+                k = np.concatenate([np.asarray(h.k) for h in host])
+                q = np.concatenate([np.asarray(h.q27).reshape(27, h.n) for h in host], axis=1).reshape(-1)
+                out[-1] = (pname + "+" + name, fn, DeviceBC(BoundaryNodes(k=k, q27=q), lattice.device, lattice.N))
+                continue
+        out.append((name, fn, bc))
+    return out
+
+
 class BCEntryTables:
     """Per-node tables of post-collision BC lists for a kernel that applies the lists itself
     (the fused bounce-back collision): per node its entry or -1, and per entry
@@ -211,6 +248,7 @@ class Simulation:
                 raise RuntimeError(f"The boundary condition {label} was not set!")
         self.t = 0
         self.fused = None
+        self.groups = None
         post = [("velocity", self.velocity_fn), ("noslip", self.noslip_fn), ("geometry", self.geometry_fn)]
         post = [(name, fn) for name, fn in post if fn and lattice.bcs.get(name) is not None and lattice.bcs[name].n]
         if fuse_bounce_back and getattr(lattice, "box", False) and post and all(fn in _BOUNCE_BACK for _, fn in post):
@@ -231,15 +269,19 @@ class Simulation:
             L.apply_bc(self.pressure_fn, "pressure", stream)
 
     def _post_collision(self, stream):
-        """Collision and the post-collision BCs as separate launches."""
+        """Collision and the post-collision BCs as separate launches (post_launch_groups)."""
+        import ctypes
+
+        from . import capi
+        from .lattice import stream_handle
+
         L = self.lattice
         L.collide(stream=stream)
-        if self.velocity_fn:
-            L.apply_bc(self.velocity_fn, "velocity", stream)
-        if self.noslip_fn:
-            L.apply_bc(self.noslip_fn, "noslip", stream)
-        if self.geometry_fn:
-            L.apply_bc(self.geometry_fn, "geometry", stream)
+        if self.groups is None:
+            self.groups = post_launch_groups(L, [("velocity", self.velocity_fn), ("noslip", self.noslip_fn),
+                                                 ("geometry", self.geometry_fn)])
+        for _, fn, bc in self.groups:
+            capi.call(fn, L.pref(), ctypes.byref(bc.struct), stream_handle(stream))
 
     def step(self, stream=None):
         self._launch(stream)
