@@ -108,7 +108,7 @@ def post_launch_groups(lattice: Lattice, lists):
     import numpy as np
 
     from .grid import BoundaryNodes
-    from .lattice import DeviceBC
+    from .lattice import DeviceBC, nodes_along_rows
 
     out = []
     for name, fn in lists:
@@ -118,9 +118,12 @@ def post_launch_groups(lattice: Lattice, lists):
         if out:
             pname, pfn, pbc = out[-1]
             host = [lattice.grid.bcs[part] for part in pname.split("+")] + [lattice.grid.bcs[name]]
-            if (fn == pfn and fn not in _VELOCITY_LAUNCHERS
-                    and np.unique(np.concatenate([h.k for h in host])).size == sum(h.n for h in host)):
-                k = np.concatenate([np.asarray(h.k) for h in host])
+            k = np.concatenate([np.asarray(h.k) for h in host])
+            # the joined list takes the kernel form (a thread per node or a lane per direction)
+            # that each list takes alone, so that it rounds as they do
+            rows = {bool(pbc.struct.nodesAlongRows), bool(bc.struct.nodesAlongRows), nodes_along_rows(k)}
+            if (fn == pfn and fn not in _VELOCITY_LAUNCHERS and len(rows) == 1
+                    and np.unique(k).size == k.size):
                 q = np.concatenate([np.asarray(h.q27).reshape(27, h.n) for h in host], axis=1).reshape(-1)
                 out[-1] = (pname + "+" + name, fn, DeviceBC(BoundaryNodes(k=k, q27=q), lattice.device, lattice.N))
                 continue
