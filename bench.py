@@ -176,7 +176,8 @@ def sphere_channel_gradient(steps=2000, checkpoint_every=100, average_last=200, 
     return out
 
 
-def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, storage="fp16", modes=(8, 4)):
+def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, storage="fp16", modes=(8, 4),
+                       keep_fraction=0.6):
     """Two-level 2:1 nested refinement: coarse channel of 256 x 128 x 128 fluid nodes (velocity
     inflow, pressure outlet, no-slip walls), Re 500 on the coarse level (inflow 0.03, channel
     height 128), a fine box of 128^3 nodes over the coarse cells [64, 128) x [32, 96)^2 around a
@@ -186,7 +187,10 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
     from the sine series of the uniform 0.03 inflow. Objective: the kinetic energy of the last
     coarse fluid plane before the outlet, averaged over the last `average_last` coarse steps; its
     gradient with respect to the 32 coefficients through k17_two_level_steps. Times a
-    forward-only run and the forward + backward run."""
+    forward-only run and two forward + backward runs: every segment replayed in the backward,
+    and (keep_fraction > 0) with keep_bytes = keep_fraction of the free device memory, where the
+    trailing segments that fit keep their states from the forward (the same gradient bit for
+    bit)."""
     import numpy as np
     import torch
 
@@ -230,13 +234,13 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
     phi_t = torch.from_numpy(phi).cuda()
     zero = torch.zeros(vel.n, device="cuda")
 
-    def objective(a, n=steps, observe_from=steps - average_last):
+    def objective(a, n=steps, observe_from=steps - average_last, keep=0):
         ux = a.to(torch.float32) @ B
         U = torch.stack([ux, zero, zero])
         return k17_two_level_steps(C.f.detach().clone(), Fl.f.detach().clone(), C, Fl, (nb.cf, nb.fc), n,
                                    checkpoint_every=checkpoint_every, even=True, bc_factory=fac, velocity=U,
                                    porosity_fine=phi_t,
-                                   observe=outlet_ke, observe_from=observe_from)[2]
+                                   observe=outlet_ke, observe_from=observe_from, keep_bytes=keep)[2]
 
     a = torch.from_numpy(a0).cuda()
     # untimed warm-up over two checkpoint segments, forward and backward (first-use costs)
@@ -249,13 +253,22 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
         J0 = float(objective(a))
     torch.cuda.synchronize()
     t_forward = time.perf_counter() - t0
-    a.requires_grad_(True)
-    t0 = time.perf_counter()
-    J = objective(a)
-    J.backward()
-    torch.cuda.synchronize()
-    t_gradient = time.perf_counter() - t0
-    g = a.grad.double().cpu().numpy()
+    def gradient(keep):
+        x = a.detach().clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        objective(x, keep=keep).backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, x.grad.double().cpu().numpy()
+
+    t_recompute, g_recompute = gradient(0)
+    keep = int(keep_fraction * torch.cuda.mem_get_info()[0])
+    t_gradient, g = gradient(keep) if keep_fraction > 0 else (t_recompute, g_recompute)
+    # the states a kept coarse step holds (two fine buffers, the coarse buffer, the BC values)
+    lists = 27 * 4 * sum(bc.n for name, bc in C.bcs.items() if name != "pressure")
+    step_bytes = 2 * Fl.f.numel() * Fl.f.element_size() + C.f.numel() * C.f.element_size() + lists
+    segments = -(-steps // checkpoint_every)
+    kept = min(segments, keep // (checkpoint_every * step_bytes)) if keep_fraction > 0 else 0
     nodes = int(nb.coarse.fluid.size + nb.fine.N)
     out = {"case": f"two-level channel 256x128x128 coarse + 128^3 fine around a porous obstacle, Re 500, {storage} "
                    f"storage, {steps} coarse steps, checkpoint every {checkpoint_every}, outlet kinetic energy "
@@ -264,8 +277,14 @@ def two_level_gradient(steps=4000, average_last=1000, checkpoint_every=50, stora
            "grad_first4": [float(v) for v in g[:4]], "_grad": g,
            "forward_s": round(t_forward, 3), "gradient_s": round(t_gradient, 3),
            "gradient_vs_forward": round(t_gradient / t_forward, 3),
+           "reverse_vs_forward": round((t_gradient - t_forward) / t_forward, 3),
            "forward_MLUPS": round((nb.coarse.fluid.size + 2 * nb.fine.N) * steps / t_forward / 1e6, 1)}
-    del C, Fl, J
+    if keep_fraction > 0:
+        out.update({"keep_GB": round(keep / 1e9, 1), "kept_segments": int(kept), "segments": segments,
+                    "recompute": {"gradient_s": round(t_recompute, 3),
+                                  "reverse_vs_forward": round((t_recompute - t_forward) / t_forward, 3)},
+                    "kept_equals_recompute": bool(np.array_equal(g, g_recompute))})
+    del C, Fl
     torch.cuda.empty_cache()
     return out
 
@@ -721,12 +740,13 @@ def main():
     two_level = None
     if not (args.no_two_level or args.no_adjoint):
         two_level = two_level_gradient()
-        ref32 = two_level_gradient(storage="fp32")
+        ref32 = two_level_gradient(storage="fp32", keep_fraction=0.0)
         g16, g32 = two_level.pop("_grad"), ref32.pop("_grad")
         import numpy as np
 
         two_level["fp32_storage"] = {k: ref32[k] for k in ("objective", "grad_l2", "forward_s", "gradient_s",
-                                                           "gradient_vs_forward", "forward_MLUPS")}
+                                                           "gradient_vs_forward", "reverse_vs_forward",
+                                                           "forward_MLUPS")}
         two_level["fp16_vs_fp32_gradient_rel_l2"] = float(np.linalg.norm(g16 - g32) / np.linalg.norm(g32))
         two_level["fp16_vs_fp32_objective_rel"] = abs(two_level["objective"] / ref32["objective"] - 1.0)
     line = {

@@ -194,3 +194,56 @@ def test_two_level_sixteen_bit_gradients_match_fp64_differences(gpu, orc, storag
     ad = float(phi.grad.cpu().numpy().astype(np.float64) @ w)
     print(f"  {storage} dJ/dphi_fine . w: adjoint {ad:.8e} FD {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}")
     assert abs(ad - fd) < tol * abs(fd)
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+@pytest.mark.parametrize("keep_segments", [1, 2, 10])
+def test_two_level_kept_segments_give_the_same_gradient(gpu, storage, keep_segments):
+    """keep_bytes on two levels: the trailing segments that fit are stepped out of place in the
+    forward and walked by the backward without a replay. The channel with a nested fine box,
+    porosity on both levels and the observed plane energy: outputs, observed sum and every
+    gradient (both initial buffers, the inflow velocities, both porosity fields) equal to those of
+    the pure-recompute run bit for bit (10 segments: all of them kept)."""
+    import torch
+
+    from virtualfluids_amd.adjoint import KineticEnergyObservation, k17_two_level_steps
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory as Fac
+
+    nb, plane, phi = _case()
+    steps, K = 9, 2
+    fac = Fac()
+    fac.set_velocity_boundary_condition(Fac.VelocityBC.VelocityInterpolatedCompressible)
+    fac.set_no_slip_boundary_condition(Fac.NoSlipBC.NoSlipInterpolatedCompressible)
+    fac.set_pressure_boundary_condition(Fac.PressureBC.PressureNonEquilibriumCompressible)
+    Lc = Lattice(nb.coarse, OMEGA_C, storage=storage, level=0)
+    Lf = Lattice(nb.fine, OMEGA_F, storage=storage, level=1)
+    for L in (Lc, Lf):
+        z = np.zeros(L.N)
+        L.init_equilibrium(z, np.full(L.N, 0.03), z, z)
+    rng = np.random.default_rng(1640)
+    pc0 = rng.uniform(0, 0.03, nb.coarse.N).astype(np.float32)
+    pf0 = phi.astype(np.float32)
+    u0 = np.stack([np.asarray(v, dtype=np.float32) for v in nb.coarse.bcs["velocity"].velocity])
+    lists = 27 * 4 * sum(bc.n for name, bc in Lc.bcs.items() if name != "pressure")
+    step_bytes = 2 * Lf.f.numel() * Lf.f.element_size() + Lc.f.numel() * Lc.f.element_size() + lists
+    seg = K * (step_bytes + Lc.f.numel() * Lc.f.element_size() + Lf.f.numel() * Lf.f.element_size())
+
+    def run(keep):
+        xc = Lc.f.detach().float().clone().requires_grad_(True)
+        xf = Lf.f.detach().float().clone().requires_grad_(True)
+        U = torch.from_numpy(u0).cuda().requires_grad_(True)
+        pc = torch.from_numpy(pc0).cuda().requires_grad_(True)
+        pf = torch.from_numpy(pf0).cuda().requires_grad_(True)
+        obs = KineticEnergyObservation(nb.coarse, plane, scale=0.25)
+        c, f, J = k17_two_level_steps(xc, xf, Lc, Lf, (nb.cf, nb.fc), steps, checkpoint_every=K, even=True,
+                                      bc_factory=fac, velocity=U, porosity_coarse=pc, porosity_fine=pf,
+                                      observe=obs, observe_from=5, keep_bytes=keep)
+        (J + 1e-3 * (c.double() ** 2).sum() + 1e-3 * (f.double() ** 2).sum()).backward()
+        return [t.detach().cpu().numpy() for t in (c, f, J, xc.grad, xf.grad, U.grad, pc.grad, pf.grad)]
+
+    print(f"{storage}: kept segments {keep_segments} of {-(-steps // K)}", flush=True)
+    ref = run(0)
+    got = run(keep_segments * seg)
+    for name, a, b in zip(("c", "f", "J", "gc", "gf", "gU", "gpc", "gpf"), ref, got):
+        assert np.array_equal(a, b), name

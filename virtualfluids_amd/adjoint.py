@@ -351,7 +351,7 @@ class _TwoLevelSteps(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, fc0, ff0, velocity, rho_bc, por_c, por_f, sc, sf, cells, steps, K, even0, observe, observe_from,
-                graphs, in_place):
+                graphs, in_place, keep_bytes):
         Lc, Lf = sc.L, sf.L
         # in place on one buffer per level: the same arithmetic as the out-of-place replay
         C, F = _storage_input(fc0, Lc, "fc0"), _storage_input(ff0, Lf, "ff0")
@@ -367,12 +367,47 @@ class _TwoLevelSteps(torch.autograd.Function):
         ctx.params = (_bc_snapshot(Lc), _bc_snapshot(Lf))
         run = _StepGraphs(lambda ev, st: _two_level_step_in_place(sc, sf, cells, C, F, ev, st), graphs)
         checkpoints = []
+        # the trailing segments whose states fit keep_bytes are stepped out of place and kept, so
+        # the backward walks them without a replay (the states of a coarse step: two fine
+        # buffers, the coarse buffer, the interface copies where the writes are not in place, and
+        # the BC lists' post-collision values)
+        lists_c, lists_f = sc.post_lists(), sf.post_lists()
+        step_bytes = (2 * F.numel() * F.element_size() + C.numel() * C.element_size() +
+                      (0 if in_place else C.numel() * C.element_size() + F.numel() * F.element_size()) +
+                      4 * 27 * (sum(nb for _, nb in lists_c) + 2 * sum(nb for _, nb in lists_f)))
+        kept_from = steps
+        while kept_from > 0:
+            start = (kept_from - 1) // K * K
+            if (steps - start) * step_bytes > keep_bytes:
+                break
+            kept_from = start
+        ctx.kept = {}
+        stream = torch.cuda.current_stream()
         even = even0
         J = torch.zeros((), dtype=torch.float64, device=fc0.device)
         i = 0
         while i < steps:
             if i % K == 0:
-                checkpoints.append((i, even, C.clone(), F.clone()))
+                # a kept segment needs no copy of its first state: it keeps all of them
+                checkpoints.append((i, even, C.clone(), F.clone()) if i < kept_from else (i, even, None, None))
+            if i >= kept_from:
+                # a kept segment: every state of it, as the backward's replay would record it
+                n = min(K, steps - i)
+                states, posts = [], []
+                for j in range(n):
+                    pj = ({name: torch.empty(27 * nb, dtype=torch.float32, device=C.device) for name, nb in lists_c},
+                          {name: torch.empty(27 * nb, dtype=torch.float32, device=C.device) for name, nb in lists_f},
+                          {name: torch.empty(27 * nb, dtype=torch.float32, device=C.device) for name, nb in lists_f})
+                    st = _two_level_step(sc, sf, cells, C, F, even, stream, pj, in_place)
+                    states.append((even, C, F) + st)
+                    posts.append(pj)
+                    C, F = st[-2], st[-1]
+                    even = not even
+                    if observe is not None and i + j >= observe_from:
+                        J += observe(C, even).detach().to(torch.float64)
+                ctx.kept[i] = (states, posts)
+                i += n
+                continue
             if observe is not None and i >= observe_from:
                 run(even)  # one step: the observation follows every step from here on
                 even = not even
@@ -390,7 +425,8 @@ class _TwoLevelSteps(torch.autograd.Function):
         ctx.checkpoints = checkpoints
         ctx.flags = (velocity is not None, rho_bc is not None, por_c is not None, por_f is not None)
         ctx.in_dtypes = (fc0.dtype, ff0.dtype)
-        return C.to(fc0.dtype), F.to(ff0.dtype), J
+        # the outputs are copies where they are kept states
+        return C.to(fc0.dtype, copy=bool(ctx.kept)), F.to(ff0.dtype, copy=bool(ctx.kept)), J
 
     @staticmethod
     def backward(ctx, gC, gF, gJ):
@@ -432,15 +468,19 @@ class _TwoLevelSteps(torch.autograd.Function):
 
         for start, even_start, C0, F0 in reversed(ctx.checkpoints):
             n = min(K, steps - start)
-            states = []
-            # the post-collision values of every level's BC nodes, step by step, for the BC adjoints
-            posts = [(post_buffers(lists_c), post_buffers(lists_f), post_buffers(lists_f)) for _ in range(n)]
-            C, F, even = C0, F0, even_start
-            for j in range(n):
-                st = _two_level_step(sc, sf, cells, C, F, even, stream, posts[j], ctx.in_place)
-                states.append((even, C, F) + st)
-                C, F = st[-2], st[-1]
-                even = not even
+            if start in ctx.kept:
+                states, posts = ctx.kept.pop(start)
+            else:
+                states = []
+                # the post-collision values of every level's BC nodes, step by step, for the BC
+                # adjoints
+                posts = [(post_buffers(lists_c), post_buffers(lists_f), post_buffers(lists_f)) for _ in range(n)]
+                C, F, even = C0, F0, even_start
+                for j in range(n):
+                    st = _two_level_step(sc, sf, cells, C, F, even, stream, posts[j], ctx.in_place)
+                    states.append((even, C, F) + st)
+                    C, F = st[-2], st[-1]
+                    even = not even
             for j, (even, C0s, F0s, F1, F2, C1, C2, _, _) in reversed(list(enumerate(states))):
                 post_c, post_f1, post_f2 = posts[j]
                 if ctx.observe is not None and start + j >= ctx.observe_from:
@@ -461,9 +501,10 @@ class _TwoLevelSteps(torch.autograd.Function):
                 sf.adjoint(F0s, F1, lf, True, None, None, pfb, stream, saved=post_f1)
             del states, posts
         ctx.checkpoints = None
+        ctx.kept = None
         gv = ubar.view(3, -1) if ubar is not None else None
         return (lc.to(ctx.in_dtypes[0]), lf.to(ctx.in_dtypes[1]), gv, rbar, pcb, pfb, None, None, None, None, None,
-                None, None, None, None, None)
+                None, None, None, None, None, None)
 
 
 def _observe_grad(observe, buf, even, lam, gJ):
@@ -563,7 +604,7 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
                         fine_bc_factory: BoundaryConditionFactory | None = None, velocity: torch.Tensor | None = None,
                         rho_bc: torch.Tensor | None = None, porosity_coarse: torch.Tensor | None = None,
                         porosity_fine: torch.Tensor | None = None, observe=None, observe_from: int = 0,
-                        graphs: bool = True):
+                        graphs: bool = True, keep_bytes: int = 0):
     """`steps` coarse steps of a two-level pair from (fc0, ff0), differentiable in both buffers and,
     when given, in the coarse velocity BC values (3, n), the coarse outlet densities and the
     porosity of either level. interface: (cf, fc) InterfaceCells of the pair (NestedBox.cf,
@@ -580,6 +621,11 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
     average over thousands of steps needs no stored states beyond the checkpoints.
     graphs: the forward sweep replays one HIP graph per coarse parity (False: every launch from
     the host; the results are the same bit for bit).
+    keep_bytes: device memory the forward may keep for the backward. The trailing checkpoint
+    segments whose coarse steps' states (two fine buffers, the coarse buffer and the BC lists'
+    post-collision values per coarse step) fit in it are stepped out of place and keep every
+    state, so the backward walks them without a replay; the outputs, the observed sum and the
+    gradients are the same bit for bit. Ignored when nothing requires a gradient.
     Returns (coarse buffer, fine buffer, observed sum)."""
     from .refinement import DeviceInterfaceCells
 
@@ -597,8 +643,11 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
     even0 = coarse.even if even is None else bool(even)
     # the replay of the backward applies the interface in place where that changes nothing it reads
     in_place = interface_writes_disjoint(coarse.grid, fine.grid, cf, fc)
+    inputs = (fc0, ff0, velocity, rho_bc, porosity_coarse, porosity_fine)
+    if not torch.is_grad_enabled() or not any(t is not None and t.requires_grad for t in inputs):
+        keep_bytes = 0
     return _TwoLevelSteps.apply(fc0, ff0, velocity, rho_bc, porosity_coarse, porosity_fine, sc, sf, cells, steps,
-                                checkpoint_every, even0, observe, observe_from, graphs, in_place)
+                                checkpoint_every, even0, observe, observe_from, graphs, in_place, int(keep_bytes))
 
 
 # ---- any number of levels ---------------------------------------------------------------------
