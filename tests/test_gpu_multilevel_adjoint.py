@@ -171,3 +171,45 @@ def test_three_level_steps_match_oracle(gpu, orc):
         err = np.abs(out[0][lev] - want[lev]).max()
         print(f"three levels, 6 coarse steps: level {lev} max |GPU - FP64 oracle| {err:.2e}")
         assert err < 2e-6, lev
+
+
+@pytest.mark.parametrize("keep_segments", [1, 3])
+def test_three_level_kept_segments_give_the_same_gradient(gpu, keep_segments):
+    """keep_bytes on three levels: the trailing segments that fit are stepped out of place in the
+    forward and walked by the backward without a replay. Outputs, observed sum and every gradient
+    (the three initial buffers, the inflow velocities, both porosity fields) equal to those of the
+    pure-recompute run bit for bit (3 segments: all of them kept)."""
+    import torch
+
+    import multilevel_reference as ref
+    from virtualfluids_amd.adjoint import KineticEnergyObservation, k17_multilevel_steps
+
+    grids, interfaces, plane, phi2 = ref.three_levels()
+    rng = np.random.default_rng(1650)
+    phi1 = rng.uniform(0, 0.05, grids[1].N).astype(np.float32)
+    f0s = ref.initial_states(grids, 1651)
+    lats = _lattices(grids)
+    u0 = np.stack([np.asarray(v, dtype=np.float32) for v in grids[0].bcs["velocity"].velocity])
+    steps, K = 7, 3
+    lists = [27 * 4 * sum(bc.n for name, bc in L.bcs.items() if name != "pressure") for L in lats]
+    nbytes = [27 * L.N * 4 for L in lats]
+    step_bytes = sum(2 ** l * (nbytes[l] + lists[l] + (nbytes[l] + nbytes[l + 1] if l < 2 else 0)) for l in range(3))
+
+    def run(keep):
+        xs = [torch.from_numpy(f.astype(np.float32)).cuda().requires_grad_(True) for f in f0s]
+        U = torch.from_numpy(u0).cuda().requires_grad_(True)
+        p1 = torch.from_numpy(phi1).cuda().requires_grad_(True)
+        p2 = torch.from_numpy(phi2.astype(np.float32)).cuda().requires_grad_(True)
+        obs = KineticEnergyObservation(grids[0], plane, scale=0.25)
+        outs, J = k17_multilevel_steps(xs, lats, interfaces, steps, checkpoint_every=K, even=True,
+                                       bc_factories=[_factory(), None, None], velocity=U, porosity=[None, p1, p2],
+                                       observe=obs, observe_from=3, keep_bytes=keep)
+        (J + sum(1e-3 * (o.double() ** 2).sum() for o in outs)).backward()
+        return ([o.detach().cpu().numpy() for o in outs] + [J.detach().cpu().numpy()] +
+                [t.grad.cpu().numpy() for t in xs + [U, p1, p2]])
+
+    print(f"three levels: kept segments {keep_segments} of {-(-steps // K)}", flush=True)
+    a = run(0)
+    b = run(keep_segments * K * step_bytes)
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x, y), i

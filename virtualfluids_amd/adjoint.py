@@ -629,8 +629,8 @@ def k17_two_level_steps(fc0: torch.Tensor, ff0: torch.Tensor, coarse: Lattice, f
     Returns (coarse buffer, fine buffer, observed sum)."""
     from .refinement import DeviceInterfaceCells
 
-    if steps < 0 or checkpoint_every < 1:
-        raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
+    if steps < 0 or checkpoint_every < 1 or keep_bytes < 0:
+        raise ValueError("steps must be >= 0, checkpoint_every >= 1 and keep_bytes >= 0")
     if fine.storage != coarse.storage:
         raise ValueError("both levels must use the same storage type")
     # a coarse step cycles through both levels' populations
@@ -735,7 +735,7 @@ class _MultiLevelSteps(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, meta, *args):
-        steppers, cells, steps, K, even0, observe, observe_from, graphs = meta
+        steppers, cells, steps, K, even0, observe, observe_from, graphs, keep_bytes = meta
         n = len(steppers)
         f0s, velocity, rho_bc, pors = args[:n], args[n], args[n + 1], args[n + 2:]
         lats = [s.L for s in steppers]
@@ -752,12 +752,40 @@ class _MultiLevelSteps(torch.autograd.Function):
         ctx.params = [_bc_snapshot(L) for L in lats]
         run = _StepGraphs(lambda ev, st: _level_step_in_place(steppers, cells, bufs, 0, ev, st), graphs)
         checkpoints = []
+        # the trailing segments whose records fit keep_bytes are stepped out of place and kept, so
+        # the backward walks them without a replay. A step of level l (2^l per coarse step) keeps
+        # the level's state after it, the BC lists' post-collision values and, above the finest
+        # level, the level's state after the interface writes and a copy of the finer level's.
+        post_lists = [s.post_lists() for s in steppers]
+        nbytes = [b.numel() * b.element_size() for b in bufs]
+        step_bytes = sum(2 ** l * (nbytes[l] + 4 * 27 * sum(nb for _, nb in post_lists[l]) +
+                                   (nbytes[l] + nbytes[l + 1] if l + 1 < n else 0)) for l in range(n))
+        kept_from = steps
+        while kept_from > 0:
+            start = (kept_from - 1) // K * K
+            if (steps - start) * step_bytes > keep_bytes:
+                break
+            kept_from = start
+        ctx.kept = {}
+        stream = torch.cuda.current_stream()
         even = even0
         J = torch.zeros((), dtype=torch.float64, device=bufs[0].device)
         i = 0
         while i < steps:
             if i % K == 0:
-                checkpoints.append((i, even, [b.clone() for b in bufs]))
+                # a kept segment needs no copy of its first states: it keeps all of them
+                checkpoints.append((i, even, [b.clone() for b in bufs] if i < kept_from else None))
+            if i >= kept_from:
+                records = []
+                for j in range(min(K, steps - i)):
+                    rec, bufs = _level_step_record(steppers, cells, bufs, 0, even, stream, post_lists)
+                    records.append(rec)
+                    even = not even
+                    if observe is not None and i + j >= observe_from:
+                        J += observe(bufs[0], even).detach().to(torch.float64)
+                ctx.kept[i] = records
+                i += len(records)
+                continue
             if observe is not None and i >= observe_from:
                 run(even)  # one step: the observation follows every step from here on
                 even = not even
@@ -772,7 +800,8 @@ class _MultiLevelSteps(torch.autograd.Function):
         ctx.checkpoints = checkpoints
         ctx.flags = (velocity is not None, rho_bc is not None, [p is not None for p in pors])
         ctx.in_dtypes = [f.dtype for f in f0s]
-        return tuple(b.to(f.dtype) for b, f in zip(bufs, f0s)) + (J,)
+        # the outputs are copies where they are kept states
+        return tuple(b.to(f.dtype, copy=bool(ctx.kept)) for b, f in zip(bufs, f0s)) + (J,)
 
     @staticmethod
     def backward(ctx, *grads):
@@ -788,7 +817,7 @@ class _MultiLevelSteps(torch.autograd.Function):
 
     @staticmethod
     def _backward(ctx, gbufs, gJ):
-        steppers, cells, steps, K, even0, observe, observe_from, graphs = ctx.meta
+        steppers, cells, steps, K, even0, observe, observe_from, graphs, _ = ctx.meta
         n = len(steppers)
         lats = [s.L for s in steppers]
         dev = lats[0].device
@@ -807,11 +836,14 @@ class _MultiLevelSteps(torch.autograd.Function):
         if gJ is not None and observe is None:
             gJ = None
         for start, even_start, bufs0 in reversed(ctx.checkpoints):
-            records, states, even = [], bufs0, even_start
-            for _ in range(min(K, steps - start)):
-                rec, states = _level_step_record(steppers, cells, states, 0, even, stream, post_lists)
-                records.append(rec)
-                even = not even
+            if start in ctx.kept:
+                records, states = ctx.kept.pop(start), None
+            else:
+                records, states, even = [], bufs0, even_start
+                for _ in range(min(K, steps - start)):
+                    rec, states = _level_step_record(steppers, cells, states, 0, even, stream, post_lists)
+                    records.append(rec)
+                    even = not even
             for j in reversed(range(len(records))):
                 rec = records[j]
                 if gJ is not None and start + j >= observe_from:
@@ -820,6 +852,7 @@ class _MultiLevelSteps(torch.autograd.Function):
                 _level_adjoint(steppers, cells, sources, rec, lams, grads, 0, stream)
             del records, states
         ctx.checkpoints = None
+        ctx.kept = None
         ubar, rbar = grads[0][0], grads[0][1]
         return ((None,) + tuple(lam.to(dt) for lam, dt in zip(lams, ctx.in_dtypes))
                 + (ubar.view(3, -1) if ubar is not None else None, rbar) + tuple(g[2] for g in grads))
@@ -827,7 +860,8 @@ class _MultiLevelSteps(torch.autograd.Function):
 
 def k17_multilevel_steps(f0s, lattices, interfaces, steps: int, checkpoint_every: int = 10, even: bool | None = None,
                          bc_factories=None, velocity: torch.Tensor | None = None, rho_bc: torch.Tensor | None = None,
-                         porosity=None, observe=None, observe_from: int = 0, graphs: bool = True):
+                         porosity=None, observe=None, observe_from: int = 0, graphs: bool = True,
+                         keep_bytes: int = 0):
     """`steps` coarse steps of a hierarchy of any number of levels (level 0 coarsest) from the
     buffers f0s, differentiable in every buffer and, when given, in the coarsest level's velocity BC
     values (3, n) and outlet densities and in each level's porosity (porosity: one tensor or None
@@ -842,14 +876,17 @@ def k17_multilevel_steps(f0s, lattices, interfaces, steps: int, checkpoint_every
     states, and walks it in reverse: per step of a level, the interface adjoints, the level's step
     adjoint, then its finer level's two steps. For two levels it computes what k17_two_level_steps
     computes. f0s hold each level's storage type or FP32; observe / observe_from as in
-    k17_two_level_steps, on the coarsest level. Returns (list of final buffers, observed sum)."""
+    k17_two_level_steps, on the coarsest level; keep_bytes as in k17_two_level_steps (a kept coarse
+    step holds every level's records: per step of level l its state after the step, its BC lists'
+    post-collision values and, above the finest level, its state after the interface writes and a
+    copy of the finer level's). Returns (list of final buffers, observed sum)."""
     from .refinement import DeviceInterfaceCells
 
     n = len(lattices)
     if n < 2 or len(interfaces) != n - 1 or len(f0s) != n:
         raise ValueError("a hierarchy needs two or more levels, one buffer per level and one interface per level pair")
-    if steps < 0 or checkpoint_every < 1:
-        raise ValueError("steps must be >= 0 and checkpoint_every >= 1")
+    if steps < 0 or checkpoint_every < 1 or keep_bytes < 0:
+        raise ValueError("steps must be >= 0, checkpoint_every >= 1 and keep_bytes >= 0")
     if any(L.storage != lattices[0].storage for L in lattices):
         raise ValueError("every level must use the same storage type")
     pors = list(porosity) if porosity is not None else [None] * n
@@ -868,7 +905,10 @@ def k17_multilevel_steps(f0s, lattices, interfaces, steps: int, checkpoint_every
     cells = [(DeviceInterfaceCells(cf, L.device), DeviceInterfaceCells(fc, L.device))
              for (cf, fc), L in zip(interfaces, lattices)]
     even0 = lattices[0].even if even is None else bool(even)
-    meta = (steppers, cells, steps, checkpoint_every, even0, observe, observe_from, graphs)
+    inputs = list(f0s) + [velocity, rho_bc] + pors
+    if not torch.is_grad_enabled() or not any(t is not None and t.requires_grad for t in inputs):
+        keep_bytes = 0  # no backward will run: nothing to keep
+    meta = (steppers, cells, steps, checkpoint_every, even0, observe, observe_from, graphs, int(keep_bytes))
     out = _MultiLevelSteps.apply(meta, *f0s, velocity, rho_bc, *pors)
     return list(out[:n]), out[n]
 
