@@ -104,7 +104,9 @@ def post_launch_groups(lattice: Lattice, lists):
     links, and a link writes a cell of the node across the wall, which no node of either list
     reads, so every value is what the two launches leave, and the adjoint keeps one thread per
     node as well. A short launch costs a few microseconds of latency whatever its size; the
-    reference launches the lists one by one (UpdateGrid27.cpp:69-77)."""
+    reference launches the lists one by one (UpdateGrid27.cpp:69-77). A joined list is a copy of
+    the grid's host lists made here (its own device arrays): callers build the groups once per
+    run, after the lists are final."""
     import numpy as np
 
     from .grid import BoundaryNodes
