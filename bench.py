@@ -464,9 +464,13 @@ def cpu_baseline(n=256, target_s=8.0):
     omega = taylor_green_omega(n, 0.05, 1600.0)
     fields = taylor_green_fields(grid, 0.05)
     out = {}
+    sample = None
     for dtype in (np.float64, np.float32):
         L = oracle.Lattice(grid.nbx, grid.nby, grid.nbz, dtype=dtype)
         L.init_eq(*fields)
+        if dtype == np.float64:
+            # node-major populations of the first 2^18 nodes, for the reference's own collision
+            sample = np.ascontiguousarray(L.f.reshape(27, -1)[:, :1 << 18].T)
         even = True
         oracle.collide_box(L.f, n, n, n, omega, even, limiters=(1e6, 1e6, 1e6))
         even = not even
@@ -481,10 +485,23 @@ def cpu_baseline(n=256, target_s=8.0):
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     threads = min(threads, os.cpu_count() or threads)
     f64, f32 = out["float64"], out["float32"]
-    return {"value": round(f64[0], 3), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"CPU oracle (C++ restatement of the reference's K17 step, OpenMP), Taylor-Green {n}^3 periodic: "
-                      f"FP64 build {f64[1]} steps in {f64[2]:.1f} s; FP32 build {f32[1]} steps in {f32[2]:.1f} s",
-            "fp32_build_MLUPS": round(f32[0], 3), "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+    rec = {"value": round(f64[0], 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+           "sample": f"CPU oracle (C++ restatement of the reference's K17 step, OpenMP), Taylor-Green {n}^3 periodic: "
+                     f"FP64 build {f64[1]} steps in {f64[2]:.1f} s; FP32 build {f32[1]} steps in {f32[2]:.1f} s",
+           "fp32_build_MLUPS": round(f32[0], 3), "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+    if oracle.ref("f64") is not None and sample is not None:
+        # the reference's own collision (runK17CompressibleNavierStokes compiled from its src/lbm
+        # headers into oracle/_ref, FP64, one thread) on the same workload's nodes, collision only
+        calls, t0 = 0, time.perf_counter()
+        while calls < 2 or time.perf_counter() - t0 < target_s / 2:
+            oracle.ref_k17_nodes(sample, omega, limiters=(1e6, 1e6, 1e6))
+            calls += 1
+        dt = time.perf_counter() - t0
+        rec["reference_collision"] = {
+            "value": round(sample.shape[0] * calls / dt / 1e6, 3), "unit": "MLUPS", "cores": 1, "kind": "reference",
+            "sample": f"the reference's K17 collision function (FP64 build of its own headers), {sample.shape[0]} "
+                      f"nodes of the {n}^3 Taylor-Green field x {calls} calls in {dt:.1f} s, one thread, no streaming"}
+    return rec
 
 
 def traffic_from_profile():
