@@ -562,9 +562,14 @@ static void launch_box_t(const vf_lbm_parameter& p, bool macro, hipStream_t s)
     if (try_launch_row<S, EVEN>(p, a, r, macro, nullptr, s)) return;
     const dim3 grid(plane_blocks(p.nx, p.ny), p.nz), bx(kBoxBlock);
     const FusedBB none{};
-    // XCD block ranges only where pinned: in place they gain nothing inside a multi-level step
-    // (258 x 130 x 130 FP16 107 against 104-109 us) and cost the sphere channel 59 -> 61 us
-    const bool xcd = p.kernelChoice.rowXcdRanges == 1;
+    // XCD block ranges where the working set streams and planes share cache lines at their ends
+    // (or where pinned): inside the two-level step they take the 258 x 130 x 130 FP16 coarse
+    // level's fetched bytes from 1.21 to 0.99 x algorithmic at the same time (107 against
+    // 104-109 us); on a cache-resident lattice (the sphere channel) they cost 59 -> 61 us
+    const uint64_t ws = p.workingSetBytes ? p.workingSetBytes : 27 * p.numberOfNodes * sizeof(S);
+    const bool xcd = p.kernelChoice.rowXcdRanges == 1 ||
+                     (p.kernelChoice.rowXcdRanges == 0 && ws > kCacheResidentBytes &&
+                      xcd_planes(p.nx, p.ny, sizeof(S), 0));
     if (mask && macro) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, true>), grid, dim3(bx), 0, s, f, a, r, none);
     else if (mask && xcd) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false, false, true>), grid, dim3(bx), 0, s, f, a, r, none);
     else if (mask) hipLaunchKernelGGL((k17_box_kernel<S, EVEN, true, false>), grid, dim3(bx), 0, s, f, a, r, none);
