@@ -9,7 +9,10 @@ Prints ONE JSON line: MLUPS, the kernel's HBM roofline (algorithmic bytes 2*27*4
 per node update, divided by the event-timed average launch duration), the FP16/BF16
 storage variants measured the same way, the adjoint step (kernel alone, and a whole
 checkpointed forward + backward through k17_steps) against the forward step, a measured
-copy bandwidth, and the CPU oracle timed on the host cores on a bounded sample.
+copy bandwidth, and the CPU oracle timed on the host cores on a bounded sample; beside it the
+full-size sphere-channel and two-level gradients, the reference's published cavity benchmark,
+and the other configurations' GPU rates with the CPU oracle on the same cases
+(`configs_gpu_cpu`: the 64^3 cavity, Taylor-Green 512^3 in FP32/FP16/BF16).
 
     python bench.py [--gpus 1] [--steps 2000] [--warmup 20]
 """
@@ -339,6 +342,89 @@ def published_cavity(steps=10000):
     return out
 
 
+def configs_gpu_cpu(cavity_steps=1000, n3=512, gpu_steps=100, cpu_seconds=5.0):
+    """The other configurations' throughput on the GPU with the CPU oracle (FP64 build, OpenMP
+    on the host cores) on the same case beside it, each CPU leg bounded to about cpu_seconds of
+    steps. Config 1: lid-driven cavity, 64^3 fluid nodes and the wall layer, lid 0.05 in x on the
+    z-max face, Re 100, NoSlipBounceBack + VelocityBounceBack, from rest, `cavity_steps` steps
+    (FP32 on the GPU through Simulation.run). Config 3: Taylor-Green n3^3 periodic, FP32, FP16 and
+    BF16 storage on the GPU (`gpu_steps` event-timed steps each); the CPU on a few steps."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from virtualfluids_amd.grid import (d0M0, d0P0, d00M, d00P, dM00, dP00, periodic_box, taylor_green_fields,
+                                        taylor_green_omega, walled_box)
+    from virtualfluids_amd.lattice import Lattice
+    from virtualfluids_amd.simulation import BoundaryConditionFactory, Simulation
+
+    out = {}
+    # config 1
+    n, u_lid = 64, 0.05
+    walls = {side: ("noslip",) for side in (dP00, dM00, d0P0, d0M0, d00M)}
+    walls[d00P] = ("velocity", (u_lid, 0.0, 0.0))
+    g = walled_box((n, n, n), walls)
+    omega = 1.0 / (3.0 * (u_lid * n / 100.0) + 0.5)
+    F = BoundaryConditionFactory
+    fac = F()
+    fac.set_velocity_boundary_condition(F.VelocityBC.VelocityBounceBack)
+    fac.set_no_slip_boundary_condition(F.NoSlipBC.NoSlipBounceBack)
+    L = Lattice(g, omega)
+    z = np.zeros(g.N)
+    L.init_equilibrium(z, z, z, z)
+    sim = Simulation(L, fac)
+    sim.run(20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.run(cavity_steps)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    del sim, L
+    C = oracle.Lattice(g.nbx, g.nby, g.nbz)
+    C.init_eq(z, z, z, z)
+    vel, ns = g.bcs["velocity"], g.bcs["noslip"]
+    steps, t0 = 0, time.perf_counter()
+    while steps < 2 or (time.perf_counter() - t0 < cpu_seconds and steps < cavity_steps):
+        C.collide(g.fluid, omega)
+        C.bc_post(1, vel.k, vel.q27, omega, vel.velocity)  # VelocityBounceBack
+        C.bc_post(0, ns.k, ns.q27, omega)                  # NoSlipBounceBack
+        C.swap()
+        steps += 1
+    t_cpu = time.perf_counter() - t0
+    del C
+    out["config1_cavity64"] = {
+        "case": f"lid-driven cavity 64^3 + wall layer ({g.N} nodes), lid 0.05, Re 100, bounce-back walls",
+        "gpu_MLUPS": round(g.N * cavity_steps / t_gpu / 1e6, 1), "gpu_steps": cavity_steps, "gpu_dtype": "f32",
+        "cpu_MLUPS": round(g.N * steps / t_cpu / 1e6, 2), "cpu_steps": steps, "cpu_dtype": "f64 (oracle)"}
+    # config 3
+    grid = periodic_box(n3, n3, n3)
+    fields = taylor_green_fields(grid, 0.05)
+    omega = taylor_green_omega(n3, 0.05, 1600.0)
+    rec = {"case": f"Taylor-Green {n3}^3 periodic, u0 0.05, Re 1600"}
+    for storage in ("fp32", "fp16", "bf16"):
+        L = Lattice(grid, omega, storage=storage, limiters=(1e6, 1e6, 1e6))
+        L.init_equilibrium(*fields)
+        dt = time_steps(L, gpu_steps, 10)
+        rec[f"gpu_{storage}_MLUPS"] = round(grid.N * gpu_steps / dt / 1e6, 1)
+        del L
+        torch.cuda.empty_cache()
+    rec["gpu_steps"] = gpu_steps
+    C = oracle.Lattice(grid.nbx, grid.nby, grid.nbz)
+    C.init_eq(*fields)
+    even, steps, t0 = True, 0, time.perf_counter()
+    while steps < 1 or (time.perf_counter() - t0 < cpu_seconds and steps < 20):
+        oracle.collide_box(C.f, n3, n3, n3, omega, even, limiters=(1e6, 1e6, 1e6))
+        even = not even
+        steps += 1
+    t_cpu = time.perf_counter() - t0
+    del C
+    rec["cpu_MLUPS"] = round(grid.N * steps / t_cpu / 1e6, 2)
+    rec["cpu_steps"] = steps
+    rec["cpu_dtype"] = "f64 (oracle)"
+    out["config3_tgv"] = rec
+    return out
+
+
 def copy_bandwidth():
     """Device-to-device copy of 2 GiB, (read + write bytes) / time, by two kernels: the
     library's aligned non-temporal 16-byte copy (vf_stream_copy) and the runtime's blit copy
@@ -646,6 +732,8 @@ def run_case(case, steps=None):
         return out
     if case == "interface":
         return interface_kernels()
+    if case == "configs":
+        return configs_gpu_cpu()
     return published_cavity() if steps is None else published_cavity(steps)
 
 
@@ -665,7 +753,7 @@ def main():
     ap.add_argument("--no-cavity", action="store_true", help="skip the reference's published cavity benchmark")
     ap.add_argument("--only", choices=["fp32", "fp16", "bf16"], default=None,
                     help="time one storage type only (for profiling runs)")
-    ap.add_argument("--only-case", choices=["sphere", "two-level", "cavity", "interface"], default=None,
+    ap.add_argument("--only-case", choices=["sphere", "two-level", "cavity", "interface", "configs"], default=None,
                     help="run one full-size case alone and print its record (for profiling runs)")
     ap.add_argument("--case-steps", type=int, default=None,
                     help="steps of --only-case (default: the case's full length)")
@@ -790,6 +878,7 @@ def main():
         "sphere_channel_gradient": None if (args.no_sphere or args.no_adjoint) else sphere_channel_gradient(),
         "two_level_gradient": two_level,
         "published_cavity": None if args.no_cavity else published_cavity(),
+        "configs_gpu_cpu": None if (args.no_cpu or args.only is not None) else configs_gpu_cpu(),
         "hbm_copy_GBps": round(copy_gbps, 1),
         "blit_copy_GBps": round(blit_gbps, 1),
         "inplace27": rmw,
